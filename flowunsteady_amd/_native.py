@@ -1,0 +1,156 @@
+"""ctypes binding of libvpmx.so (the C ABI declared in include/vpmx.h).
+
+The argument lists here are exactly what a Julia `ccall` passes: plain
+pointers, 64-bit counts and the two small parameter structs.  There is no
+fallback: if the built library is missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvpmx.so")
+
+# status codes
+OK, EINVAL, ENOMEM, EHIP, ENODEV, EINJECT, EUNSUPPORTED = 0, 1, 2, 3, 4, 5, 6
+# kernels
+SINGULAR, GAUSSIAN, GAUSSIANERF, WINCKELMANS = 0, 1, 2, 3
+# precision
+FP64, MIXED = 0, 1
+# relaxation
+RELAX_NONE, RELAX_PEDRIZZETTI, RELAX_CORRECTED = 0, 1, 2
+# SFS
+SFS_NONE, SFS_CONSTANT = 0, 1
+# timing classes
+T_UJ, T_ESTR, T_UPDATE = 0, 1, 2
+
+
+class Layout(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int32), ("X", ctypes.c_int32), ("Gamma", ctypes.c_int32),
+                ("sigma", ctypes.c_int32), ("vol", ctypes.c_int32), ("U", ctypes.c_int32),
+                ("J", ctypes.c_int32), ("SFS", ctypes.c_int32), ("C", ctypes.c_int32),
+                ("static_", ctypes.c_int32)]
+
+
+class UJParams(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("transposed", ctypes.c_int32), ("reset", ctypes.c_int32),
+                ("sfs", ctypes.c_int32), ("fmm", ctypes.c_int32), ("p", ctypes.c_int32),
+                ("ncrit", ctypes.c_int32), ("theta", ctypes.c_double)]
+
+
+class StepParams(ctypes.Structure):
+    _fields_ = [("f", ctypes.c_double), ("g", ctypes.c_double), ("Uinf", ctypes.c_double * 3),
+                ("sfs", ctypes.c_int32), ("Cs", ctypes.c_double),
+                ("clip_backscatter", ctypes.c_int32), ("relax_kind", ctypes.c_int32),
+                ("rlxf", ctypes.c_double)]
+
+
+class VPMXError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"vpmx error {code}: {msg}")
+        self.code = code
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ctxp = ctypes.c_void_p
+
+# name -> (restype, argtypes); every function the header declares
+SIGNATURES = {
+    "vpmx_create": (ctypes.c_int, [ctypes.POINTER(_ctxp), ctypes.c_int32]),
+    "vpmx_destroy": (None, [_ctxp]),
+    "vpmx_last_error": (ctypes.c_char_p, [_ctxp]),
+    "vpmx_version": (ctypes.c_char_p, []),
+    "vpmx_uj": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64, ctypes.POINTER(Layout),
+                               ctypes.POINTER(UJParams)]),
+    "vpmx_rk3_step": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64, ctypes.POINTER(Layout),
+                                     ctypes.POINTER(UJParams), ctypes.POINTER(StepParams),
+                                     ctypes.c_double, ctypes.c_int32]),
+    "vpmx_euler_step": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64, ctypes.POINTER(Layout),
+                                       ctypes.POINTER(UJParams), ctypes.POINTER(StepParams),
+                                       ctypes.c_double, ctypes.c_int32]),
+    "vpmx_upload": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64, ctypes.POINTER(Layout)]),
+    "vpmx_download": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64, ctypes.POINTER(Layout)]),
+    "vpmx_uj_resident": (ctypes.c_int, [_ctxp, ctypes.POINTER(UJParams)]),
+    "vpmx_rk3_resident": (ctypes.c_int, [_ctxp, ctypes.POINTER(UJParams),
+                                         ctypes.POINTER(StepParams), ctypes.c_double,
+                                         ctypes.c_int32]),
+    "vpmx_euler_resident": (ctypes.c_int, [_ctxp, ctypes.POINTER(UJParams),
+                                           ctypes.POINTER(StepParams), ctypes.c_double,
+                                           ctypes.c_int32]),
+    "vpmx_diagnostics": (ctypes.c_int, [_ctxp, _dp, _dp]),
+    "vpmx_resident_np": (ctypes.c_int64, [_ctxp]),
+    "vpmx_set_timing": (ctypes.c_int, [_ctxp, ctypes.c_int32]),
+    "vpmx_get_timing": (ctypes.c_int, [_ctxp, ctypes.c_int32, _dp,
+                                       ctypes.POINTER(ctypes.c_int64)]),
+    "vpmx_stream": (ctypes.c_void_p, [_ctxp]),
+    "vpmx_inject_fault": (ctypes.c_int, [_ctxp, ctypes.c_int32]),
+    "vpmx_radial_f32": (ctypes.c_int, [ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                       ctypes.POINTER(ctypes.c_float),
+                                       ctypes.POINTER(ctypes.c_float)]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libvpmx.so; raise if it has not been built (no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with "
+                              "`python -m flowunsteady_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error(ctx=None) -> str:
+    m = lib().vpmx_last_error(ctx)
+    return m.decode() if m else ""
+
+
+def check(code: int, ctx=None):
+    if code != OK:
+        raise VPMXError(code, last_error(ctx))
+
+
+class Context:
+    """Owns one vpmx_ctx (device buffers, stream, timers)."""
+
+    def __init__(self, device: int = 0):
+        p = _ctxp()
+        check(lib().vpmx_create(ctypes.byref(p), int(device)))
+        self.ptr = p
+        self.device = device
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().vpmx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, code):
+        check(code, self.ptr)
+
+    def set_timing(self, on: bool):
+        self.check(lib().vpmx_set_timing(self.ptr, int(on)))
+
+    def timing(self, which: int):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self.check(lib().vpmx_get_timing(self.ptr, which, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def stream(self) -> int:
+        return lib().vpmx_stream(self.ptr)

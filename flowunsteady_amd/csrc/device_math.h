@@ -1,0 +1,255 @@
+// device_math.h -- radial kernel functions of the regularised Biot-Savart law.
+//
+// Kernels (docs/src/theory/rvpm.md:84,89-99,365,380), rho = r/sigma:
+//   gaussianerf  zeta = (2 pi)^-3/2 e^{-rho^2/2}
+//                g    = erf(rho/sqrt2) - sqrt(2/pi) rho e^{-rho^2/2}
+//                g'   = sqrt(2/pi) rho^2 e^{-rho^2/2}
+//   winckelmans  zeta = 15/(8 pi) (rho^2+1)^{-7/2}
+//                g    = rho^3 (rho^2 + 5/2) (rho^2+1)^{-5/2}
+//                g'   = 15/2 rho^2 (rho^2+1)^{-7/2}
+//   gaussian     zeta = 3/(4 pi) e^{-rho^3},  g = 1 - e^{-rho^3},  g' = 3 rho^2 e^{-rho^3}
+//   singular     g = 1, g' = 0
+//
+// A pair (target i, source j, dX = x_i - x_j, r = |dX|) contributes
+//   U_k    += -1/(4 pi) A c_k                        c = dX x Gamma_j
+//   J[k,l] += -1/(4 pi) ( B c_k dX_l + A eps_klm Gamma_j,m )
+// with the two radial coefficients
+//   A = g / r^3,     B = g'/(sigma r^4) - 3 g / r^5,
+// the same algebra as src/FLOWUnsteady_processing_force.jl:895-905 extended to
+// J = grad U.  Both coefficients are finite as r -> 0 (A -> g/rho^3 sigma^-3).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vpmx {
+
+enum Kernel : int { SINGULAR = 0, GAUSSIAN = 1, GAUSSIANERF = 2, WINCKELMANS = 3 };
+
+constexpr double INV4PI = 0.07957747154594767;   // 1/(4 pi), processing_force.jl:900
+constexpr double SQRT2_PI = 0.7978845608028654;  // sqrt(2/pi)
+constexpr double INV_SQRT2 = 0.7071067811865476;
+
+// ---------------------------------------------------------------------------
+// FP64, reference form: g and g' exactly as the closed forms above (libm-style
+// erf/exp from the ROCm device math library), A and B assembled as in the
+// reference pair loop.  This is the correctness path.
+template <int K>
+__device__ __forceinline__ void g_dg_f64(double rho, double &g, double &dg)
+{
+    if constexpr (K == GAUSSIANERF) {
+        double e = exp(-0.5 * rho * rho);
+        g = erf(rho * INV_SQRT2) - SQRT2_PI * rho * e;
+        dg = SQRT2_PI * rho * rho * e;
+    } else if constexpr (K == GAUSSIAN) {
+        double e = exp(-rho * rho * rho);
+        g = 1.0 - e;
+        dg = 3.0 * rho * rho * e;
+    } else if constexpr (K == WINCKELMANS) {
+        double q = rho * rho + 1.0;
+        double sq = sqrt(q);
+        double q52 = q * q * sq;
+        g = rho * rho * rho * (rho * rho + 2.5) / q52;
+        dg = 7.5 * rho * rho / (q52 * q);
+    } else {
+        g = 1.0;
+        dg = 0.0;
+    }
+}
+
+template <int K>
+__device__ __forceinline__ double zeta_f64(double rho)
+{
+    if constexpr (K == GAUSSIANERF) {
+        return 0.06349363593424097 * exp(-0.5 * rho * rho);   // (2 pi)^-3/2
+    } else if constexpr (K == GAUSSIAN) {
+        return 0.238732414637843 * exp(-rho * rho * rho);     // 3/(4 pi)
+    } else if constexpr (K == WINCKELMANS) {
+        double q = rho * rho + 1.0;
+        double sq = sqrt(q);
+        return 0.5968310365946075 / (q * q * q * sq);          // 15/(8 pi)
+    } else {
+        return 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 forms used by the mixed-precision kernels.
+//
+// gaussianerf, rho >= 1.5: erfc(x) = t P(t) e^{-x^2}, t = 1/(1 + p x), the
+//   five-term rational approximation 7.1.26 of Abramowitz & Stegun (1964),
+//   |error| <= 1.5e-7.  With x = rho/sqrt2 its e^{-x^2} is the same
+//   e^{-rho^2/2} that g' needs, so one hardware exp serves both:
+//   g = 1 - e (t P(t) + sqrt(2/pi) rho).
+// gaussianerf, rho < 1.5: Taylor series of g/rho^3 and of
+//   sigma^5 B = (g'/rho^2 - 3 g/rho^3)/rho^2 obtained by integrating the series
+//   of g' = sqrt(2/pi) rho^2 e^{-rho^2/2} term by term:
+//   g/rho^3      = sqrt(2/pi) sum_n (-1)^n rho^{2n} / (2^n n! (2n+3))
+//   sigma^5 B    = sqrt(2/pi) sum_{n>=1} (-1)^n 2n rho^{2n-2} / (2^n n! (2n+3))
+//   which avoid the cancellation of the closed form as rho -> 0.
+// rho^2 >= 49 for every lane of a wave: g = 1 and g' = 0 to FP32 rounding
+//   (e^{-24.5} rho^3 < 1e-8), so A = 1/r^3, B = -3/r^5 without exp.
+// winckelmans (any rho): with s^2 = r^2 + sigma^2,
+//   A = (r^2 + 5/2 sigma^2) s^-5,   B = -3 (r^2 + 7/2 sigma^2) s^-7,
+//   an exact rewrite of the closed forms that needs one rsqrt.
+
+constexpr float RHO2_FAR = 49.0f;      // far-field fast path threshold (rho >= 7)
+constexpr float RHO2_SERIES = 2.25f;   // series below rho = 1.5
+constexpr float RHO3_SERIES_G = 0.5f;  // gaussian kernel: series below rho^3 = 0.5
+
+// Hardware transcendentals on the GPU (v_exp_f32, v_rcp_f32, v_rsq_f32, each
+// within 1 ulp); the host build of these formulas (used only by the CPU unit
+// test of their accuracy) takes the C library equivalents.
+#define VPMX_HD __host__ __device__ __forceinline__
+VPMX_HD float fexp2(float x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_exp2f(x);
+#else
+    return exp2f(x);
+#endif
+}
+VPMX_HD float frcp(float x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+VPMX_HD float frsq(float x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+
+// Series coefficients, evaluated at compile time from their closed forms:
+//   cg(n) = sqrt(2/pi) (-1)^n / (2^n n! (2n+3))         g/rho^3 = sum_n cg(n) x^n
+//   cb(n) = sqrt(2/pi) (-1)^n 2n / (2^n n! (2n+3))      sigma^5 B = sum_{n>=1} cb(n) x^{n-1}
+// with x = rho^2.  Twelve terms leave a truncation error below 2e-9 relative
+// at x = 2.25, the top of the series range.
+constexpr int NSERIES = 12;
+
+constexpr double series_den(int n)   // 2^n n!
+{
+    double d = 1.0;
+    for (int k = 1; k <= n; ++k) d *= 2.0 * k;
+    return d;
+}
+constexpr float cg(int n)
+{
+    return (float)((n % 2 ? -1.0 : 1.0) * SQRT2_PI / (series_den(n) * (2.0 * n + 3.0)));
+}
+constexpr float cb(int n)
+{
+    return (float)((n % 2 ? -1.0 : 1.0) * SQRT2_PI * 2.0 * n / (series_den(n) * (2.0 * n + 3.0)));
+}
+
+VPMX_HD float series_hA(float x)  // g/rho^3
+{
+    float h = cg(NSERIES - 1);
+#pragma unroll
+    for (int n = NSERIES - 2; n >= 0; --n) h = fmaf(h, x, cg(n));
+    return h;
+}
+
+VPMX_HD float series_hB(float x)  // sigma^5 B
+{
+    float h = cb(NSERIES);
+#pragma unroll
+    for (int n = NSERIES - 1; n >= 1; --n) h = fmaf(h, x, cb(n));
+    return h;
+}
+
+// Gaussian kernel (zeta = 3/(4 pi) e^{-rho^3}), u = rho^3 < 1/2: from the series
+// of e^{-u},
+//   g/rho^3   = sum_{n>=0} (-1)^n u^n / (n+1)!
+//   sigma^5 B = 3 rho sum_{n>=1} (-1)^n n u^{n-1} / (n+1)!
+constexpr int NSERIES_G = 9;
+constexpr double fact(int n)
+{
+    double d = 1.0;
+    for (int k = 2; k <= n; ++k) d *= k;
+    return d;
+}
+constexpr float gg(int n) { return (float)((n % 2 ? -1.0 : 1.0) / fact(n + 1)); }
+constexpr float gb(int n) { return (float)((n % 2 ? -1.0 : 1.0) * 3.0 * n / fact(n + 1)); }
+
+VPMX_HD float series_gauss_hA(float u)
+{
+    float h = gg(NSERIES_G - 1);
+#pragma unroll
+    for (int n = NSERIES_G - 2; n >= 0; --n) h = fmaf(h, u, gg(n));
+    return h;
+}
+VPMX_HD float series_gauss_hB(float u, float rho)
+{
+    float h = gb(NSERIES_G);
+#pragma unroll
+    for (int n = NSERIES_G - 1; n >= 1; --n) h = fmaf(h, u, gb(n));
+    return h * rho;
+}
+
+// Abramowitz & Stegun 7.1.26 constants (Hastings' approximation).
+constexpr float AS_P = 0.3275911f;
+constexpr float AS_A1 = 0.254829592f, AS_A2 = -0.284496736f, AS_A3 = 1.421413741f,
+                AS_A4 = -1.453152027f, AS_A5 = 1.061405429f;
+constexpr float LOG2E = 1.4426950408889634f;
+
+// Radial coefficients (A, B) of one pair in FP32, full evaluation (near field).
+// r2 > 0 is guaranteed by the caller's mask (rinv = 0 for coincident pairs);
+// p0, p1 are the per-source constants: gaussian(-erf) p0 = 1/sigma,
+// p1 = 1/sigma^2; winckelmans p0 = sigma^2.
+template <int K>
+VPMX_HD void radial_near_f32(float r2, float rinv, float p0, float p1,
+                                                float &A, float &B)
+{
+    if constexpr (K == GAUSSIANERF) {
+        float rho2 = r2 * p1;
+        if (rho2 < RHO2_SERIES) {
+            float is3 = p1 * p0, is5 = is3 * p1;
+            A = series_hA(rho2) * is3;
+            B = series_hB(rho2) * is5;
+        } else {
+            float rinv2 = rinv * rinv, rinv3 = rinv2 * rinv;
+            float rho = r2 * rinv * p0;
+            float e = fexp2(-0.5f * LOG2E * rho2);
+            float t = frcp(fmaf(AS_P * (float)INV_SQRT2, rho, 1.0f));
+            float P = fmaf(fmaf(fmaf(fmaf(AS_A5, t, AS_A4), t, AS_A3), t, AS_A2), t, AS_A1) * t;
+            float g = fmaf(-e, fmaf((float)SQRT2_PI, rho, P), 1.0f);
+            float dg = (float)SQRT2_PI * rho2 * e;
+            A = g * rinv3;
+            B = (dg * p0 * rinv - 3.0f * g * rinv2) * rinv3;
+        }
+    } else if constexpr (K == GAUSSIAN) {
+        float rho = r2 * rinv * p0;
+        float u = rho * rho * rho;
+        if (u < RHO3_SERIES_G) {
+            float is3 = p1 * p0, is5 = is3 * p1;
+            A = series_gauss_hA(u) * is3;
+            B = series_gauss_hB(u, rho) * is5;
+        } else {
+            float rinv2 = rinv * rinv, rinv3 = rinv2 * rinv;
+            float e = fexp2(-LOG2E * u);
+            float g = 1.0f - e;
+            float dg = 3.0f * rho * rho * e;
+            A = g * rinv3;
+            B = (dg * p0 * rinv - 3.0f * g * rinv2) * rinv3;
+        }
+    } else if constexpr (K == WINCKELMANS) {
+        float s2 = r2 + p0;
+        float si = frsq(s2);
+        float si2 = si * si;
+        float si5 = si2 * si2 * si;
+        A = fmaf(2.5f, p0, r2) * si5;
+        B = -3.0f * fmaf(3.5f, p0, r2) * si5 * si2;
+    } else {
+        float rinv2 = rinv * rinv;
+        A = rinv2 * rinv;
+        B = -3.0f * A * rinv2;
+    }
+}
+
+}  // namespace vpmx

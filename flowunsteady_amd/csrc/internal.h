@@ -1,0 +1,88 @@
+// internal.h -- context and launcher declarations shared by the library's
+// translation units (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "vpmx.h"
+
+namespace vpmx {
+
+// Timed kernel classes (vpmx_get_timing's `which`).
+enum TimedKind { T_UJ = 0, T_ESTR = 1, T_UPDATE = 2, T_NKIND = 3 };
+
+struct Timer {
+    bool on = false;
+    double total_ms[T_NKIND] = {0, 0, 0};
+    int64_t count[T_NKIND] = {0, 0, 0};
+    struct Pending { int kind; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+};
+
+// Device state of one particle field, SoA FP64 with stride `cap`:
+// component k of particle i of a vector field F lives at F[k*cap + i].
+struct DeviceField {
+    int64_t np = 0, cap = 0;
+    double *X = nullptr, *G = nullptr, *sigma = nullptr, *U = nullptr, *J = nullptr;
+    double *E = nullptr, *C = nullptr, *M = nullptr;
+    uint8_t *is_static = nullptr;
+    bool any_static = false;
+};
+
+// Scratch of the pairwise passes, grown with the field.
+struct Scratch {
+    int64_t cap = 0;
+    double4 *src64 = nullptr;        // FP64 UJ source records, 2 per particle
+    double4 *srcE = nullptr;         // FP64 Estr source records, 5 per particle
+    double4 *spos = nullptr;         // Morton-sorted (x, y, z, p0) FP64
+    float4 *sgam = nullptr;          // Morton-sorted (Gx, Gy, Gz, p1) FP32
+    uint32_t *kin = nullptr, *kout = nullptr;
+    int32_t *vin = nullptr, *perm = nullptr;
+    uint8_t *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    double *part = nullptr;          // per-block partials (bbox, reductions)
+    int64_t part_cap = 0;
+};
+
+}  // namespace vpmx
+
+struct vpmx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int inject = 0;
+    vpmx::DeviceField f;
+    vpmx::Scratch s;
+    vpmx::Timer timer;
+    double *h_stage = nullptr;       // pinned host staging
+    size_t h_stage_bytes = 0;
+};
+
+namespace vpmx {
+
+// Launchers (kernels.hip).  All enqueue on `st` and return hipSuccess or the
+// launch error.
+hipError_t launch_pack_f64(hipStream_t st, const DeviceField &f, Scratch &s);
+hipError_t launch_uj_f64(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
+                         int reset);
+hipError_t launch_sort_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, Scratch &s);
+hipError_t launch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
+                           int reset);
+hipError_t launch_estr_f64(hipStream_t st, int kernel, int transposed, const DeviceField &f,
+                           Scratch &s);
+hipError_t launch_sfs_coeff(hipStream_t st, const DeviceField &f, double Cs, int clip);
+hipError_t launch_fill(hipStream_t st, double *p, int64_t n, double v);
+hipError_t launch_update(hipStream_t st, const DeviceField &f, int transposed, int sfs,
+                         double fcoef, double gcoef, double zeta0, const double Uinf[3],
+                         double a, double b, double dt);
+hipError_t launch_relax(hipStream_t st, const DeviceField &f, int kind, double rlxf);
+hipError_t launch_diagnostics(hipStream_t st, const DeviceField &f, Scratch &s, double *out2);
+size_t sort_temp_bytes(int64_t n);
+
+double zeta0_of(int kernel);
+
+}  // namespace vpmx

@@ -1,0 +1,784 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the rVPM particle-interaction step.
+//
+//   uj_f64_kernel      all-pairs U and J in FP64, reference pair algebra
+//                      (src/FLOWUnsteady_processing_force.jl:888-905 extended to
+//                      J = grad U); LDS-tiled sources, one target per lane.
+//   uj_mixed_kernel    all-pairs U and J with FP32 pair math on Morton-sorted
+//                      targets, coordinates relative to the block's first
+//                      target, FP32 tile partials folded into FP64 per tile.
+//   estr_f64_kernel    SFS stretching Estr (docs/src/theory/rvpm.md:251-262).
+//   update_kernel      fused low-storage RK / Euler substep of the rVPM
+//                      equations (rvpm.md:107-178,196-236,364) in FP64.
+//   relax_kernel       Pedrizzetti relaxation (rvpm.md:363).
+//   diag kernels       enstrophy 0.5 sum Gamma.omega (src/FLOWUnsteady_monitors.jl:573)
+//                      and energy sum U.(x x Gamma).
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "device_math.h"
+#include "internal.h"
+
+namespace vpmx {
+
+constexpr int UJ_BLOCK = 256;
+constexpr int EW_BLOCK = 256;      // elementwise kernels
+constexpr int RED_BLOCKS = 1024;   // partial-sum blocks of the reductions
+
+static inline unsigned grid_for(int64_t n, int block)
+{
+    return (unsigned)((n + block - 1) / block);
+}
+
+// ===========================================================================
+// FP64 path
+// ===========================================================================
+
+// Source record (x, y, z, Gx) (Gy, Gz, sigma, 1/sigma).
+__global__ __launch_bounds__(EW_BLOCK) void pack_f64_kernel(const double *__restrict__ X,
+                                                            const double *__restrict__ G,
+                                                            const double *__restrict__ sg,
+                                                            int64_t n, int64_t cap,
+                                                            double4 *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double s = sg[i];
+    out[2 * i] = make_double4(X[i], X[cap + i], X[2 * cap + i], G[i]);
+    out[2 * i + 1] = make_double4(G[cap + i], G[2 * cap + i], s, 1.0 / s);
+}
+
+template <int K>
+__global__ __launch_bounds__(UJ_BLOCK) void uj_f64_kernel(const double4 *__restrict__ src,
+                                                          int64_t ns,
+                                                          const double *__restrict__ X,
+                                                          int64_t nt, int64_t cap,
+                                                          double *__restrict__ U,
+                                                          double *__restrict__ J, int reset)
+{
+    __shared__ double4 tile[2 * UJ_BLOCK];
+    const int t = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * UJ_BLOCK + t;
+    const bool active = i < nt;
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    if (active) {
+        xi = X[i];
+        yi = X[cap + i];
+        zi = X[2 * cap + i];
+    }
+    double u0 = 0, u1 = 0, u2 = 0;
+    double j0 = 0, j1 = 0, j2 = 0, j3 = 0, j4 = 0, j5 = 0, j6 = 0, j7 = 0, j8 = 0;
+
+    for (int64_t base = 0; base < ns; base += UJ_BLOCK) {
+        __syncthreads();
+        const int64_t js = base + t;
+        if (js < ns) {
+            tile[2 * t] = src[2 * js];
+            tile[2 * t + 1] = src[2 * js + 1];
+        }
+        __syncthreads();
+        const int cnt = (int)((ns - base) < UJ_BLOCK ? (ns - base) : UJ_BLOCK);
+        for (int jj = 0; jj < cnt; ++jj) {
+            const double4 a = tile[2 * jj], b = tile[2 * jj + 1];
+            const double dX1 = xi - a.x, dX2 = yi - a.y, dX3 = zi - a.z;
+            const double r2 = dX1 * dX1 + dX2 * dX2 + dX3 * dX3;
+            if (r2 > 0.0) {   // the reference skips r == 0
+                const double r = sqrt(r2);
+                double g, dg;
+                g_dg_f64<K>(r * b.w, g, dg);
+                const double rinv = 1.0 / r;
+                const double r3i = rinv * rinv * rinv;
+                const double G1 = a.w, G2 = b.x, G3 = b.y;
+                const double c1 = -INV4PI * r3i * (dX2 * G3 - dX3 * G2);
+                const double c2 = -INV4PI * r3i * (dX3 * G1 - dX1 * G3);
+                const double c3 = -INV4PI * r3i * (dX1 * G2 - dX2 * G1);
+                u0 += g * c1;
+                u1 += g * c2;
+                u2 += g * c3;
+                const double aux = dg * b.w * rinv - 3.0 * g * rinv * rinv;
+                j0 += aux * c1 * dX1; j1 += aux * c2 * dX1; j2 += aux * c3 * dX1;
+                j3 += aux * c1 * dX2; j4 += aux * c2 * dX2; j5 += aux * c3 * dX2;
+                j6 += aux * c1 * dX3; j7 += aux * c2 * dX3; j8 += aux * c3 * dX3;
+                const double a2 = -INV4PI * g * r3i;
+                j1 -= a2 * G3;
+                j2 += a2 * G2;
+                j3 += a2 * G3;
+                j5 -= a2 * G1;
+                j6 -= a2 * G2;
+                j7 += a2 * G1;
+            }
+        }
+    }
+    if (!active) return;
+    if (!reset) {
+        u0 += U[i]; u1 += U[cap + i]; u2 += U[2 * cap + i];
+        j0 += J[i]; j1 += J[cap + i]; j2 += J[2 * cap + i];
+        j3 += J[3 * cap + i]; j4 += J[4 * cap + i]; j5 += J[5 * cap + i];
+        j6 += J[6 * cap + i]; j7 += J[7 * cap + i]; j8 += J[8 * cap + i];
+    }
+    U[i] = u0; U[cap + i] = u1; U[2 * cap + i] = u2;
+    J[i] = j0; J[cap + i] = j1; J[2 * cap + i] = j2;
+    J[3 * cap + i] = j3; J[4 * cap + i] = j4; J[5 * cap + i] = j5;
+    J[6 * cap + i] = j6; J[7 * cap + i] = j7; J[8 * cap + i] = j8;
+}
+
+hipError_t launch_pack_f64(hipStream_t st, const DeviceField &f, Scratch &s)
+{
+    if (f.np == 0) return hipSuccess;
+    pack_f64_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(f.X, f.G, f.sigma, f.np, f.cap,
+                                                                   s.src64);
+    return hipGetLastError();
+}
+
+hipError_t launch_uj_f64(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
+                         int reset)
+{
+    if (f.np == 0) return hipSuccess;
+    const unsigned g = grid_for(f.np, UJ_BLOCK);
+    switch (kernel) {
+    case SINGULAR:
+        uj_f64_kernel<SINGULAR><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset);
+        break;
+    case GAUSSIAN:
+        uj_f64_kernel<GAUSSIAN><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset);
+        break;
+    case GAUSSIANERF:
+        uj_f64_kernel<GAUSSIANERF><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset);
+        break;
+    case WINCKELMANS:
+        uj_f64_kernel<WINCKELMANS><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// Mixed path: Morton sort, packing, FP32 pair kernel
+// ===========================================================================
+
+__global__ __launch_bounds__(EW_BLOCK) void bbox_partial_kernel(const double *__restrict__ X,
+                                                                int64_t n, int64_t cap,
+                                                                double *__restrict__ part)
+{
+    __shared__ double red[6][EW_BLOCK];
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * EW_BLOCK) {
+        for (int k = 0; k < 3; ++k) {
+            const double v = X[k * cap + i];
+            lo[k] = fmin(lo[k], v);
+            hi[k] = fmax(hi[k], v);
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        red[k][threadIdx.x] = lo[k];
+        red[3 + k][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    for (int w = EW_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 3; ++k) {
+                red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+                red[3 + k][threadIdx.x] = fmax(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Reduces the bbox partials (every block redoes the same fixed-order reduction
+// of `nparts` partials, which is cheap) and writes 30-bit Morton keys.
+__global__ __launch_bounds__(EW_BLOCK) void morton_kernel(const double *__restrict__ X,
+                                                          int64_t n, int64_t cap,
+                                                          const double *__restrict__ part,
+                                                          int nparts, uint32_t *__restrict__ key,
+                                                          int32_t *__restrict__ val)
+{
+    __shared__ double box[6];
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        double v = part[k];
+        for (int b = 1; b < nparts; ++b)
+            v = (k < 3) ? fmin(v, part[6 * b + k]) : fmax(v, part[6 * b + k]);
+        box[k] = v;
+    }
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double ext = fmax(fmax(box[3] - box[0], box[4] - box[1]), box[5] - box[2]);
+    const double scale = ext > 0.0 ? 1023.999 / ext : 0.0;
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        double v = (X[k * cap + i] - box[k]) * scale;
+        v = fmin(fmax(v, 0.0), 1023.0);
+        q[k] = (uint32_t)v;
+    }
+    key[i] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+    val[i] = (int32_t)i;
+}
+
+// Per-source constants of the FP32 radial functions: gaussian(-erf)
+// p0 = 1/sigma, p1 = 1/sigma^2; winckelmans p0 = sigma^2.
+template <int K>
+__global__ __launch_bounds__(EW_BLOCK) void pack_mixed_kernel(const double *__restrict__ X,
+                                                              const double *__restrict__ G,
+                                                              const double *__restrict__ sg,
+                                                              const int32_t *__restrict__ perm,
+                                                              int64_t n, int64_t cap,
+                                                              double4 *__restrict__ spos,
+                                                              float4 *__restrict__ sgam)
+{
+    const int64_t s = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = perm[s];
+    const double sig = sg[i];
+    double p0 = 0.0, p1 = 0.0;
+    if constexpr (K == GAUSSIANERF || K == GAUSSIAN) {
+        p0 = 1.0 / sig;
+        p1 = p0 * p0;
+    } else if constexpr (K == WINCKELMANS) {
+        p0 = sig * sig;
+    }
+    spos[s] = make_double4(X[i], X[cap + i], X[2 * cap + i], p0);
+    sgam[s] = make_float4((float)G[i], (float)G[cap + i], (float)G[2 * cap + i], (float)p1);
+}
+
+// FP32 tile partials of one target: U, W = sum A Gamma (Kronecker part of J)
+// and eight radial entries of J (the ninth follows from trace J = 0).
+struct MixedAcc {
+    float U0 = 0.f, U1 = 0.f, U2 = 0.f, W0 = 0.f, W1 = 0.f, W2 = 0.f;
+    float J0 = 0.f, J1 = 0.f, J2 = 0.f, J3 = 0.f, J4 = 0.f, J5 = 0.f, J6 = 0.f, J7 = 0.f;
+};
+
+// One pair in FP32.  a = (source position relative to the block origin, p0),
+// b = (Gamma, p1).  When every lane of the wave is in the far field
+// (rho^2 >= 49) the gaussian kernels skip exp/erf: g = 1, g' = 0 to FP32
+// rounding.  Coincident pairs contribute nothing.
+template <int K>
+__device__ __forceinline__ void mixed_pair(const float4 a, const float4 b, float xi, float yi,
+                                           float zi, MixedAcc &acc)
+{
+    const float dx = xi - a.x, dy = yi - a.y, dz = zi - a.z;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const bool self = !(r2 > 0.f);
+    const float rinv = self ? 0.f : frsq(r2);
+    float A, B;
+    if constexpr (K == GAUSSIANERF || K == GAUSSIAN) {
+        const bool near = r2 * b.w < RHO2_FAR;
+        if (__any(near)) {
+            radial_near_f32<K>(r2, rinv, a.w, b.w, A, B);
+        } else {
+            const float ri2 = rinv * rinv;
+            A = ri2 * rinv;
+            B = -3.0f * A * ri2;
+        }
+    } else {
+        radial_near_f32<K>(r2, rinv, a.w, b.w, A, B);
+    }
+    A = self ? 0.f : A;
+    B = self ? 0.f : B;
+    // c = dX x Gamma
+    const float c0 = fmaf(dy, b.z, -dz * b.y);
+    const float c1 = fmaf(dz, b.x, -dx * b.z);
+    const float c2 = fmaf(dx, b.y, -dy * b.x);
+    acc.U0 = fmaf(A, c0, acc.U0);
+    acc.U1 = fmaf(A, c1, acc.U1);
+    acc.U2 = fmaf(A, c2, acc.U2);
+    acc.W0 = fmaf(A, b.x, acc.W0);
+    acc.W1 = fmaf(A, b.y, acc.W1);
+    acc.W2 = fmaf(A, b.z, acc.W2);
+    const float ex = B * dx, ey = B * dy, ez = B * dz;
+    // radial part c_k dX_l B, column-major (k + 3 l)
+    acc.J0 = fmaf(c0, ex, acc.J0);
+    acc.J1 = fmaf(c1, ex, acc.J1);
+    acc.J2 = fmaf(c2, ex, acc.J2);
+    acc.J3 = fmaf(c0, ey, acc.J3);
+    acc.J4 = fmaf(c1, ey, acc.J4);
+    acc.J5 = fmaf(c2, ey, acc.J5);
+    acc.J6 = fmaf(c0, ez, acc.J6);
+    acc.J7 = fmaf(c1, ez, acc.J7);
+}
+
+template <int K>
+__global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_kernel(const double4 *__restrict__ spos,
+                                                            const float4 *__restrict__ sgam,
+                                                            int64_t n,
+                                                            const int32_t *__restrict__ perm,
+                                                            int64_t cap, double *__restrict__ U,
+                                                            double *__restrict__ J, int reset)
+{
+    __shared__ float4 tpos[UJ_BLOCK];
+    __shared__ float4 tgam[UJ_BLOCK];
+    const int t = threadIdx.x;
+    const int64_t s0 = (int64_t)blockIdx.x * UJ_BLOCK;
+    const int64_t s = s0 + t;
+    const bool active = s < n;
+    // Block origin: the block's first (Morton-sorted) target, in FP64.
+    const double4 o = spos[s0];
+    float xi = 0.f, yi = 0.f, zi = 0.f;
+    if (active) {
+        const double4 p = spos[s];
+        xi = (float)(p.x - o.x);
+        yi = (float)(p.y - o.y);
+        zi = (float)(p.z - o.z);
+    }
+    double dU0 = 0, dU1 = 0, dU2 = 0, dW0 = 0, dW1 = 0, dW2 = 0;
+    double dJ0 = 0, dJ1 = 0, dJ2 = 0, dJ3 = 0, dJ4 = 0, dJ5 = 0, dJ6 = 0, dJ7 = 0;
+
+    for (int64_t base = 0; base < n; base += UJ_BLOCK) {
+        __syncthreads();
+        const int64_t js = base + t;
+        if (js < n) {
+            const double4 p = spos[js];
+            tpos[t] = make_float4((float)(p.x - o.x), (float)(p.y - o.y), (float)(p.z - o.z),
+                                  (float)p.w);
+            tgam[t] = sgam[js];
+        }
+        __syncthreads();
+        const int cnt = (int)((n - base) < UJ_BLOCK ? (n - base) : UJ_BLOCK);
+        MixedAcc acc;
+        if (cnt == UJ_BLOCK) {
+#pragma unroll 4
+            for (int jj = 0; jj < UJ_BLOCK; ++jj)
+                mixed_pair<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
+        } else {
+            for (int jj = 0; jj < cnt; ++jj)
+                mixed_pair<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
+        }
+        dU0 += acc.U0; dU1 += acc.U1; dU2 += acc.U2;
+        dW0 += acc.W0; dW1 += acc.W1; dW2 += acc.W2;
+        dJ0 += acc.J0; dJ1 += acc.J1; dJ2 += acc.J2; dJ3 += acc.J3;
+        dJ4 += acc.J4; dJ5 += acc.J5; dJ6 += acc.J6; dJ7 += acc.J7;
+    }
+    if (!active) return;
+    const int64_t i = perm[s];
+    const double k = -INV4PI;
+    double u0 = k * dU0, u1 = k * dU1, u2 = k * dU2;
+    // J = -1/(4 pi) (radial + eps_klm W_m)
+    double j0 = k * dJ0, j1 = k * (dJ1 - dW2), j2 = k * (dJ2 + dW1);
+    double j3 = k * (dJ3 + dW2), j4 = k * dJ4, j5 = k * (dJ5 - dW0);
+    double j6 = k * (dJ6 - dW1), j7 = k * (dJ7 + dW0), j8 = -k * (dJ0 + dJ4);
+    if (!reset) {
+        u0 += U[i]; u1 += U[cap + i]; u2 += U[2 * cap + i];
+        j0 += J[i]; j1 += J[cap + i]; j2 += J[2 * cap + i];
+        j3 += J[3 * cap + i]; j4 += J[4 * cap + i]; j5 += J[5 * cap + i];
+        j6 += J[6 * cap + i]; j7 += J[7 * cap + i]; j8 += J[8 * cap + i];
+    }
+    U[i] = u0; U[cap + i] = u1; U[2 * cap + i] = u2;
+    J[i] = j0; J[cap + i] = j1; J[2 * cap + i] = j2;
+    J[3 * cap + i] = j3; J[4 * cap + i] = j4; J[5 * cap + i] = j5;
+    J[6 * cap + i] = j6; J[7 * cap + i] = j7; J[8 * cap + i] = j8;
+}
+
+size_t sort_temp_bytes(int64_t n)
+{
+    size_t bytes = 0;
+    (void)rocprim::radix_sort_pairs((void *)nullptr, bytes, (const uint32_t *)nullptr,
+                              (uint32_t *)nullptr, (const int32_t *)nullptr, (int32_t *)nullptr,
+                              (unsigned int)n, 0, 30, (hipStream_t)0);
+    return bytes;
+}
+
+hipError_t launch_sort_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, Scratch &s)
+{
+    const int64_t n = f.np;
+    if (n == 0) return hipSuccess;
+    const int nb = (int)std::min<int64_t>(RED_BLOCKS, grid_for(n, EW_BLOCK));
+    bbox_partial_kernel<<<nb, EW_BLOCK, 0, st>>>(f.X, n, f.cap, s.part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    morton_kernel<<<grid_for(n, EW_BLOCK), EW_BLOCK, 0, st>>>(f.X, n, f.cap, s.part, nb, s.kin,
+                                                             s.vin);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t bytes = s.sort_tmp_bytes;
+    e = rocprim::radix_sort_pairs(s.sort_tmp, bytes, s.kin, s.kout, s.vin, s.perm,
+                                  (unsigned int)n, 0, 30, st);
+    if (e != hipSuccess) return e;
+    const unsigned g = grid_for(n, EW_BLOCK);
+    switch (kernel) {
+    case SINGULAR:
+        pack_mixed_kernel<SINGULAR><<<g, EW_BLOCK, 0, st>>>(f.X, f.G, f.sigma, s.perm, n, f.cap, s.spos, s.sgam);
+        break;
+    case GAUSSIAN:
+        pack_mixed_kernel<GAUSSIAN><<<g, EW_BLOCK, 0, st>>>(f.X, f.G, f.sigma, s.perm, n, f.cap, s.spos, s.sgam);
+        break;
+    case GAUSSIANERF:
+        pack_mixed_kernel<GAUSSIANERF><<<g, EW_BLOCK, 0, st>>>(f.X, f.G, f.sigma, s.perm, n, f.cap, s.spos, s.sgam);
+        break;
+    case WINCKELMANS:
+        pack_mixed_kernel<WINCKELMANS><<<g, EW_BLOCK, 0, st>>>(f.X, f.G, f.sigma, s.perm, n, f.cap, s.spos, s.sgam);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
+                           int reset)
+{
+    const int64_t n = f.np;
+    if (n == 0) return hipSuccess;
+    const unsigned g = grid_for(n, UJ_BLOCK);
+    switch (kernel) {
+    case SINGULAR:
+        uj_mixed_kernel<SINGULAR><<<g, UJ_BLOCK, 0, st>>>(s.spos, s.sgam, n, s.perm, f.cap, f.U, f.J, reset);
+        break;
+    case GAUSSIAN:
+        uj_mixed_kernel<GAUSSIAN><<<g, UJ_BLOCK, 0, st>>>(s.spos, s.sgam, n, s.perm, f.cap, f.U, f.J, reset);
+        break;
+    case GAUSSIANERF:
+        uj_mixed_kernel<GAUSSIANERF><<<g, UJ_BLOCK, 0, st>>>(s.spos, s.sgam, n, s.perm, f.cap, f.U, f.J, reset);
+        break;
+    case WINCKELMANS:
+        uj_mixed_kernel<WINCKELMANS><<<g, UJ_BLOCK, 0, st>>>(s.spos, s.sgam, n, s.perm, f.cap, f.U, f.J, reset);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// Estr (SFS stretching), FP64
+// ===========================================================================
+
+// Source record: (x, y, z, 1/sigma) (Gx, Gy, Gz, 0) (J0..J3) (J4..J7) (J8, 0, 0, 0)
+__global__ __launch_bounds__(EW_BLOCK) void pack_estr_kernel(const double *__restrict__ X,
+                                                             const double *__restrict__ G,
+                                                             const double *__restrict__ sg,
+                                                             const double *__restrict__ Jm,
+                                                             int64_t n, int64_t cap,
+                                                             double4 *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    out[5 * i] = make_double4(X[i], X[cap + i], X[2 * cap + i], 1.0 / sg[i]);
+    out[5 * i + 1] = make_double4(G[i], G[cap + i], G[2 * cap + i], 0.0);
+    out[5 * i + 2] = make_double4(Jm[i], Jm[cap + i], Jm[2 * cap + i], Jm[3 * cap + i]);
+    out[5 * i + 3] = make_double4(Jm[4 * cap + i], Jm[5 * cap + i], Jm[6 * cap + i], Jm[7 * cap + i]);
+    out[5 * i + 4] = make_double4(Jm[8 * cap + i], 0.0, 0.0, 0.0);
+}
+
+template <int K, bool TRANSPOSED>
+__global__ __launch_bounds__(UJ_BLOCK) void estr_f64_kernel(const double4 *__restrict__ src,
+                                                            int64_t n, int64_t cap,
+                                                            double *__restrict__ E)
+{
+    __shared__ double4 tile[5 * UJ_BLOCK];
+    const int t = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * UJ_BLOCK + t;
+    const bool active = i < n;
+    double xi = 0, yi = 0, zi = 0, Ji[9];
+    for (int k = 0; k < 9; ++k) Ji[k] = 0.0;
+    if (active) {
+        const double4 p = src[5 * i];
+        xi = p.x; yi = p.y; zi = p.z;
+        const double4 q2 = src[5 * i + 2], q3 = src[5 * i + 3], q4 = src[5 * i + 4];
+        Ji[0] = q2.x; Ji[1] = q2.y; Ji[2] = q2.z; Ji[3] = q2.w;
+        Ji[4] = q3.x; Ji[5] = q3.y; Ji[6] = q3.z; Ji[7] = q3.w; Ji[8] = q4.x;
+    }
+    double e0 = 0, e1 = 0, e2 = 0;
+    for (int64_t base = 0; base < n; base += UJ_BLOCK) {
+        __syncthreads();
+        const int64_t js = base + t;
+        if (js < n) {
+            for (int k = 0; k < 5; ++k) tile[5 * t + k] = src[5 * js + k];
+        }
+        __syncthreads();
+        const int cnt = (int)((n - base) < UJ_BLOCK ? (n - base) : UJ_BLOCK);
+        for (int jj = 0; jj < cnt; ++jj) {
+            const double4 a = tile[5 * jj], gm = tile[5 * jj + 1];
+            const double4 q2 = tile[5 * jj + 2], q3 = tile[5 * jj + 3], q4 = tile[5 * jj + 4];
+            const double d1 = xi - a.x, d2 = yi - a.y, d3 = zi - a.z;
+            const double r = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+            const double zs = zeta_f64<K>(r * a.w) * (a.w * a.w * a.w);
+            double dJ[9];
+            dJ[0] = Ji[0] - q2.x; dJ[1] = Ji[1] - q2.y; dJ[2] = Ji[2] - q2.z;
+            dJ[3] = Ji[3] - q2.w; dJ[4] = Ji[4] - q3.x; dJ[5] = Ji[5] - q3.y;
+            dJ[6] = Ji[6] - q3.z; dJ[7] = Ji[7] - q3.w; dJ[8] = Ji[8] - q4.x;
+            double S0, S1, S2;
+            if constexpr (TRANSPOSED) {   // S_k = sum_l dJ[l,k] G_l
+                S0 = dJ[0] * gm.x + dJ[1] * gm.y + dJ[2] * gm.z;
+                S1 = dJ[3] * gm.x + dJ[4] * gm.y + dJ[5] * gm.z;
+                S2 = dJ[6] * gm.x + dJ[7] * gm.y + dJ[8] * gm.z;
+            } else {                      // S_k = sum_l dJ[k,l] G_l
+                S0 = dJ[0] * gm.x + dJ[3] * gm.y + dJ[6] * gm.z;
+                S1 = dJ[1] * gm.x + dJ[4] * gm.y + dJ[7] * gm.z;
+                S2 = dJ[2] * gm.x + dJ[5] * gm.y + dJ[8] * gm.z;
+            }
+            e0 += zs * S0;
+            e1 += zs * S1;
+            e2 += zs * S2;
+        }
+    }
+    if (!active) return;
+    E[i] = e0;
+    E[cap + i] = e1;
+    E[2 * cap + i] = e2;
+}
+
+hipError_t launch_estr_f64(hipStream_t st, int kernel, int transposed, const DeviceField &f,
+                           Scratch &s)
+{
+    const int64_t n = f.np;
+    if (n == 0) return hipSuccess;
+    pack_estr_kernel<<<grid_for(n, EW_BLOCK), EW_BLOCK, 0, st>>>(f.X, f.G, f.sigma, f.J, n, f.cap,
+                                                                 s.srcE);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const unsigned g = grid_for(n, UJ_BLOCK);
+#define VPMX_ESTR(KK)                                                                        \
+    if (transposed)                                                                          \
+        estr_f64_kernel<KK, true><<<g, UJ_BLOCK, 0, st>>>(s.srcE, n, f.cap, f.E);            \
+    else                                                                                     \
+        estr_f64_kernel<KK, false><<<g, UJ_BLOCK, 0, st>>>(s.srcE, n, f.cap, f.E);
+    switch (kernel) {
+    case GAUSSIAN: VPMX_ESTR(GAUSSIAN); break;
+    case GAUSSIANERF: VPMX_ESTR(GAUSSIANERF); break;
+    case WINCKELMANS: VPMX_ESTR(WINCKELMANS); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef VPMX_ESTR
+    return hipGetLastError();
+}
+
+// C = Cs, clipped to 0 where C Gamma.Estr < 0 (docs/src/theory/rvpm.md:296).
+__global__ __launch_bounds__(EW_BLOCK) void sfs_coeff_kernel(int64_t n, int64_t cap,
+                                                             const double *__restrict__ G,
+                                                             const double *__restrict__ E,
+                                                             double *__restrict__ C, double Cs,
+                                                             int clip)
+{
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double c = Cs;
+    if (clip) {
+        const double ge = G[i] * E[i] + G[cap + i] * E[cap + i] + G[2 * cap + i] * E[2 * cap + i];
+        if (c * ge < 0.0) c = 0.0;
+    }
+    C[i] = c;
+}
+
+hipError_t launch_sfs_coeff(hipStream_t st, const DeviceField &f, double Cs, int clip)
+{
+    if (f.np == 0) return hipSuccess;
+    sfs_coeff_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(f.np, f.cap, f.G, f.E, f.C, Cs,
+                                                                    clip);
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// Streaming passes (FP64)
+// ===========================================================================
+
+__global__ __launch_bounds__(EW_BLOCK) void fill_kernel(double *__restrict__ p, int64_t n, double v)
+{
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill(hipStream_t st, double *p, int64_t n, double v)
+{
+    if (n == 0) return hipSuccess;
+    fill_kernel<<<grid_for(n, EW_BLOCK), EW_BLOCK, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+// One low-storage RK substep (Euler is a = 0, b = 1) of every non-static
+// particle:  q <- a q + dt F(y);  y <- y + b q  for y = (X, Gamma, sigma), with
+// F the rVPM right-hand side
+//   Z     = [ (g+f)/(1+3f) S.G - f/(1+3f) (C sigma^3/zeta0) E.G ] / |G|^2
+//   dX/dt = U + Uinf,  dG/dt = S - 3 Z G - (C sigma^3/zeta0) E,  dsigma/dt = -sigma Z.
+struct UpdateArgs {
+    double a, b, dt, f, g, zeta0, Ux, Uy, Uz;
+    int transposed, sfs;
+};
+
+__global__ __launch_bounds__(EW_BLOCK) void update_kernel(int64_t n, int64_t cap, UpdateArgs q,
+                                                          double *__restrict__ X,
+                                                          double *__restrict__ G,
+                                                          double *__restrict__ sg,
+                                                          const double *__restrict__ U,
+                                                          const double *__restrict__ J,
+                                                          const double *__restrict__ E,
+                                                          const double *__restrict__ C,
+                                                          double *__restrict__ M,
+                                                          const uint8_t *__restrict__ st)
+{
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (st && st[i]) return;
+    double Jm[9];
+    for (int k = 0; k < 9; ++k) Jm[k] = J[k * cap + i];
+    const double G0 = G[i], G1 = G[cap + i], G2 = G[2 * cap + i];
+    const double sig = sg[i];
+    double S0, S1, S2;
+    if (q.transposed) {
+        S0 = Jm[0] * G0 + Jm[1] * G1 + Jm[2] * G2;
+        S1 = Jm[3] * G0 + Jm[4] * G1 + Jm[5] * G2;
+        S2 = Jm[6] * G0 + Jm[7] * G1 + Jm[8] * G2;
+    } else {
+        S0 = Jm[0] * G0 + Jm[3] * G1 + Jm[6] * G2;
+        S1 = Jm[1] * G0 + Jm[4] * G1 + Jm[7] * G2;
+        S2 = Jm[2] * G0 + Jm[5] * G1 + Jm[8] * G2;
+    }
+    double e0 = 0, e1 = 0, e2 = 0, csfs = 0;
+    if (q.sfs) {
+        e0 = E[i]; e1 = E[cap + i]; e2 = E[2 * cap + i];
+        csfs = C[i] * sig * sig * sig / q.zeta0;
+    }
+    const double Gm2 = G0 * G0 + G1 * G1 + G2 * G2;
+    const double SG = S0 * G0 + S1 * G1 + S2 * G2;
+    const double EG = e0 * G0 + e1 * G1 + e2 * G2;
+    double Z = 0.0;
+    if (Gm2 > 0.0)
+        Z = ((q.g + q.f) / (1.0 + 3.0 * q.f) * SG - q.f / (1.0 + 3.0 * q.f) * csfs * EG) / Gm2;
+    const double F[7] = {U[i] + q.Ux, U[cap + i] + q.Uy, U[2 * cap + i] + q.Uz,
+                         S0 - 3.0 * Z * G0 - csfs * e0, S1 - 3.0 * Z * G1 - csfs * e1,
+                         S2 - 3.0 * Z * G2 - csfs * e2, -sig * Z};
+    double Mq[7];
+    for (int k = 0; k < 7; ++k) Mq[k] = q.a * M[k * cap + i] + q.dt * F[k];
+    for (int k = 0; k < 7; ++k) M[k * cap + i] = Mq[k];
+    X[i] += q.b * Mq[0];
+    X[cap + i] += q.b * Mq[1];
+    X[2 * cap + i] += q.b * Mq[2];
+    G[i] = G0 + q.b * Mq[3];
+    G[cap + i] = G1 + q.b * Mq[4];
+    G[2 * cap + i] = G2 + q.b * Mq[5];
+    sg[i] = sig + q.b * Mq[6];
+}
+
+hipError_t launch_update(hipStream_t st, const DeviceField &f, int transposed, int sfs,
+                         double fcoef, double gcoef, double zeta0, const double Uinf[3],
+                         double a, double b, double dt)
+{
+    if (f.np == 0) return hipSuccess;
+    UpdateArgs q{a, b, dt, fcoef, gcoef, zeta0, Uinf[0], Uinf[1], Uinf[2], transposed, sfs};
+    update_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(
+        f.np, f.cap, q, f.X, f.G, f.sigma, f.U, f.J, f.E, f.C, f.M,
+        f.any_static ? f.is_static : nullptr);
+    return hipGetLastError();
+}
+
+// Pedrizzetti relaxation; kind 2 rescales to keep |Gamma|.
+__global__ __launch_bounds__(EW_BLOCK) void relax_kernel(int64_t n, int64_t cap, int kind,
+                                                         double rlxf, const double *__restrict__ J,
+                                                         double *__restrict__ G,
+                                                         const uint8_t *__restrict__ st)
+{
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (st && st[i]) return;
+    const double w1 = J[5 * cap + i] - J[7 * cap + i];
+    const double w2 = J[6 * cap + i] - J[2 * cap + i];
+    const double w3 = J[cap + i] - J[3 * cap + i];
+    const double nw = sqrt(w1 * w1 + w2 * w2 + w3 * w3);
+    if (nw == 0.0) return;
+    double g0 = G[i], g1 = G[cap + i], g2 = G[2 * cap + i];
+    const double nG = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+    const double cosang = (g0 * w1 + g1 * w2 + g2 * w3) / (nG * nw);
+    g0 = (1.0 - rlxf) * g0 + rlxf * nG * w1 / nw;
+    g1 = (1.0 - rlxf) * g1 + rlxf * nG * w2 / nw;
+    g2 = (1.0 - rlxf) * g2 + rlxf * nG * w3 / nw;
+    if (kind == 2) {
+        const double bb = sqrt(1.0 - 2.0 * (1.0 - rlxf) * rlxf * (1.0 - cosang));
+        g0 /= bb; g1 /= bb; g2 /= bb;
+    }
+    G[i] = g0;
+    G[cap + i] = g1;
+    G[2 * cap + i] = g2;
+}
+
+hipError_t launch_relax(hipStream_t st, const DeviceField &f, int kind, double rlxf)
+{
+    if (f.np == 0 || kind == 0) return hipSuccess;
+    relax_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(
+        f.np, f.cap, kind, rlxf, f.J, f.G, f.any_static ? f.is_static : nullptr);
+    return hipGetLastError();
+}
+
+// Enstrophy and energy: per-block partials, then one block sums them in a
+// fixed order (deterministic).
+__global__ __launch_bounds__(EW_BLOCK) void diag_partial_kernel(int64_t n, int64_t cap,
+                                                                const double *__restrict__ X,
+                                                                const double *__restrict__ G,
+                                                                const double *__restrict__ U,
+                                                                const double *__restrict__ J,
+                                                                double *__restrict__ part)
+{
+    __shared__ double r0[EW_BLOCK], r1[EW_BLOCK];
+    double ens = 0.0, en = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * EW_BLOCK) {
+        const double g0 = G[i], g1 = G[cap + i], g2 = G[2 * cap + i];
+        const double w1 = J[5 * cap + i] - J[7 * cap + i];
+        const double w2 = J[6 * cap + i] - J[2 * cap + i];
+        const double w3 = J[cap + i] - J[3 * cap + i];
+        ens += g0 * w1 + g1 * w2 + g2 * w3;
+        const double x0 = X[i], x1 = X[cap + i], x2 = X[2 * cap + i];
+        const double c0 = x1 * g2 - x2 * g1, c1 = x2 * g0 - x0 * g2, c2 = x0 * g1 - x1 * g0;
+        en += U[i] * c0 + U[cap + i] * c1 + U[2 * cap + i] * c2;
+    }
+    r0[threadIdx.x] = ens;
+    r1[threadIdx.x] = en;
+    __syncthreads();
+    for (int w = EW_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            r0[threadIdx.x] += r0[threadIdx.x + w];
+            r1[threadIdx.x] += r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = r0[0];
+        part[2 * blockIdx.x + 1] = r1[0];
+    }
+}
+
+__global__ void diag_final_kernel(const double *__restrict__ part, int nparts,
+                                  double *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < nparts; ++k) {
+        a += part[2 * k];
+        b += part[2 * k + 1];
+    }
+    out[0] = 0.5 * a;
+    out[1] = b;
+}
+
+hipError_t launch_diagnostics(hipStream_t st, const DeviceField &f, Scratch &s, double *out2)
+{
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, grid_for(f.np, EW_BLOCK)));
+    diag_partial_kernel<<<nb, EW_BLOCK, 0, st>>>(f.np, f.cap, f.X, f.G, f.U, f.J, s.part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    diag_final_kernel<<<1, 64, 0, st>>>(s.part, nb, out2);
+    return hipGetLastError();
+}
+
+double zeta0_of(int kernel)
+{
+    switch (kernel) {
+    case GAUSSIAN: return 0.238732414637843;        // 3/(4 pi)
+    case GAUSSIANERF: return 0.06349363593424097;   // (2 pi)^-3/2
+    case WINCKELMANS: return 0.5968310365946075;    // 15/(8 pi)
+    default: return 1.0;
+    }
+}
+
+}  // namespace vpmx

@@ -1,0 +1,393 @@
+"""Host-side mirror of the FLOWVPM particle-field interface that FLOWUnsteady
+drives, backed by libvpmx.so.
+
+FLOWUnsteady builds its field as
+
+    vpm.ParticleField(max_particles, Float64; Uinf, formulation, viscous,
+                      kernel, UJ, SFS, integration, transposed, relaxation, fmm)
+                                              (src/FLOWUnsteady_simulation.jl:239-253)
+
+and then calls `vpm.add_particle` (:486), `vpm.nextstep(pfield, dt; relax)`
+(:358), `pfield.UJ(pfield)` (:544), `vpm.get_np` / `vpm.remove_particle`
+(:350-365) and `vpm.monitor_enstrophy` (src/FLOWUnsteady_monitors.jl:614).
+This module keeps those names, argument meanings and the column-per-particle
+`particles` matrix, so the same driver logic runs on the GPU path:
+`UJ=UJ_gpu` replaces `vpm.UJ_direct`/`vpm.UJ_fmm` and
+`integration=rungekutta3_gpu` replaces `vpm.rungekutta3`.
+
+Row layout of `particles` (0-based rows, one column per particle): the fields
+FLOWVPM stores per particle, in its order (position, circulation, core size,
+volume, circulation magnitude, velocity, vorticity, Jacobian, PSE,
+integration memory, SFS coefficient, SFS term, static flag).  The library only
+sees the row offsets through `vpmx_layout`, so a different row order (another
+FLOWVPM version) is a different layout, not a different library.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import _native as N
+
+# ---- particle matrix rows -------------------------------------------------------
+X_INDEX = slice(0, 3)
+GAMMA_INDEX = slice(3, 6)
+SIGMA_INDEX = 6
+VOL_INDEX = 7
+CIRCULATION_INDEX = 8
+U_INDEX = slice(9, 12)
+VORTICITY_INDEX = slice(12, 15)
+J_INDEX = slice(15, 24)
+PSE_INDEX = slice(24, 27)
+M_INDEX = slice(27, 36)
+C_INDEX = slice(36, 39)
+SFS_INDEX = slice(39, 42)
+STATIC_INDEX = 42
+NFIELDS = 43
+
+
+def default_layout() -> N.Layout:
+    L = N.Layout()
+    L.nrows = NFIELDS
+    L.X, L.Gamma, L.sigma, L.vol = X_INDEX.start, GAMMA_INDEX.start, SIGMA_INDEX, VOL_INDEX
+    L.U, L.J, L.SFS, L.C = U_INDEX.start, J_INDEX.start, SFS_INDEX.start, C_INDEX.start
+    L.static_ = STATIC_INDEX
+    return L
+
+
+# ---- kernels (src/FLOWUnsteady_simulation.jl:37,128) ------------------------------
+@dataclass(frozen=True)
+class Kernel:
+    """Radial basis zeta(rho), regularising function g(rho), g'(rho)
+    (docs/src/theory/rvpm.md:84,89-99)."""
+    name: str
+    id: int
+    zeta: Callable
+    g: Callable
+    dg: Callable
+
+    def zeta0(self) -> float:
+        return float(self.zeta(0.0))
+
+
+def _erf(x):
+    from scipy.special import erf
+    return erf(x)
+
+
+singular = Kernel("singular", N.SINGULAR, lambda r: np.zeros_like(np.asarray(r, float)),
+                  lambda r: np.ones_like(np.asarray(r, float)),
+                  lambda r: np.zeros_like(np.asarray(r, float)))
+gaussian = Kernel("gaussian", N.GAUSSIAN, lambda r: 3 / (4 * np.pi) * np.exp(-np.asarray(r) ** 3),
+                  lambda r: 1 - np.exp(-np.asarray(r) ** 3),
+                  lambda r: 3 * np.asarray(r) ** 2 * np.exp(-np.asarray(r) ** 3))
+gaussianerf = Kernel(
+    "gaussianerf", N.GAUSSIANERF,
+    lambda r: (2 * np.pi) ** -1.5 * np.exp(-np.asarray(r) ** 2 / 2),
+    lambda r: _erf(np.asarray(r) / np.sqrt(2)) - np.sqrt(2 / np.pi) * np.asarray(r) * np.exp(-np.asarray(r) ** 2 / 2),
+    lambda r: np.sqrt(2 / np.pi) * np.asarray(r) ** 2 * np.exp(-np.asarray(r) ** 2 / 2))
+winckelmans = Kernel(
+    "winckelmans", N.WINCKELMANS,
+    lambda r: 15 / (8 * np.pi) * (np.asarray(r) ** 2 + 1) ** -3.5,
+    lambda r: np.asarray(r) ** 3 * (np.asarray(r) ** 2 + 2.5) * (np.asarray(r) ** 2 + 1) ** -2.5,
+    lambda r: 7.5 * np.asarray(r) ** 2 * (np.asarray(r) ** 2 + 1) ** -3.5)
+
+
+# ---- formulations (src/FLOWUnsteady_simulation.jl:36,127) --------------------------
+@dataclass(frozen=True)
+class Formulation:
+    name: str
+    f: float
+    g: float
+
+
+rVPM = Formulation("rVPM", 0.0, 0.2)   # docs/src/theory/rvpm.md:196-236
+cVPM = Formulation("cVPM", 0.0, 0.0)
+
+
+# ---- relaxation (src/FLOWUnsteady_simulation.jl:44,135) ----------------------------
+@dataclass(frozen=True)
+class Relaxation:
+    name: str
+    kind: int
+    nsteps_relax: int
+    rlxf: float
+
+
+norelaxation = Relaxation("norelaxation", N.RELAX_NONE, 0, 0.0)
+pedrizzetti = Relaxation("pedrizzetti", N.RELAX_PEDRIZZETTI, 1, 0.3)
+correctedpedrizzetti = Relaxation("correctedpedrizzetti", N.RELAX_CORRECTED, 1, 0.3)
+relaxation_none = norelaxation
+
+
+# ---- SFS (src/FLOWUnsteady_simulation.jl:39,130) -------------------------------------
+@dataclass(frozen=True)
+class SubFilterScale:
+    name: str
+    kind: int
+    Cs: float = 0.0
+    clip_backscatter: bool = True
+
+
+noSFS = SubFilterScale("noSFS", N.SFS_NONE)
+SFS_none = noSFS
+
+
+def ConstantSFS(Cs: float, clip_backscatter: bool = True) -> SubFilterScale:
+    """Constant-coefficient model C = Cs with Estr_direct and, by default,
+    backscatter clipping (docs/src/theory/rvpm.md:251-262,296)."""
+    return SubFilterScale("ConstantSFS", N.SFS_CONSTANT, float(Cs), bool(clip_backscatter))
+
+
+def SFS_Cs_nobackscatter(Cs: float = 0.15) -> SubFilterScale:
+    return ConstantSFS(Cs, clip_backscatter=True)
+
+
+# ---- FMM settings (src/FLOWUnsteady_simulation.jl:43) -------------------------------
+@dataclass(frozen=True)
+class FMM:
+    p: int = 4
+    ncrit: int = 50
+    theta: float = 0.4
+
+
+# ---- contexts: one per device, created on first use -----------------------------------
+_contexts: dict = {}
+
+
+def _context(device: int) -> N.Context:
+    ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = _contexts[device] = N.Context(device)
+    return ctx
+
+
+# ---- the particle field -----------------------------------------------------------------
+class ParticleField:
+    """Column-per-particle particle matrix plus the solver hooks.
+
+    `UJ` and `integration` are function-valued, as in FLOWVPM: the defaults
+    are the GPU drop-ins `UJ_gpu` and `rungekutta3_gpu`.  `precision` selects
+    the library's FP64 path or its mixed FP32/FP64 path.
+    """
+
+    def __init__(self, maxparticles: int, *, formulation: Formulation = rVPM,
+                 kernel: Kernel = gaussianerf, UJ: Optional[Callable] = None,
+                 SFS: SubFilterScale = noSFS, integration: Optional[Callable] = None,
+                 transposed: bool = True, relaxation: Relaxation = pedrizzetti,
+                 fmm: FMM = FMM(), Uinf: Callable = lambda t: np.zeros(3),
+                 precision: str = "fp64", device: int = 0):
+        if maxparticles < 0:
+            raise ValueError("maxparticles must be non-negative")
+        self.particles = np.zeros((NFIELDS, maxparticles), dtype=np.float64, order="F")
+        self.maxparticles = maxparticles
+        self.np = 0
+        self.nt = 0
+        self.t = 0.0
+        self.formulation = formulation
+        self.kernel = kernel
+        self.UJ = UJ if UJ is not None else UJ_gpu
+        self.SFS = SFS
+        self.integration = integration if integration is not None else rungekutta3_gpu
+        self.transposed = transposed
+        self.relaxation = relaxation
+        self.fmm = fmm
+        self.Uinf = Uinf
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.device = device
+        self.layout = default_layout()
+
+    # -- pointer handed to the C ABI: the whole column-major matrix
+    def _ptr(self):
+        return self.particles.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def _uj_params(self, *, reset=True, sfs=False, transposed=None) -> N.UJParams:
+        P = N.UJParams()
+        P.kernel = self.kernel.id
+        P.precision = N.MIXED if self.precision == "mixed" else N.FP64
+        P.transposed = int(self.transposed if transposed is None else transposed)
+        P.reset = int(reset)
+        P.sfs = int(bool(sfs))
+        P.fmm = 0
+        P.p, P.ncrit, P.theta = self.fmm.p, self.fmm.ncrit, self.fmm.theta
+        return P
+
+    def _step_params(self) -> N.StepParams:
+        S = N.StepParams()
+        S.f, S.g = self.formulation.f, self.formulation.g
+        u = np.asarray(self.Uinf(self.t), dtype=float)
+        for k in range(3):
+            S.Uinf[k] = float(u[k])
+        S.sfs = self.SFS.kind
+        S.Cs = self.SFS.Cs
+        S.clip_backscatter = int(self.SFS.clip_backscatter)
+        S.relax_kind = self.relaxation.kind
+        S.rlxf = self.relaxation.rlxf
+        return S
+
+
+def add_particle(pfield: ParticleField, X, Gamma, sigma, *, vol=0.0, circulation=1.0,
+                 C=0.0, static=False):
+    """Append one particle (vpm.add_particle, src/FLOWUnsteady_simulation.jl:486)."""
+    if pfield.np >= pfield.maxparticles:
+        raise ValueError(f"PARTICLE OVERFLOW: maximum of {pfield.maxparticles} particles reached")
+    col = pfield.particles[:, pfield.np]
+    col[:] = 0.0
+    col[X_INDEX] = X
+    col[GAMMA_INDEX] = Gamma
+    col[SIGMA_INDEX] = sigma
+    col[VOL_INDEX] = vol
+    col[CIRCULATION_INDEX] = circulation
+    col[C_INDEX] = C
+    col[STATIC_INDEX] = 1.0 if static else 0.0
+    pfield.np += 1
+
+
+def add_particles(pfield: ParticleField, X, Gamma, sigma, vol=None, static=None):
+    """Vectorised add_particle for many particles at once."""
+    X = np.asarray(X, float).reshape(-1, 3)
+    n = X.shape[0]
+    if pfield.np + n > pfield.maxparticles:
+        raise ValueError(f"PARTICLE OVERFLOW: maximum of {pfield.maxparticles} particles reached")
+    sl = slice(pfield.np, pfield.np + n)
+    P = pfield.particles
+    P[:, sl] = 0.0
+    P[X_INDEX, sl] = X.T
+    P[GAMMA_INDEX, sl] = np.asarray(Gamma, float).reshape(-1, 3).T
+    P[SIGMA_INDEX, sl] = np.broadcast_to(np.asarray(sigma, float), (n,))
+    if vol is not None:
+        P[VOL_INDEX, sl] = np.broadcast_to(np.asarray(vol, float), (n,))
+    P[CIRCULATION_INDEX, sl] = 1.0
+    if static is not None:
+        P[STATIC_INDEX, sl] = np.asarray(static, float)
+    pfield.np += n
+
+
+def remove_particle(pfield: ParticleField, i: int):
+    """Remove particle i (0-based) by moving the last particle into its slot,
+    as FLOWVPM's remove_particle does."""
+    if not 0 <= i < pfield.np:
+        raise IndexError(i)
+    last = pfield.np - 1
+    if i != last:
+        pfield.particles[:, i] = pfield.particles[:, last]
+    pfield.particles[:, last] = 0.0
+    pfield.np -= 1
+
+
+def get_np(pfield: ParticleField) -> int:
+    return pfield.np
+
+
+def get_X(pfield, i=None):
+    return _rows(pfield, X_INDEX, i)
+
+
+def get_Gamma(pfield, i=None):
+    return _rows(pfield, GAMMA_INDEX, i)
+
+
+def get_sigma(pfield, i=None):
+    return pfield.particles[SIGMA_INDEX, :pfield.np] if i is None else pfield.particles[SIGMA_INDEX, i]
+
+
+def get_U(pfield, i=None):
+    return _rows(pfield, U_INDEX, i)
+
+
+def get_J(pfield, i=None):
+    """J[:, k + 3*l] = dU_k/dx_l (column-major 3x3 per particle)."""
+    return _rows(pfield, J_INDEX, i)
+
+
+def get_SFS(pfield, i=None):
+    return _rows(pfield, SFS_INDEX, i)
+
+
+def get_C(pfield, i=None):
+    return pfield.particles[C_INDEX.start, :pfield.np] if i is None else pfield.particles[C_INDEX.start, i]
+
+
+def _rows(pfield, sl, i):
+    if i is None:
+        return pfield.particles[sl, :pfield.np].T
+    return pfield.particles[sl, i]
+
+
+def _reset_particles(pfield: ParticleField):
+    """Zero U and J (processing_force.jl:237)."""
+    pfield.particles[U_INDEX, :pfield.np] = 0.0
+    pfield.particles[J_INDEX, :pfield.np] = 0.0
+
+
+# ---- the drop-in hooks --------------------------------------------------------------------
+def UJ_gpu(pfield: ParticleField, *, reset: bool = True, reset_sfs: bool = False,
+           sfs: bool = False, transposed: Optional[bool] = None, **kwargs):
+    """Drop-in for vpm.UJ_direct / vpm.UJ_fmm (called as pfield.UJ(pfield) at
+    src/FLOWUnsteady_simulation.jl:544).  Writes U and J of all particles
+    (static and probes included) into the particle matrix; with `sfs`, also
+    the Estr term into the SFS rows.  Extra keyword arguments are accepted and
+    ignored, as FLOWVPM's integrators pass their own."""
+    if reset_sfs and sfs:
+        pfield.particles[SFS_INDEX, :pfield.np] = 0.0
+    ctx = _context(pfield.device)
+    P = pfield._uj_params(reset=reset, sfs=sfs, transposed=transposed)
+    ctx.check(N.lib().vpmx_uj(ctx.ptr, pfield._ptr(), pfield.np, ctypes.byref(pfield.layout),
+                              ctypes.byref(P)))
+
+
+UJ_direct = UJ_gpu
+
+
+def rungekutta3_gpu(pfield: ParticleField, dt: float, *, relax: bool = False, **kwargs):
+    """Drop-in for vpm.rungekutta3 (vpm_integration,
+    src/FLOWUnsteady_simulation.jl:40,131): three low-storage RK stages with
+    the particle state resident on the GPU, then relaxation if `relax`."""
+    ctx = _context(pfield.device)
+    U = pfield._uj_params(sfs=pfield.SFS.kind != N.SFS_NONE)
+    S = pfield._step_params()
+    ctx.check(N.lib().vpmx_rk3_step(ctx.ptr, pfield._ptr(), pfield.np,
+                                    ctypes.byref(pfield.layout), ctypes.byref(U),
+                                    ctypes.byref(S), float(dt), int(bool(relax))))
+
+
+def euler_gpu(pfield: ParticleField, dt: float, *, relax: bool = False, **kwargs):
+    """Drop-in for vpm.euler (src/FLOWUnsteady_simulation.jl:40)."""
+    ctx = _context(pfield.device)
+    U = pfield._uj_params(sfs=pfield.SFS.kind != N.SFS_NONE)
+    S = pfield._step_params()
+    ctx.check(N.lib().vpmx_euler_step(ctx.ptr, pfield._ptr(), pfield.np,
+                                      ctypes.byref(pfield.layout), ctypes.byref(U),
+                                      ctypes.byref(S), float(dt), int(bool(relax))))
+
+
+rungekutta3 = rungekutta3_gpu
+euler = euler_gpu
+
+
+def nextstep(pfield: ParticleField, dt: float, *, relax: bool = False, custom_UJ=None):
+    """vpm.nextstep (src/FLOWUnsteady_simulation.jl:358): one integration step,
+    then advance t and nt."""
+    pfield.integration(pfield, dt, relax=relax)
+    pfield.t += dt
+    pfield.nt += 1
+
+
+def relax_due(pfield: ParticleField, i: int) -> bool:
+    """Relaxation cadence of src/FLOWUnsteady_simulation.jl:346-348."""
+    r = pfield.relaxation
+    return r.kind != N.RELAX_NONE and r.nsteps_relax >= 1 and i > 0 and i % r.nsteps_relax == 0
+
+
+def monitor_enstrophy(pfield: ParticleField) -> float:
+    """0.5 sum_p Gamma_p . omega(x_p) with omega = curl U from J
+    (src/FLOWUnsteady_monitors.jl:573-574)."""
+    J = get_J(pfield)
+    G = get_Gamma(pfield)
+    w = np.stack([J[:, 5] - J[:, 7], J[:, 6] - J[:, 2], J[:, 1] - J[:, 3]], axis=1)
+    return 0.5 * float(np.sum(G * w))

@@ -1,0 +1,189 @@
+/*
+ * vpmx.h -- C ABI of the MI355X rVPM particle-interaction library (libvpmx.so).
+ *
+ * Drop-in boundary for FLOWUnsteady's particle-field hooks.  FLOWUnsteady
+ * builds its particle field with function-valued hooks
+ *
+ *   vpm.ParticleField(max_particles, R; kernel=vpm_kernel, UJ=vpm_UJ,
+ *                     SFS=vpm_SFS, integration=vpm_integration,
+ *                     transposed=vpm_transposed, relaxation=vpm_relaxation,
+ *                     fmm=vpm_fmm)           src/FLOWUnsteady_simulation.jl:239-253
+ *
+ * and calls them as pfield.UJ(pfield) (src/FLOWUnsteady_simulation.jl:544,
+ * src/FLOWUnsteady_processing_force.jl:238) and
+ * vpm.nextstep(pfield, dt; relax) -> pfield.integration(pfield, dt; relax)
+ * (src/FLOWUnsteady_simulation.jl:358).  A Julia `UJ_gpu(pfield; ...)` and
+ * `rungekutta3_gpu(pfield, dt; relax)` bind vpmx_uj / vpmx_rk3_step with one
+ * `ccall` each, passing pfield.particles (a column-per-particle Matrix{Float64})
+ * and the row offsets of its fields.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a nonzero VPMX_E* code on
+ *    failure; vpmx_last_error() gives the message.  Nothing aborts.
+ *  - Host pointers are never retained past a call.  The context owns device
+ *    buffers, grows them geometrically and reuses them across calls.
+ *  - A context is single-threaded: one per host thread, or guard it.
+ *  - All calls are blocking unless stated otherwise.
+ *  - Host particle matrix: column-major, `nrows` doubles per particle, field
+ *    row r of particle p at particles[r + p*nrows] (0-based rows).
+ *  - J[k,l] = dU_k/dx_l stored column-major: row J + k + 3*l.
+ *  - Results are deterministic: repeated calls are bitwise identical.
+ */
+#ifndef VPMX_H
+#define VPMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define VPMX_OK 0
+#define VPMX_EINVAL 1     /* bad argument, layout, sigma <= 0, NaN/Inf input   */
+#define VPMX_ENOMEM 2     /* device or pinned-host allocation failed           */
+#define VPMX_EHIP 3       /* HIP runtime or kernel-launch error                */
+#define VPMX_ENODEV 4     /* no usable GPU / bad device ordinal                */
+#define VPMX_EINJECT 5    /* fault injected with vpmx_inject_fault             */
+#define VPMX_EUNSUPPORTED 6
+
+/* ---- enumerations ---------------------------------------------------------- */
+/* Radial basis / regularising kernels: vpm.singular, vpm.gaussian,
+ * vpm.gaussianerf, vpm.winckelmans (src/FLOWUnsteady_simulation.jl:37,128). */
+#define VPMX_KERNEL_SINGULAR 0
+#define VPMX_KERNEL_GAUSSIAN 1
+#define VPMX_KERNEL_GAUSSIANERF 2
+#define VPMX_KERNEL_WINCKELMANS 3
+
+/* Arithmetic of the pairwise kernels. */
+#define VPMX_FP64 0       /* FP64 throughout (the correctness path)            */
+#define VPMX_MIXED 1      /* FP32 pair math on block-relative coordinates,
+                             FP32 tile partials folded into FP64 accumulators */
+
+/* Relaxation schemes: vpm.norelaxation, vpm.pedrizzetti,
+ * vpm.correctedpedrizzetti (src/FLOWUnsteady_simulation.jl:44,135). */
+#define VPMX_RELAX_NONE 0
+#define VPMX_RELAX_PEDRIZZETTI 1
+#define VPMX_RELAX_CORRECTED 2
+
+/* SFS models (src/FLOWUnsteady_simulation.jl:39,130). */
+#define VPMX_SFS_NONE 0
+#define VPMX_SFS_CONSTANT 1   /* C = Cs with Estr_direct, optional clipping   */
+
+/* ---- types ----------------------------------------------------------------- */
+typedef struct vpmx_ctx vpmx_ctx;
+
+/* Row offsets (0-based) of the particle fields inside one particle column;
+ * -1 marks a field the caller does not hold.  Filled by the Julia shim from
+ * FLOWVPM's *_INDEX constants. */
+typedef struct {
+    int32_t nrows;        /* leading dimension: doubles per particle column */
+    int32_t X;            /* 3 rows: position                              */
+    int32_t Gamma;        /* 3 rows: vectorial circulation                 */
+    int32_t sigma;        /* 1 row : smoothing radius                      */
+    int32_t vol;          /* 1 row : volume (-1 allowed)                   */
+    int32_t U;            /* 3 rows: velocity (output)                     */
+    int32_t J;            /* 9 rows: velocity Jacobian (output)            */
+    int32_t SFS;          /* 3 rows: Estr (output when sfs != 0; -1 allowed) */
+    int32_t C;            /* 1 row : SFS coefficient (-1 allowed)          */
+    int32_t static_;      /* 1 row : static flag, nonzero = static (-1 allowed) */
+} vpmx_layout;
+
+/* Settings of one UJ evaluation (the ParticleField's kernel/UJ/fmm fields). */
+typedef struct {
+    int32_t kernel;       /* VPMX_KERNEL_*                                  */
+    int32_t precision;    /* VPMX_FP64 or VPMX_MIXED                        */
+    int32_t transposed;   /* stretching scheme used by the SFS pass         */
+    int32_t reset;        /* 1: overwrite U,J; 0: accumulate into them      */
+    int32_t sfs;          /* VPMX_SFS_*: also evaluate Estr into SFS rows   */
+    int32_t fmm;          /* 0 direct (the only scheme in this version)     */
+    int32_t p, ncrit;     /* FMM expansion order and leaf size (reserved)   */
+    double theta;         /* FMM multipole acceptance criterion (reserved)  */
+} vpmx_uj_params;
+
+/* Settings of one time step (formulation, freestream, SFS, relaxation). */
+typedef struct {
+    double f, g;          /* formulation: rVPM f=0 g=1/5, cVPM f=g=0        */
+    double Uinf[3];       /* freestream at the step's start time            */
+    int32_t sfs;          /* VPMX_SFS_*                                     */
+    double Cs;            /* constant SFS coefficient                       */
+    int32_t clip_backscatter; /* clip C to 0 where C Gamma.Estr < 0         */
+    int32_t relax_kind;   /* VPMX_RELAX_*                                   */
+    double rlxf;          /* relaxation factor                              */
+} vpmx_step_params;
+
+/* ---- context ----------------------------------------------------------------- */
+/* No reference counterpart: the reference keeps state in Julia objects. */
+int vpmx_create(vpmx_ctx **out, int32_t device);
+void vpmx_destroy(vpmx_ctx *ctx);
+/* Message of the last failing call on ctx (or of vpmx_create when ctx is NULL). */
+const char *vpmx_last_error(const vpmx_ctx *ctx);
+const char *vpmx_version(void);
+
+/* ---- drop-in entry points (host particle matrix) ----------------------------- */
+/* Replaces FLOWVPM's UJ_direct(pfield) / UJ_fmm(pfield) as called at
+ * src/FLOWUnsteady_simulation.jl:544 and src/FLOWUnsteady_processing_force.jl:238:
+ * U and J of all np particles (static and probes included) induced by all np
+ * particles, coincident pairs skipped, source sigma in g, no freestream.
+ * When params->sfs is set, also writes Estr into the SFS rows.  U, J (and SFS)
+ * rows are valid in `particles` on return. */
+int vpmx_uj(vpmx_ctx *ctx, double *particles, int64_t np, const vpmx_layout *layout,
+            const vpmx_uj_params *params);
+
+/* Replaces FLOWVPM's rungekutta3(pfield, dt; relax) as selected by
+ * vpm_integration (src/FLOWUnsteady_simulation.jl:40,131,242) and called
+ * through vpm.nextstep (src/FLOWUnsteady_simulation.jl:358): three low-storage
+ * stages of UJ (+SFS) and a fused FP64 update of non-static particles, then,
+ * if relax, a fresh UJ and the relaxation.  Uploads once, keeps the state
+ * resident in HBM across the stages, writes X, Gamma, sigma, U, J (and SFS, C)
+ * back into `particles`. */
+int vpmx_rk3_step(vpmx_ctx *ctx, double *particles, int64_t np, const vpmx_layout *layout,
+                  const vpmx_uj_params *uj, const vpmx_step_params *step, double dt,
+                  int32_t relax);
+
+/* Replaces FLOWVPM's euler(pfield, dt; relax) (src/FLOWUnsteady_simulation.jl:40). */
+int vpmx_euler_step(vpmx_ctx *ctx, double *particles, int64_t np, const vpmx_layout *layout,
+                    const vpmx_uj_params *uj, const vpmx_step_params *step, double dt,
+                    int32_t relax);
+
+/* ---- resident state (particles kept in HBM between calls) -------------------- */
+/* Upload np particles from a host matrix into the context's device state. */
+int vpmx_upload(vpmx_ctx *ctx, const double *particles, int64_t np, const vpmx_layout *layout);
+/* Write the device state (X, Gamma, sigma, U, J and, when present in the
+ * layout, SFS and C) back into a host matrix. */
+int vpmx_download(vpmx_ctx *ctx, double *particles, int64_t np, const vpmx_layout *layout);
+/* UJ / time steps on the resident state (same semantics as above). */
+int vpmx_uj_resident(vpmx_ctx *ctx, const vpmx_uj_params *params);
+int vpmx_rk3_resident(vpmx_ctx *ctx, const vpmx_uj_params *uj, const vpmx_step_params *step,
+                      double dt, int32_t relax);
+int vpmx_euler_resident(vpmx_ctx *ctx, const vpmx_uj_params *uj, const vpmx_step_params *step,
+                        double dt, int32_t relax);
+/* Enstrophy 0.5 sum_p Gamma_p . omega(x_p), omega = curl U from the resident J
+ * (vpm.monitor_enstrophy, src/FLOWUnsteady_monitors.jl:573-574,614), and the
+ * energy sum_p U_p . (x_p x Gamma_p). */
+int vpmx_diagnostics(vpmx_ctx *ctx, double *enstrophy, double *energy);
+/* Number of particles currently resident. */
+int64_t vpmx_resident_np(const vpmx_ctx *ctx);
+
+/* ---- instrumentation ----------------------------------------------------------- */
+/* Time every launch of the pairwise kernels with HIP events on the context's
+ * stream.  which: 0 = UJ pair kernel, 1 = Estr pair kernel, 2 = fused update.
+ * Returns the summed milliseconds and the launch count since the last reset. */
+int vpmx_set_timing(vpmx_ctx *ctx, int32_t on);
+int vpmx_get_timing(vpmx_ctx *ctx, int32_t which, double *total_ms, int64_t *count);
+/* The context's HIP stream (hipStream_t), for callers that order work on it. */
+void *vpmx_stream(vpmx_ctx *ctx);
+/* Make the next `count` entry-point calls fail with VPMX_EINJECT before any
+ * device work (error-path testing). */
+int vpmx_inject_fault(vpmx_ctx *ctx, int32_t count);
+/* Host evaluation of the FP32 radial coefficients A = g/r^3 and
+ * B = g'/(sigma r^4) - 3 g/r^5 exactly as the mixed-precision kernel's
+ * near-field branch computes them (for unit tests of their accuracy; the
+ * hardware exp/rcp/rsqrt are replaced by the C library's). */
+int vpmx_radial_f32(int32_t kernel, float r, float sigma, float *A, float *B);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VPMX_H */

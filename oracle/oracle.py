@@ -1,0 +1,218 @@
+"""ctypes wrapper around the CPU FP64 oracle (oracle/vpm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the product package.  Arrays are numpy
+float64, AoS per particle (X shape (n, 3), J shape (n, 9) with
+J[:, k + 3*l] = dU_k/dx_l).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liborc.so")
+
+SINGULAR, GAUSSIAN, GAUSSIANERF, WINCKELMANS = 0, 1, 2, 3
+KERNELS = {"singular": SINGULAR, "gaussian": GAUSSIAN,
+           "gaussianerf": GAUSSIANERF, "winckelmans": WINCKELMANS}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class _Field(ctypes.Structure):
+    _fields_ = [("np", ctypes.c_int64),
+                ("X", _dp), ("G", _dp), ("sigma", _dp), ("U", _dp), ("J", _dp),
+                ("E", _dp), ("C", _dp), ("M", _dp),
+                ("is_static", ctypes.POINTER(ctypes.c_uint8))]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int), ("transposed", ctypes.c_int),
+                ("f", ctypes.c_double), ("g", ctypes.c_double),
+                ("Uinf", ctypes.c_double * 3),
+                ("sfs", ctypes.c_int), ("Cs", ctypes.c_double),
+                ("clip_backscatter", ctypes.c_int),
+                ("relax_kind", ctypes.c_int), ("rlxf", ctypes.c_double)]
+
+
+def build(force: bool = False) -> str:
+    """Compile liborc.so with the committed Makefile (gcc, OpenMP)."""
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liborc.so"],
+                       check=True, stdout=subprocess.DEVNULL)
+    return _LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        L.orc_kernel.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_zeta0.argtypes = [ctypes.c_int]
+        L.orc_zeta0.restype = ctypes.c_double
+        L.orc_uj_direct.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_int64,
+                                    _dp, _dp, _dp, _dp, _dp]
+        L.orc_estr_direct.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _dp, _dp,
+                                      ctypes.c_int64, _dp, _dp, _dp, _dp, _dp]
+        L.orc_stretching.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_rhs.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                              _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp,
+                              _dp, _dp, _dp]
+        L.orc_lsrk3_linear.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       _dp, _dp]
+        L.orc_relax.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.orc_rk3_step.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(_Field),
+                                   ctypes.c_double, ctypes.c_int]
+        L.orc_euler_step.argtypes = L.orc_rk3_step.argtypes
+        L.orc_enstrophy.argtypes = [ctypes.c_int64, _dp, _dp]
+        L.orc_enstrophy.restype = ctypes.c_double
+        L.orc_energy.argtypes = [ctypes.c_int64, _dp, _dp, _dp]
+        L.orc_energy.restype = ctypes.c_double
+        L.orc_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def _kid(kernel):
+    return KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+
+
+def kernel(kernel, rho):
+    z, g, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib().orc_kernel(_kid(kernel), float(rho), ctypes.byref(z), ctypes.byref(g), ctypes.byref(d))
+    return z.value, g.value, d.value
+
+
+def zeta0(kernel):
+    return lib().orc_zeta0(_kid(kernel))
+
+
+def uj_direct(kernel, tX, sX, sG, ssig):
+    """U (nt,3) and J (nt,9) at targets tX induced by sources (sX, sG, ssig)."""
+    tX = _c(tX, (-1, 3)); sX = _c(sX, (-1, 3)); sG = _c(sG, (-1, 3)); ssig = _c(ssig, (-1,))
+    nt, ns = tX.shape[0], sX.shape[0]
+    U = np.zeros((nt, 3)); J = np.zeros((nt, 9))
+    lib().orc_uj_direct(_kid(kernel), nt, _p(tX), ns, _p(sX), _p(sG), _p(ssig), _p(U), _p(J))
+    return U, J
+
+
+def estr_direct(kernel, transposed, tX, tJ, sX, sG, ssig, sJ):
+    tX = _c(tX, (-1, 3)); tJ = _c(tJ, (-1, 9)); sX = _c(sX, (-1, 3)); sG = _c(sG, (-1, 3))
+    ssig = _c(ssig, (-1,)); sJ = _c(sJ, (-1, 9))
+    E = np.zeros((tX.shape[0], 3))
+    lib().orc_estr_direct(_kid(kernel), int(transposed), tX.shape[0], _p(tX), _p(tJ),
+                          sX.shape[0], _p(sX), _p(sG), _p(ssig), _p(sJ), _p(E))
+    return E
+
+
+def stretching(transposed, J, G):
+    J = _c(J, (9,)); G = _c(G, (3,)); S = np.zeros(3)
+    lib().orc_stretching(int(transposed), _p(J), _p(G), _p(S))
+    return S
+
+
+def rhs(transposed, f, g, zeta0_, Uinf, U, J, G, sigma, C=0.0, E=None):
+    Uinf = _c(Uinf, (3,)); U = _c(U, (3,)); J = _c(J, (9,)); G = _c(G, (3,))
+    Ep = None
+    if E is not None:
+        Ea = _c(E, (3,)); Ep = _p(Ea)
+    dX = np.zeros(3); dG = np.zeros(3); ds = ctypes.c_double()
+    lib().orc_rhs(int(transposed), f, g, zeta0_, _p(Uinf), _p(U), _p(J), _p(G), sigma, C,
+                  Ep, _p(dX), _p(dG), ctypes.byref(ds))
+    return dX, dG, ds.value
+
+
+def lsrk3_linear(lam: complex, dt: float, y0: complex = 1.0 + 0j) -> complex:
+    yr, yi = ctypes.c_double(y0.real), ctypes.c_double(y0.imag)
+    lib().orc_lsrk3_linear(lam.real, lam.imag, dt, ctypes.byref(yr), ctypes.byref(yi))
+    return complex(yr.value, yi.value)
+
+
+def relax(kind, rlxf, J, G):
+    J = _c(J, (9,)); G = _c(G, (3,)).copy()
+    lib().orc_relax(int(kind), rlxf, _p(J), _p(G))
+    return G
+
+
+def enstrophy(G, J):
+    G = _c(G, (-1, 3)); J = _c(J, (-1, 9))
+    return lib().orc_enstrophy(G.shape[0], _p(G), _p(J))
+
+
+def energy(X, G, U):
+    X = _c(X, (-1, 3)); G = _c(G, (-1, 3)); U = _c(U, (-1, 3))
+    return lib().orc_energy(X.shape[0], _p(X), _p(G), _p(U))
+
+
+class Field:
+    """A particle field for the oracle integrators (AoS float64 arrays)."""
+
+    def __init__(self, X, G, sigma, is_static=None):
+        self.X = _c(X, (-1, 3)).copy()
+        self.G = _c(G, (-1, 3)).copy()
+        self.sigma = _c(sigma, (-1,)).copy()
+        n = self.X.shape[0]
+        self.U = np.zeros((n, 3)); self.J = np.zeros((n, 9)); self.E = np.zeros((n, 3))
+        self.C = np.zeros(n); self.M = np.zeros((n, 7))
+        self.is_static = None if is_static is None else np.ascontiguousarray(is_static, np.uint8)
+
+    def _struct(self):
+        F = _Field()
+        F.np = self.X.shape[0]
+        F.X, F.G, F.sigma = _p(self.X), _p(self.G), _p(self.sigma)
+        F.U, F.J, F.E, F.C, F.M = _p(self.U), _p(self.J), _p(self.E), _p(self.C), _p(self.M)
+        F.is_static = (self.is_static.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+                       if self.is_static is not None else None)
+        return F
+
+
+def make_params(kernel="gaussianerf", transposed=True, formulation="rVPM", Uinf=(0, 0, 0),
+                sfs=False, Cs=0.0, clip_backscatter=True, relax_kind=0, rlxf=0.3):
+    P = _Params()
+    P.kernel = _kid(kernel)
+    P.transposed = int(transposed)
+    if formulation == "rVPM":
+        P.f, P.g = 0.0, 0.2
+    elif formulation == "cVPM":
+        P.f, P.g = 0.0, 0.0
+    else:
+        P.f, P.g = formulation
+    for k in range(3):
+        P.Uinf[k] = float(Uinf[k])
+    P.sfs = int(bool(sfs)); P.Cs = float(Cs); P.clip_backscatter = int(clip_backscatter)
+    P.relax_kind = int(relax_kind); P.rlxf = float(rlxf)
+    return P
+
+
+def rk3_step(params, field: Field, dt: float, relax: bool = False):
+    F = field._struct()
+    lib().orc_rk3_step(ctypes.byref(params), ctypes.byref(F), dt, int(relax))
+
+
+def euler_step(params, field: Field, dt: float, relax: bool = False):
+    F = field._struct()
+    lib().orc_euler_step(ctypes.byref(params), ctypes.byref(F), dt, int(relax))
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()

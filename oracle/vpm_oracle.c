@@ -1,0 +1,418 @@
+/*
+ * vpm_oracle.c -- CPU FP64 restatement of the rVPM particle-interaction step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it.  The
+ * product (flowunsteady_amd/, libvpmx.so) never links or calls it.
+ *
+ * The path it restates is FLOWVPM's UJ_direct / Estr_direct / rungekutta3 /
+ * euler / relaxation, which FLOWUnsteady drives through
+ *   vpm.ParticleField(...; kernel, UJ, SFS, integration, transposed,
+ *                     relaxation)            src/FLOWUnsteady_simulation.jl:239-253
+ *   vpm.nextstep(pfield, dt; relax)          src/FLOWUnsteady_simulation.jl:358
+ *   pfield.UJ(pfield)                        src/FLOWUnsteady_simulation.jl:544
+ * FLOWVPM itself is not vendored (Project.toml:25 pins FLOWVPM = "4.0"), so
+ * every formula below follows the math the reference documents in
+ * docs/src/theory/rvpm.md and the pair-loop form of
+ * src/FLOWUnsteady_processing_force.jl:879-929.
+ *
+ * Layout: particle arrays are AoS per particle, e.g. X[3*i+k], J[9*i+k+3*l]
+ * with J[k,l] = dU_k/dx_l (column-major 3x3, as a Julia 3x3 matrix).
+ *
+ * Pins (each checked by tests/test_oracle.py):
+ *   orc_kernel        -- closed forms vs quadrature g(rho)=int_0^rho 4 pi s^2 zeta ds
+ *                        (tests/golden/kernels.json), g(0)=0, g(inf)=1.
+ *   orc_uj_direct     -- vs an independent 50-digit vector-potential route
+ *                        u = curl sum_j Gamma_j G_sigma(x-x_j) with mpmath
+ *                        numerical derivatives (tests/golden/uj_*.json); J vs
+ *                        central differences of U; singular ring on-axis closed
+ *                        form; trace(J)=0.
+ *   orc_estr_direct   -- vs golden Estr built from the golden J; transposed vs
+ *                        classic convention vs finite differences of U.
+ *   orc_rhs           -- |Gamma| sigma^2 invariant of rVPM (f=0, g=1/5);
+ *                        classic/transposed stretching vs finite differences.
+ *   orc_lsrk3_linear  -- one step of y'=lambda*y equals 1+z+z^2/2+z^3/6.
+ *   orc_rk3_step      -- third-order convergence on a vortex ring; uniform
+ *                        free-stream translation is exact.
+ *   orc_relax         -- rlxf=0 identity, rlxf=1 gives |Gamma| omega_hat,
+ *                        corrected variant preserves |Gamma|, omega vs curl of U
+ *                        by finite differences.
+ *   orc_enstrophy     -- 0.5 sum Gamma.omega (src/FLOWUnsteady_monitors.jl:573-574).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* 1/(4 pi) exactly as the reference pair loop writes it
+ * (src/FLOWUnsteady_processing_force.jl:900-902). */
+#define ORC_CONST4 0.07957747154594767
+#define ORC_PI 3.14159265358979323846
+
+enum { ORC_SINGULAR = 0, ORC_GAUSSIAN = 1, ORC_GAUSSIANERF = 2, ORC_WINCKELMANS = 3 };
+
+/* Radial basis zeta(rho), regularising function g(rho) and g'(rho).
+ * docs/src/theory/rvpm.md:84 (zeta_sigma = sigma^-3 zeta(r/sigma)),
+ * rvpm.md:89-97 (g_sigma in the regularised Biot-Savart law),
+ * rvpm.md:365 (Gaussian filter), rvpm.md:380 (Winckelmans & Leonard 1993).
+ * All satisfy g'(rho) = 4 pi rho^2 zeta(rho). */
+void orc_kernel(int kernel, double rho, double *zeta, double *g, double *dg)
+{
+    double z = 0.0, gg = 1.0, d = 0.0;
+    switch (kernel) {
+    case ORC_SINGULAR:
+        z = 0.0; gg = 1.0; d = 0.0;
+        break;
+    case ORC_GAUSSIAN: {
+        double e = exp(-rho * rho * rho);
+        z = 3.0 / (4.0 * ORC_PI) * e;
+        gg = 1.0 - e;
+        d = 3.0 * rho * rho * e;
+        break;
+    }
+    case ORC_GAUSSIANERF: {
+        const double c2 = 0.7978845608028654;   /* sqrt(2/pi) */
+        double e = exp(-rho * rho / 2.0);
+        z = 1.0 / pow(2.0 * ORC_PI, 1.5) * e;
+        gg = erf(rho / 1.4142135623730951) - c2 * rho * e;
+        d = c2 * rho * rho * e;
+        break;
+    }
+    case ORC_WINCKELMANS: {
+        double q = rho * rho + 1.0;
+        double q52 = pow(q, 2.5), q72 = pow(q, 3.5);
+        z = 15.0 / (8.0 * ORC_PI) / q72;
+        gg = rho * rho * rho * (rho * rho + 2.5) / q52;
+        d = 7.5 * rho * rho / q72;
+        break;
+    }
+    default:
+        z = NAN; gg = NAN; d = NAN;
+    }
+    if (zeta) *zeta = z;
+    if (g) *g = gg;
+    if (dg) *dg = d;
+}
+
+double orc_zeta0(int kernel)
+{
+    double z;
+    if (kernel == ORC_SINGULAR) return 1.0;   /* unused: SFS needs a smooth kernel */
+    orc_kernel(kernel, 0.0, &z, NULL, NULL);
+    return z;
+}
+
+/* One source j acting on one target: accumulates U[3] and J[9].
+ * Pair algebra of src/FLOWUnsteady_processing_force.jl:888-905 (dX = x_t - x_s,
+ * skip r == 0, source sigma in g, 1/(4 pi) constant), extended to
+ * J[k,l] = dU_k/dx_l by the product rule:
+ *   J += (g'/(sigma r) - 3 g / r^2) * crss_k * dX_l   (radial part)
+ *      + (-g/(4 pi r^3)) * eps_{klm} Gamma_m          (Kronecker part)
+ * with crss = -(1/(4 pi r^3)) dX x Gamma. */
+static inline void pair_uj(int kernel, const double *xt, const double *xs, const double *gs,
+                           double sgm, double *U, double *J)
+{
+    double dX1 = xt[0] - xs[0], dX2 = xt[1] - xs[1], dX3 = xt[2] - xs[2];
+    double r2 = dX1 * dX1 + dX2 * dX2 + dX3 * dX3;
+    if (r2 == 0.0) return;
+    double r = sqrt(r2);
+    double g, dg;
+    orc_kernel(kernel, r / sgm, NULL, &g, &dg);
+    double r3 = r * r2;
+    double c1 = -ORC_CONST4 / r3 * (dX2 * gs[2] - dX3 * gs[1]);
+    double c2 = -ORC_CONST4 / r3 * (dX3 * gs[0] - dX1 * gs[2]);
+    double c3 = -ORC_CONST4 / r3 * (dX1 * gs[1] - dX2 * gs[0]);
+    U[0] += g * c1;
+    U[1] += g * c2;
+    U[2] += g * c3;
+    double aux = dg / (sgm * r) - 3.0 * g / r2;
+    /* column l = derivative direction */
+    J[0] += aux * c1 * dX1; J[1] += aux * c2 * dX1; J[2] += aux * c3 * dX1;
+    J[3] += aux * c1 * dX2; J[4] += aux * c2 * dX2; J[5] += aux * c3 * dX2;
+    J[6] += aux * c1 * dX3; J[7] += aux * c2 * dX3; J[8] += aux * c3 * dX3;
+    double a2 = -ORC_CONST4 * g / r3;
+    /* eps_{klm} Gamma_m: (k,l) = (2,1):-G3 (3,1):+G2 (1,2):+G3 (3,2):-G1 (1,3):-G2 (2,3):+G1 */
+    J[1] -= a2 * gs[2];
+    J[2] += a2 * gs[1];
+    J[3] += a2 * gs[2];
+    J[5] -= a2 * gs[0];
+    J[6] -= a2 * gs[1];
+    J[7] += a2 * gs[0];
+}
+
+/* UJ_direct: U and J at every target from every source (accumulating; the
+ * caller zeroes first, as _reset_particles does at
+ * src/FLOWUnsteady_processing_force.jl:237).  Sources and targets may be the
+ * same arrays; coincident pairs are skipped. */
+void orc_uj_direct(int kernel, int64_t nt, const double *tX, int64_t ns, const double *sX,
+                   const double *sG, const double *ssig, double *U, double *J)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nt; i++) {
+        double u[3] = {0, 0, 0}, jj[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int64_t j = 0; j < ns; j++)
+            pair_uj(kernel, tX + 3 * i, sX + 3 * j, sG + 3 * j, ssig[j], u, jj);
+        for (int k = 0; k < 3; k++) U[3 * i + k] += u[k];
+        for (int k = 0; k < 9; k++) J[9 * i + k] += jj[k];
+    }
+}
+
+/* Vortex stretching from J (docs/src/theory/rvpm.md:105-178 and :363):
+ *   classic     S_k = sum_l J[k,l] Gamma_l   ((Gamma . grad) u)
+ *   transposed  S_k = sum_l J[l,k] Gamma_l   ((Gamma . grad^T) u), the default
+ *               (vpm_transposed = true, src/FLOWUnsteady_simulation.jl:41). */
+void orc_stretching(int transposed, const double *J, const double *G, double *S)
+{
+    for (int k = 0; k < 3; k++) {
+        double s = 0.0;
+        for (int l = 0; l < 3; l++)
+            s += (transposed ? J[l + 3 * k] : J[k + 3 * l]) * G[l];
+        S[k] = s;
+    }
+}
+
+/* Estr_direct: SFS model of vortex stretching (docs/src/theory/rvpm.md:251-262)
+ *   E(x_i) = sum_j zeta_sigma_j(x_i - x_j) (Gamma_j . grad)(U(x_i) - U(x_j)),
+ * transposed or classic as above.  Accumulates into E. */
+void orc_estr_direct(int kernel, int transposed, int64_t nt, const double *tX, const double *tJ,
+                     int64_t ns, const double *sX, const double *sG, const double *ssig,
+                     const double *sJ, double *E)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nt; i++) {
+        double e[3] = {0, 0, 0};
+        for (int64_t j = 0; j < ns; j++) {
+            double d1 = tX[3 * i] - sX[3 * j], d2 = tX[3 * i + 1] - sX[3 * j + 1],
+                   d3 = tX[3 * i + 2] - sX[3 * j + 2];
+            double r = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+            double sg = ssig[j], z;
+            orc_kernel(kernel, r / sg, &z, NULL, NULL);
+            double zs = z / (sg * sg * sg);
+            double dJ[9], S[3];
+            for (int k = 0; k < 9; k++) dJ[k] = tJ[9 * i + k] - sJ[9 * j + k];
+            orc_stretching(transposed, dJ, sG + 3 * j, S);
+            for (int k = 0; k < 3; k++) e[k] += zs * S[k];
+        }
+        for (int k = 0; k < 3; k++) E[3 * i + k] += e[k];
+    }
+}
+
+/* Right-hand side of the rVPM governing equations for one particle
+ * (docs/src/theory/rvpm.md:107-178 with the general f, g of :196-236):
+ *   Z       = [ (g+f)/(1+3f) S.Gamma - f/(1+3f) (C sigma^3/zeta0) E.Gamma ] / |Gamma|^2
+ *   dX/dt   = U + Uinf
+ *   dG/dt   = S - 3 Z Gamma - (C sigma^3/zeta0) E
+ *   dsg/dt  = -sigma Z
+ * rVPM is f = 0, g = 1/5; classic VPM is f = g = 0.  A particle with
+ * |Gamma| = 0 gets Z = 0 (probes carry Gamma = 0, simulation.jl:572). */
+void orc_rhs(int transposed, double f, double gcoef, double zeta0, const double *Uinf,
+             const double *U, const double *J, const double *G, double sigma, double C,
+             const double *E, double *dX, double *dG, double *dsigma)
+{
+    double S[3];
+    orc_stretching(transposed, J, G, S);
+    double G2 = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+    double csfs = C * sigma * sigma * sigma / zeta0;
+    double e0 = E ? E[0] : 0.0, e1 = E ? E[1] : 0.0, e2 = E ? E[2] : 0.0;
+    double SG = S[0] * G[0] + S[1] * G[1] + S[2] * G[2];
+    double EG = e0 * G[0] + e1 * G[1] + e2 * G[2];
+    double Z = 0.0;
+    if (G2 > 0.0)
+        Z = ((gcoef + f) / (1.0 + 3.0 * f) * SG - f / (1.0 + 3.0 * f) * csfs * EG) / G2;
+    for (int k = 0; k < 3; k++) dX[k] = U[k] + Uinf[k];
+    dG[0] = S[0] - 3.0 * Z * G[0] - csfs * e0;
+    dG[1] = S[1] - 3.0 * Z * G[1] - csfs * e1;
+    dG[2] = S[2] - 3.0 * Z * G[2] - csfs * e2;
+    *dsigma = -sigma * Z;
+}
+
+/* Williamson (1980) low-storage 3-stage RK coefficients; the scheme is the
+ * low-storage third-order RK of docs/src/theory/rvpm.md:364. */
+static const double LSRK3_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+static const double LSRK3_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+
+/* One step of the same scheme applied to y' = lambda y (complex lambda given as
+ * re/im): the closed-form pin is 1 + z + z^2/2 + z^3/6 with z = lambda dt. */
+void orc_lsrk3_linear(double lre, double lim, double dt, double *yre, double *yim)
+{
+    double qr = 0.0, qi = 0.0, yr = *yre, yi = *yim;
+    for (int s = 0; s < 3; s++) {
+        double fr = lre * yr - lim * yi, fi = lre * yi + lim * yr;
+        qr = LSRK3_A[s] * qr + dt * fr;
+        qi = LSRK3_A[s] * qi + dt * fi;
+        yr += LSRK3_B[s] * qr;
+        yi += LSRK3_B[s] * qi;
+    }
+    *yre = yr;
+    *yim = yi;
+}
+
+/* Pedrizzetti relaxation (docs/src/theory/rvpm.md:363, Pedrizzetti 1992):
+ *   Gamma <- (1 - rlxf) Gamma + rlxf |Gamma| omega / |omega|
+ * with omega = curl U from J: (J[3,2]-J[2,3], J[1,3]-J[3,1], J[2,1]-J[1,2]).
+ * kind 1 = pedrizzetti, kind 2 = corrected (rescaled to keep |Gamma|). */
+void orc_relax(int kind, double rlxf, const double *J, double *G)
+{
+    if (kind == 0) return;
+    double w1 = J[5] - J[7], w2 = J[6] - J[2], w3 = J[1] - J[3];
+    double nw = sqrt(w1 * w1 + w2 * w2 + w3 * w3);
+    double nG = sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
+    if (nw == 0.0) return;
+    double cosang = (G[0] * w1 + G[1] * w2 + G[2] * w3) / (nG * nw);
+    G[0] = (1.0 - rlxf) * G[0] + rlxf * nG * w1 / nw;
+    G[1] = (1.0 - rlxf) * G[1] + rlxf * nG * w2 / nw;
+    G[2] = (1.0 - rlxf) * G[2] + rlxf * nG * w3 / nw;
+    if (kind == 2) {
+        double b2 = 1.0 - 2.0 * (1.0 - rlxf) * rlxf * (1.0 - cosang);
+        double b = sqrt(b2);
+        G[0] /= b; G[1] /= b; G[2] /= b;
+    }
+}
+
+/* Particle field state for the integrators: AoS per particle. */
+typedef struct {
+    int64_t np;
+    double *X, *G, *sigma, *U, *J, *E, *C, *M; /* M: 7 registers per particle */
+    const uint8_t *is_static;                  /* may be NULL: no static particles */
+} orc_field;
+
+typedef struct {
+    int kernel, transposed;
+    double f, g;               /* formulation: rVPM f=0 g=1/5, cVPM f=g=0 */
+    double Uinf[3];
+    int sfs;                   /* 0 none, 1 constant Cs with Estr_direct */
+    double Cs;
+    int clip_backscatter;
+    int relax_kind;            /* 0 none, 1 pedrizzetti, 2 corrected */
+    double rlxf;
+} orc_params;
+
+static void field_uj(const orc_params *P, orc_field *F)
+{
+    memset(F->U, 0, sizeof(double) * 3 * F->np);
+    memset(F->J, 0, sizeof(double) * 9 * F->np);
+    orc_uj_direct(P->kernel, F->np, F->X, F->np, F->X, F->G, F->sigma, F->U, F->J);
+}
+
+/* Constant-coefficient SFS after UJ: E = Estr_direct, C = Cs, then backscatter
+ * clipping C = 0 where C Gamma.E < 0 (docs/src/theory/rvpm.md:296). */
+static void field_sfs(const orc_params *P, orc_field *F)
+{
+    if (!P->sfs) return;
+    memset(F->E, 0, sizeof(double) * 3 * F->np);
+    orc_estr_direct(P->kernel, P->transposed, F->np, F->X, F->J, F->np, F->X, F->G, F->sigma,
+                    F->J, F->E);
+    for (int64_t i = 0; i < F->np; i++) {
+        double c = P->Cs;
+        if (P->clip_backscatter) {
+            const double *g = F->G + 3 * i, *e = F->E + 3 * i;
+            if (c * (g[0] * e[0] + g[1] * e[1] + g[2] * e[2]) < 0.0) c = 0.0;
+        }
+        F->C[i] = c;
+    }
+}
+
+static void field_relax(const orc_params *P, orc_field *F)
+{
+    field_uj(P, F);
+    for (int64_t i = 0; i < F->np; i++) {
+        if (F->is_static && F->is_static[i]) continue;
+        orc_relax(P->relax_kind, P->rlxf, F->J + 9 * i, F->G + 3 * i);
+    }
+}
+
+/* rungekutta3: three low-storage stages of (UJ, SFS, update of non-static
+ * particles), then optional relaxation after a fresh UJ.  Stage structure per
+ * vpm.nextstep (src/FLOWUnsteady_simulation.jl:358); relax cadence per
+ * src/FLOWUnsteady_simulation.jl:346-348. */
+void orc_rk3_step(const orc_params *P, orc_field *F, double dt, int relax)
+{
+    double zeta0 = orc_zeta0(P->kernel);
+    memset(F->M, 0, sizeof(double) * 7 * F->np);
+    for (int s = 0; s < 3; s++) {
+        double a = LSRK3_A[s], b = LSRK3_B[s];
+        field_uj(P, F);
+        field_sfs(P, F);
+        for (int64_t i = 0; i < F->np; i++) {
+            if (F->is_static && F->is_static[i]) continue;
+            double dX[3], dG[3], ds;
+            orc_rhs(P->transposed, P->f, P->g, zeta0, P->Uinf, F->U + 3 * i, F->J + 9 * i,
+                    F->G + 3 * i, F->sigma[i], P->sfs ? F->C[i] : 0.0,
+                    P->sfs ? F->E + 3 * i : NULL, dX, dG, &ds);
+            double *M = F->M + 7 * i;
+            for (int k = 0; k < 3; k++) {
+                M[k] = a * M[k] + dt * dX[k];
+                F->X[3 * i + k] += b * M[k];
+            }
+            for (int k = 0; k < 3; k++) {
+                M[3 + k] = a * M[3 + k] + dt * dG[k];
+                F->G[3 * i + k] += b * M[3 + k];
+            }
+            M[6] = a * M[6] + dt * ds;
+            F->sigma[i] += b * M[6];
+        }
+    }
+    if (relax && P->relax_kind) field_relax(P, F);
+}
+
+/* euler: one UJ (+SFS), forward-Euler update, then relaxation with that same J
+ * when requested. */
+void orc_euler_step(const orc_params *P, orc_field *F, double dt, int relax)
+{
+    double zeta0 = orc_zeta0(P->kernel);
+    field_uj(P, F);
+    field_sfs(P, F);
+    for (int64_t i = 0; i < F->np; i++) {
+        if (F->is_static && F->is_static[i]) continue;
+        double dX[3], dG[3], ds;
+        orc_rhs(P->transposed, P->f, P->g, zeta0, P->Uinf, F->U + 3 * i, F->J + 9 * i,
+                F->G + 3 * i, F->sigma[i], P->sfs ? F->C[i] : 0.0, P->sfs ? F->E + 3 * i : NULL,
+                dX, dG, &ds);
+        for (int k = 0; k < 3; k++) {
+            F->X[3 * i + k] += dt * dX[k];
+            F->G[3 * i + k] += dt * dG[k];
+        }
+        F->sigma[i] += dt * ds;
+        if (relax && P->relax_kind)
+            orc_relax(P->relax_kind, P->rlxf, F->J + 9 * i, F->G + 3 * i);
+    }
+}
+
+/* Enstrophy 0.5 sum_p Gamma_p . omega(x_p), omega from J
+ * (src/FLOWUnsteady_monitors.jl:573-574). */
+double orc_enstrophy(int64_t np, const double *G, const double *J)
+{
+    double s = 0.0;
+    for (int64_t i = 0; i < np; i++) {
+        const double *j = J + 9 * i, *g = G + 3 * i;
+        s += g[0] * (j[5] - j[7]) + g[1] * (j[6] - j[2]) + g[2] * (j[1] - j[3]);
+    }
+    return 0.5 * s;
+}
+
+/* Kinetic energy in Saffman's impulse form, E = sum_p U(x_p) . (x_p x Gamma_p).
+ * The reference defines no energy monitor; this definition is this oracle's. */
+double orc_energy(int64_t np, const double *X, const double *G, const double *U)
+{
+    double s = 0.0;
+    for (int64_t i = 0; i < np; i++) {
+        const double *x = X + 3 * i, *g = G + 3 * i, *u = U + 3 * i;
+        double c0 = x[1] * g[2] - x[2] * g[1], c1 = x[2] * g[0] - x[0] * g[2],
+               c2 = x[0] * g[1] - x[1] * g[0];
+        s += u[0] * c0 + u[1] * c1 + u[2] * c2;
+    }
+    return s;
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

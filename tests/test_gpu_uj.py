@@ -1,0 +1,234 @@
+"""GPU parity of the UJ evaluation (vpmx_uj through the C ABI) against the
+FP64 oracle and the 40-digit golden vectors.
+
+Tolerances (normwise: max_i |dU_i| / max_j |U_j|):
+  FP64 path   <= 1e-12 against the oracle (different summation order only)
+  mixed path  <= 1e-6 for U and <= 3e-6 for J (FP32 pair math, FP64 folds)
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from flowunsteady_amd import _native as N
+from flowunsteady_amd import vpm, workloads
+from helpers import make_field, normwise, sample_targets
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": (1e-12, 1e-12), "mixed": (1e-6, 3e-6)}
+
+
+def run_uj(pf, **kw):
+    vpm.UJ_gpu(pf, **kw)
+    return vpm.get_U(pf).copy(), vpm.get_J(pf).copy()
+
+
+def oracle_uj(kernel, X, G, s, idx=None):
+    T = X if idx is None else X[idx]
+    return O.uj_direct(kernel, T, X, G, s)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+@pytest.mark.parametrize("case", ["spread", "cluster"])
+def test_golden(golden_dir, precision, kernel, case):
+    with open(os.path.join(golden_dir, f"uj_{kernel}_{case}.json")) as fh:
+        d = json.load(fh)
+    X, G, s = (np.array(d[k]) for k in ("X", "Gamma", "sigma"))
+    pf = make_field(X, G, s, kernel=kernel, precision=precision)
+    U, J = run_uj(pf)
+    tu, tj = TOL[precision]
+    assert normwise(U, d["U"])[0] < max(tu, 1e-13)
+    assert normwise(J, d["J"])[0] < max(tj, 1e-13)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans", "gaussian", "singular"])
+def test_vortex_ring(precision, kernel):
+    X, G, s, v = workloads.vortex_ring()          # N = 8000
+    pf = make_field(X, G, s, kernel=kernel, precision=precision, vol=v)
+    U, J = run_uj(pf)
+    idx = sample_targets(len(X), 1500, seed=1)
+    Ur, Jr = oracle_uj(kernel, X, G, s, idx)
+    tu, tj = TOL[precision]
+    eu, ej = normwise(U[idx], Ur), normwise(J[idx], Jr)
+    print(f"ring {kernel} {precision}: U max/rms {eu}, J max/rms {ej}")
+    assert eu[0] < tu and eu[1] < tu
+    assert ej[0] < tj and ej[1] < tj
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_uniform_cube_subset(precision, kernel):
+    X, G, s, v = workloads.uniform_cube(n=40000, seed=2)
+    pf = make_field(X, G, s, kernel=kernel, precision=precision)
+    U, J = run_uj(pf)
+    idx = sample_targets(len(X), 1024, seed=2)
+    Ur, Jr = oracle_uj(kernel, X, G, s, idx)
+    eu, ej = normwise(U[idx], Ur), normwise(J[idx], Jr)
+    print(f"cube40k {kernel} {precision}: U {eu}, J {ej}")
+    tu, tj = TOL[precision]
+    assert eu[0] < tu and ej[0] < tj
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_rotor_wake_subset(precision):
+    X, G, s, v = workloads.rotor_wake(n=1 << 15, seed=3)
+    pf = make_field(X, G, s, precision=precision)
+    U, J = run_uj(pf)
+    idx = sample_targets(len(X), 1024, seed=3)
+    Ur, Jr = oracle_uj("gaussianerf", X, G, s, idx)
+    eu, ej = normwise(U[idx], Ur), normwise(J[idx], Jr)
+    print(f"rotor32k {precision}: U {eu}, J {ej}")
+    tu, tj = TOL[precision]
+    assert eu[0] < tu and ej[0] < tj
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n", [1, 2, 255, 256, 257, 1000])
+def test_ragged_sizes(precision, n):
+    X, G, s, _ = workloads.uniform_cube(n=n, seed=10 + n, smin=0.2, smax=0.5)
+    pf = make_field(X, G, s, precision=precision)
+    U, J = run_uj(pf)
+    Ur, Jr = oracle_uj("gaussianerf", X, G, s)
+    if n == 1:
+        assert np.all(U == 0) and np.all(J == 0)
+        return
+    tu, tj = TOL[precision]
+    assert normwise(U, Ur)[0] < tu and normwise(J, Jr)[0] < tj
+
+
+def test_empty_field():
+    pf = make_field(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0), extra=4)
+    vpm.UJ_gpu(pf)
+    assert vpm.get_np(pf) == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_accumulate_without_reset(precision):
+    X, G, s, _ = workloads.uniform_cube(n=3000, seed=4, smin=0.1, smax=0.2)
+    pf = make_field(X, G, s, precision=precision)
+    U1, J1 = run_uj(pf)
+    U2, J2 = run_uj(pf, reset=False)
+    np.testing.assert_allclose(U2, 2 * U1, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(J2, 2 * J1, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_probes_and_static(precision):
+    """Probes (Gamma = 0, sigma = 1e-6, simulation.jl:572) receive U and
+    induce nothing; static particles act as sources and receive U too."""
+    X, G, s, _ = workloads.vortex_ring(n_stations=20, n_cross=30)
+    rng = np.random.default_rng(8)
+    P = rng.uniform(-1.5, 1.5, (300, 3))
+    st = np.zeros(len(X))
+    st[::5] = 1.0
+    allX = np.concatenate([X, P])
+    allG = np.concatenate([G, np.zeros_like(P)])
+    alls = np.concatenate([s, np.full(len(P), 1e-6)])
+    pf = make_field(allX, allG, alls, precision=precision,
+                    static=np.concatenate([st, np.zeros(len(P))]))
+    U, J = run_uj(pf)
+    Ur, Jr = O.uj_direct("gaussianerf", allX, X, G, s)
+    tu, tj = TOL[precision]
+    assert normwise(U, Ur)[0] < tu
+    assert normwise(J, Jr)[0] < tj
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_coincident_particles(precision):
+    X, G, s, _ = workloads.uniform_cube(n=500, seed=6, smin=0.1, smax=0.3)
+    X[100] = X[7]
+    X[200] = X[7]
+    pf = make_field(X, G, s, precision=precision)
+    U, J = run_uj(pf)
+    Ur, Jr = oracle_uj("gaussianerf", X, G, s)
+    assert np.all(np.isfinite(U)) and np.all(np.isfinite(J))
+    tu, tj = TOL[precision]
+    assert normwise(U, Ur)[0] < tu and normwise(J, Jr)[0] < tj
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_deterministic(precision):
+    X, G, s, _ = workloads.uniform_cube(n=20000, seed=7)
+    pf = make_field(X, G, s, precision=precision)
+    U1, J1 = run_uj(pf)
+    U2, J2 = run_uj(pf)
+    assert np.array_equal(U1, U2) and np.array_equal(J1, J2)
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_estr_vs_oracle(kernel):
+    X, G, s, _ = workloads.vortex_ring(n_stations=30, n_cross=60)
+    for transposed in (True, False):
+        pf = make_field(X, G, s, kernel=kernel, transposed=transposed)
+        vpm.UJ_gpu(pf, sfs=True)
+        E = vpm.get_SFS(pf).copy()
+        Ur, Jr = O.uj_direct(kernel, X, X, G, s)
+        Er = O.estr_direct(kernel, transposed, X, Jr, X, G, s, Jr)
+        assert normwise(E, Er)[0] < 1e-11
+
+
+def _ctx_call(fn, *args):
+    ctx = vpm._context(0)
+    return fn(ctx.ptr, *args), N.last_error(ctx.ptr)
+
+
+def test_error_paths():
+    import ctypes
+    X, G, s, _ = workloads.uniform_cube(n=64, seed=9)
+    pf = make_field(X, G, s)
+    L = N.lib()
+    P = pf._uj_params()
+    # sigma <= 0
+    pf.particles[vpm.SIGMA_INDEX, 5] = 0.0
+    code, msg = _ctx_call(L.vpmx_uj, pf._ptr(), pf.np, ctypes.byref(pf.layout), ctypes.byref(P))
+    assert code == N.EINVAL and "sigma" in msg
+    pf.particles[vpm.SIGMA_INDEX, 5] = 0.03
+    # NaN circulation
+    pf.particles[vpm.GAMMA_INDEX.start, 9] = np.nan
+    code, msg = _ctx_call(L.vpmx_uj, pf._ptr(), pf.np, ctypes.byref(pf.layout), ctypes.byref(P))
+    assert code == N.EINVAL and "non-finite" in msg
+    pf.particles[vpm.GAMMA_INDEX.start, 9] = G[9, 0]
+    # bad layout
+    bad = vpm.default_layout()
+    bad.J = 40
+    code, msg = _ctx_call(L.vpmx_uj, pf._ptr(), pf.np, ctypes.byref(bad), ctypes.byref(P))
+    assert code == N.EINVAL and "layout" in msg
+    # negative np, unknown kernel, unsupported fmm
+    code, _ = _ctx_call(L.vpmx_uj, pf._ptr(), -1, ctypes.byref(pf.layout), ctypes.byref(P))
+    assert code == N.EINVAL
+    P2 = pf._uj_params()
+    P2.kernel = 9
+    code, _ = _ctx_call(L.vpmx_uj, pf._ptr(), pf.np, ctypes.byref(pf.layout), ctypes.byref(P2))
+    assert code == N.EINVAL
+    P2 = pf._uj_params()
+    P2.fmm = 1
+    code, _ = _ctx_call(L.vpmx_uj, pf._ptr(), pf.np, ctypes.byref(pf.layout), ctypes.byref(P2))
+    assert code == N.EUNSUPPORTED
+    # injected fault, then recovery
+    ctx = vpm._context(0)
+    assert L.vpmx_inject_fault(ctx.ptr, 1) == 0
+    code, msg = _ctx_call(L.vpmx_uj, pf._ptr(), pf.np, ctypes.byref(pf.layout), ctypes.byref(P))
+    assert code == N.EINJECT
+    vpm.UJ_gpu(pf)
+    Ur, _ = O.uj_direct("gaussianerf", X, X, G, s)
+    assert normwise(vpm.get_U(pf), Ur)[0] < 1e-12
+    with pytest.raises(N.VPMXError):
+        pf.particles[vpm.SIGMA_INDEX, 0] = -1.0
+        vpm.UJ_gpu(pf)
+
+
+def test_timing_counts_launches():
+    X, G, s, _ = workloads.uniform_cube(n=5000, seed=12)
+    pf = make_field(X, G, s, precision="mixed")
+    ctx = vpm._context(0)
+    ctx.set_timing(True)
+    vpm.UJ_gpu(pf)
+    vpm.UJ_gpu(pf)
+    ms, n = ctx.timing(N.T_UJ)
+    ctx.set_timing(False)
+    assert n == 2 and ms > 0.0
