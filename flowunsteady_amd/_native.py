@@ -23,7 +23,9 @@ RELAX_NONE, RELAX_PEDRIZZETTI, RELAX_CORRECTED = 0, 1, 2
 # SFS
 SFS_NONE, SFS_CONSTANT = 0, 1
 # timing classes
-T_UJ, T_ESTR, T_UPDATE = 0, 1, 2
+T_UJ, T_ESTR, T_UPDATE, T_FMM = 0, 1, 2, 3
+FMM_STATS = ("ms_tree", "ms_traverse", "ms_upload", "cells", "leaves", "m2l_pairs",
+             "p2p_leaf_pairs", "p2p_pairs", "ms_eval")
 
 
 class Layout(ctypes.Structure):
@@ -37,7 +39,8 @@ class UJParams(ctypes.Structure):
     _fields_ = [("kernel", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("transposed", ctypes.c_int32), ("reset", ctypes.c_int32),
                 ("sfs", ctypes.c_int32), ("fmm", ctypes.c_int32), ("p", ctypes.c_int32),
-                ("ncrit", ctypes.c_int32), ("theta", ctypes.c_double)]
+                ("ncrit", ctypes.c_int32), ("theta", ctypes.c_double),
+                ("sigma_cut", ctypes.c_double)]
 
 
 class StepParams(ctypes.Structure):
@@ -80,11 +83,17 @@ SIGNATURES = {
                                            ctypes.POINTER(StepParams), ctypes.c_double,
                                            ctypes.c_int32]),
     "vpmx_diagnostics": (ctypes.c_int, [_ctxp, _dp, _dp]),
+    "vpmx_uj_targets": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64, _dp, _dp,
+                                       ctypes.POINTER(UJParams)]),
     "vpmx_resident_np": (ctypes.c_int64, [_ctxp]),
     "vpmx_set_timing": (ctypes.c_int, [_ctxp, ctypes.c_int32]),
     "vpmx_get_timing": (ctypes.c_int, [_ctxp, ctypes.c_int32, _dp,
                                        ctypes.POINTER(ctypes.c_int64)]),
     "vpmx_stream": (ctypes.c_void_p, [_ctxp]),
+    "vpmx_fmm_stats": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int32]),
+    "vpmx_fmm_plan_counts": (ctypes.c_int, [ctypes.c_int64, _dp, _dp, ctypes.c_int32,
+                                            ctypes.c_double, ctypes.c_double,
+                                            ctypes.POINTER(ctypes.c_int64)]),
     "vpmx_inject_fault": (ctypes.c_int, [_ctxp, ctypes.c_int32]),
     "vpmx_radial_f32": (ctypes.c_int, [ctypes.c_int32, ctypes.c_float, ctypes.c_float,
                                        ctypes.POINTER(ctypes.c_float),
@@ -154,3 +163,19 @@ class Context:
 
     def stream(self) -> int:
         return lib().vpmx_stream(self.ptr)
+
+    def fmm_stats(self) -> dict:
+        out = (ctypes.c_double * len(FMM_STATS))()
+        self.check(lib().vpmx_fmm_stats(self.ptr, out, len(FMM_STATS)))
+        return dict(zip(FMM_STATS, list(out)))
+
+
+def fmm_plan_counts(X, sigma, ncrit=50, theta=0.4, sigma_cut=7.0) -> dict:
+    """Tree and traversal counts of the library's FMM plan, built on the host."""
+    import numpy as np
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3)
+    s = np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1)
+    out = (ctypes.c_int64 * 5)()
+    check(lib().vpmx_fmm_plan_counts(X.shape[0], X.ctypes.data_as(_dp), s.ctypes.data_as(_dp),
+                                     int(ncrit), float(theta), float(sigma_cut), out))
+    return dict(zip(("ncell", "nleaf", "nm2l", "np2p", "p2p_pairs"), list(out)))

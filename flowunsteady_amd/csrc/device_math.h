@@ -252,4 +252,72 @@ VPMX_HD void radial_near_f32(float r2, float rinv, float p0, float p1,
     }
 }
 
+// FP32 tile partials of one target: U, W = sum A Gamma (Kronecker part of J)
+// and eight radial entries of J (the ninth follows from trace J = 0).
+struct MixedAcc {
+    float U0 = 0.f, U1 = 0.f, U2 = 0.f, W0 = 0.f, W1 = 0.f, W2 = 0.f;
+    float J0 = 0.f, J1 = 0.f, J2 = 0.f, J3 = 0.f, J4 = 0.f, J5 = 0.f, J6 = 0.f, J7 = 0.f;
+};
+
+// One pair in FP32.  a = (source position relative to the block origin, p0),
+// b = (Gamma, p1).  When every lane of the wave is in the far field
+// (rho^2 >= 49) the gaussian kernels skip exp/erf: g = 1, g' = 0 to FP32
+// rounding.  Coincident pairs contribute nothing.
+template <int K>
+__device__ __forceinline__ void mixed_pair_d(float dx, float dy, float dz, float p0,
+                                             const float4 b, MixedAcc &acc);
+
+template <int K>
+__device__ __forceinline__ void mixed_pair(const float4 a, const float4 b, float xi, float yi,
+                                           float zi, MixedAcc &acc)
+{
+    mixed_pair_d<K>(xi - a.x, yi - a.y, zi - a.z, a.w, b, acc);
+}
+
+// The same pair from its separation dX (target minus source) in FP32.
+template <int K>
+__device__ __forceinline__ void mixed_pair_d(float dx, float dy, float dz, float p0,
+                                             const float4 b, MixedAcc &acc)
+{
+    const float4 a = make_float4(0.f, 0.f, 0.f, p0);
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const bool self = !(r2 > 0.f);
+    const float rinv = self ? 0.f : frsq(r2);
+    float A, B;
+    if constexpr (K == GAUSSIANERF || K == GAUSSIAN) {
+        const bool near = r2 * b.w < RHO2_FAR;
+        if (__any(near)) {
+            radial_near_f32<K>(r2, rinv, a.w, b.w, A, B);
+        } else {
+            const float ri2 = rinv * rinv;
+            A = ri2 * rinv;
+            B = -3.0f * A * ri2;
+        }
+    } else {
+        radial_near_f32<K>(r2, rinv, a.w, b.w, A, B);
+    }
+    A = self ? 0.f : A;
+    B = self ? 0.f : B;
+    // c = dX x Gamma
+    const float c0 = fmaf(dy, b.z, -dz * b.y);
+    const float c1 = fmaf(dz, b.x, -dx * b.z);
+    const float c2 = fmaf(dx, b.y, -dy * b.x);
+    acc.U0 = fmaf(A, c0, acc.U0);
+    acc.U1 = fmaf(A, c1, acc.U1);
+    acc.U2 = fmaf(A, c2, acc.U2);
+    acc.W0 = fmaf(A, b.x, acc.W0);
+    acc.W1 = fmaf(A, b.y, acc.W1);
+    acc.W2 = fmaf(A, b.z, acc.W2);
+    const float ex = B * dx, ey = B * dy, ez = B * dz;
+    // radial part c_k dX_l B, column-major (k + 3 l)
+    acc.J0 = fmaf(c0, ex, acc.J0);
+    acc.J1 = fmaf(c1, ex, acc.J1);
+    acc.J2 = fmaf(c2, ex, acc.J2);
+    acc.J3 = fmaf(c0, ey, acc.J3);
+    acc.J4 = fmaf(c1, ey, acc.J4);
+    acc.J5 = fmaf(c2, ey, acc.J5);
+    acc.J6 = fmaf(c0, ez, acc.J6);
+    acc.J7 = fmaf(c1, ez, acc.J7);
+}
+
 }  // namespace vpmx

@@ -1,0 +1,111 @@
+// fmm.h -- tree, traversal lists, operator tables and device views of the
+// fast multipole UJ (fmm.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+#include <vector>
+
+#include "internal.h"
+
+namespace vpmx {
+
+constexpr int FMM_MAX_P = 12;   // K = 455 <= 2 coefficients per thread of 256
+
+struct FmmCellDev {
+    int32_t begin, end, child0, nchild;
+    double cx, cy, cz;
+};
+
+struct FmmCellHost {
+    int64_t begin, end, child0;
+    int nchild, level;
+    uint64_t prefix;
+    double c[3], R, smax;
+};
+
+// Operator tables of expansion order P (host copies).
+struct FmmTables {
+    int P = -1, K = 0;
+    std::vector<int> mi, rec1, rec2, d1, d2a, d2b;
+    std::vector<int> m2m_off, m2m_idx, l2l_off, l2l_idx, m2l_off, m2l_idx;
+    std::vector<double> m2m_coef, l2l_coef, m2l_coef;
+};
+
+// Tree + lists of one evaluation (host).
+struct FmmPlan {
+    double lo[3] = {0, 0, 0}, ext = 1.0;
+    std::vector<FmmCellHost> cells;
+    std::vector<int64_t> m2l_off, p2p_off;
+    std::vector<int32_t> m2l_src, p2p_src;
+    std::vector<int32_t> leaves, parents, kids, parent;
+    std::vector<std::pair<int, int>> level_parents, level_kids;   // (offset, count) per level
+    int64_t p2p_pairs = 0;
+    double ms_tree = 0, ms_traverse = 0, ms_upload = 0;
+};
+
+// Device views used by the kernels.
+struct FmmDev {
+    int P = 0, K = 0, ncell = 0, nleaf = 0;
+    const FmmCellDev *cells = nullptr;
+    const int32_t *leaves = nullptr, *parents = nullptr, *kids = nullptr, *parent = nullptr;
+    const int64_t *m2l_off = nullptr, *p2p_off = nullptr;
+    const int32_t *m2l_src = nullptr, *p2p_src = nullptr;
+    const double4 *rec = nullptr;      // sorted source records
+    const int32_t *perm = nullptr;     // sorted slot -> original index
+    double *M = nullptr, *L = nullptr;
+    const int *mi = nullptr;
+    const int4 *rec1 = nullptr, *rec2 = nullptr, *d1 = nullptr, *d2a = nullptr, *d2b = nullptr;
+    const int *m2m_off = nullptr, *l2l_off = nullptr, *m2l_toff = nullptr;
+    const int2 *m2m_idx = nullptr, *l2l_idx = nullptr, *m2l_tidx = nullptr;
+    const double *m2m_coef = nullptr, *l2l_coef = nullptr, *m2l_tcoef = nullptr;
+};
+
+// Per-context FMM state: host plan, operator tables and device buffers.
+struct FmmState {
+    FmmPlan plan;
+    FmmTables tab;
+    int dev_tab_p = -1;
+    // device operator tables
+    int *mi = nullptr, *m2m_off = nullptr, *l2l_off = nullptr, *m2l_toff = nullptr;
+    int4 *rec1 = nullptr, *rec2 = nullptr, *d1 = nullptr, *d2a = nullptr, *d2b = nullptr;
+    int2 *m2m_idx = nullptr, *l2l_idx = nullptr, *m2l_tidx = nullptr;
+    double *m2m_coef = nullptr, *l2l_coef = nullptr, *m2l_tcoef = nullptr;
+    // device tree and lists
+    int64_t cap_cells = 0, cap_m2l = 0, cap_p2p = 0, cap_coef = 0, cap_n = 0;
+    FmmCellDev *cells = nullptr;
+    int32_t *leaves = nullptr, *parents = nullptr, *kids = nullptr, *parent = nullptr;
+    int64_t *m2l_off = nullptr, *p2p_off = nullptr;
+    int32_t *m2l_src = nullptr, *p2p_src = nullptr;
+    double *M = nullptr, *L = nullptr;
+    // sorting
+    uint64_t *kin = nullptr, *kout = nullptr;
+    double *ssig = nullptr;
+    uint8_t *sort_tmp = nullptr;
+    size_t sort_bytes = 0;
+    // pinned host copies of the sorted keys and sigmas
+    uint64_t *h_keys = nullptr;
+    double *h_ssig = nullptr;
+    int64_t h_cap = 0;
+    // statistics of the last evaluation
+    double stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+void fmm_build_tree(FmmPlan &pl, const uint64_t *key, const double *ssig, int64_t n, int ncrit);
+void fmm_traverse(FmmPlan &pl, double theta, double sigcut);
+void fmm_build_tables(FmmTables &T, int P);
+void fmm_plan_host(FmmPlan &pl, int64_t n, const double *X, const double *sig, int ncrit,
+                   double theta, double sigcut);
+
+hipError_t launch_fmm_keys(hipStream_t st, const DeviceField &f, const double lo[3], double scale,
+                           uint64_t *kin, int32_t *vin);
+size_t fmm_sort_temp_bytes(int64_t n);
+hipError_t launch_fmm_sort(hipStream_t st, void *tmp, size_t bytes, const uint64_t *kin,
+                           uint64_t *kout, const int32_t *vin, int32_t *perm, int64_t n);
+hipError_t launch_fmm_gather(hipStream_t st, const DeviceField &f, const int32_t *perm,
+                             double4 *rec, double *ssig);
+hipError_t launch_fmm_eval(hipStream_t st, const FmmDev &d, const FmmPlan &pl, int kernel,
+                           int precision, const DeviceField &f, int reset);
+
+}  // namespace vpmx

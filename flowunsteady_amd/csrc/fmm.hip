@@ -1,0 +1,948 @@
+// fmm.hip -- fast multipole UJ (the UJ_fmm scheme) for gfx950.
+//
+// Same outputs as the direct UJ (U and J of every particle from every
+// particle) in O(N) work, controlled by the expansion order p, the leaf size
+// ncrit and the acceptance parameter theta (vpm.FMM(; p, ncrit, theta),
+// src/FLOWUnsteady_simulation.jl:43,134).
+//
+// Far field: the vector potential psi = (1/4 pi) sum_j Gamma_j / |x - x_j| as
+// three Laplace potentials, Cartesian Taylor expansions truncated at total
+// degree p (raw multipole moments m_a = sum q (x - c)^a, locals
+// phi = sum l_b (x - a)^b, b_k = d^k(1/r)/k! by the recurrence
+// R^2 n b_k = -(2n-1) sum_i r_i b_{k-e_i} - (n-1) sum_i b_{k-2e_i}), then
+// U = curl psi and J = grad U from the first and second derivatives of the
+// local expansions.  Near field: the regularised pair kernel (FP64 or mixed),
+// exactly as the direct UJ.
+//
+// Tree: adaptive octree over 63-bit Morton keys of the bounding cube, leaves of
+// at most ncrit particles, breadth-first cell order.  Traversal: dual tree,
+// (target A, source B) goes to M2L when (R_A + R_B) < theta |c_A - c_B| and the
+// gap |c_A - c_B| - R_A - R_B >= sigcut * max sigma in B (so the singular far
+// field matches the regularised kernel to its tail), else the larger cell is
+// split; leaf-leaf pairs go to P2P.  The tree build and traversal run on the
+// host from the GPU-sorted keys; every expansion and pair kernel runs on the
+// GPU.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "device_math.h"
+#include "fmm.h"
+#include "internal.h"
+
+namespace vpmx {
+
+constexpr int KEYBITS = 21;
+constexpr int FMM_BLOCK = 256;
+constexpr int P2P_BLOCK = 64;
+
+static inline unsigned gridn(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// Multi-index tables (host)
+// ---------------------------------------------------------------------------
+struct MIndex {
+    int P = -1, K = 0;
+    std::vector<int> mi;     // 3 per index, graded order
+    std::vector<int> lut;    // (a*(P+1)+b)*(P+1)+c -> index or -1
+    void build(int P_)
+    {
+        P = P_;
+        K = (P + 1) * (P + 2) * (P + 3) / 6;
+        mi.assign(3 * K, 0);
+        lut.assign((P + 1) * (P + 1) * (P + 1), -1);
+        int k = 0;
+        for (int n = 0; n <= P; n++)
+            for (int a = n; a >= 0; a--)
+                for (int b = n - a; b >= 0; b--) {
+                    int c = n - a - b;
+                    mi[3 * k] = a; mi[3 * k + 1] = b; mi[3 * k + 2] = c;
+                    lut[(a * (P + 1) + b) * (P + 1) + c] = k;
+                    k++;
+                }
+    }
+    int idx(int a, int b, int c) const
+    {
+        if (a < 0 || b < 0 || c < 0 || a + b + c > P) return -1;
+        return lut[(a * (P + 1) + b) * (P + 1) + c];
+    }
+};
+
+static double binom(int n, int k)
+{
+    double r = 1.0;
+    for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Device kernels
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t spread21(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+__global__ __launch_bounds__(FMM_BLOCK) void morton63_kernel(const double *__restrict__ X, int64_t n,
+                                                             int64_t cap, double lx, double ly,
+                                                             double lz, double scale,
+                                                             uint64_t *__restrict__ key,
+                                                             int32_t *__restrict__ val)
+{
+    const int64_t i = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double lo[3] = {lx, ly, lz};
+    uint64_t q[3];
+    for (int k = 0; k < 3; k++) {
+        double v = (X[k * cap + i] - lo[k]) * scale;
+        v = fmin(fmax(v, 0.0), 2097151.0);
+        q[k] = (uint64_t)v;
+    }
+    key[i] = spread21(q[0]) | (spread21(q[1]) << 1) | (spread21(q[2]) << 2);
+    val[i] = (int32_t)i;
+}
+
+// Sorted copies: FP64 source records (x, y, z, Gx) (Gy, Gz, sigma, 1/sigma)
+// and sigma in sorted order.
+__global__ __launch_bounds__(FMM_BLOCK) void fmm_gather_kernel(const double *__restrict__ X,
+                                                               const double *__restrict__ G,
+                                                               const double *__restrict__ sg,
+                                                               const int32_t *__restrict__ perm,
+                                                               int64_t n, int64_t cap,
+                                                               double4 *__restrict__ rec,
+                                                               double *__restrict__ ssig)
+{
+    const int64_t s = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = perm[s];
+    const double sig = sg[i];
+    rec[2 * s] = make_double4(X[i], X[cap + i], X[2 * cap + i], G[i]);
+    rec[2 * s + 1] = make_double4(G[cap + i], G[2 * cap + i], sig, 1.0 / sig);
+    ssig[s] = sig;
+}
+
+// P2M: one block per leaf; thread k accumulates m_k for the three components.
+__global__ __launch_bounds__(FMM_BLOCK) void p2m_kernel(const int32_t *__restrict__ leaves,
+                                                        const FmmCellDev *__restrict__ cells,
+                                                        const double4 *__restrict__ rec, int K,
+                                                        const int *__restrict__ mi,
+                                                        double *__restrict__ M)
+{
+    __shared__ double tp[FMM_BLOCK][3];
+    __shared__ double gp[FMM_BLOCK][3];
+    const int c = leaves[blockIdx.x];
+    const FmmCellDev C = cells[c];
+    double acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    for (int base = C.begin; base < C.end; base += FMM_BLOCK) {
+        __syncthreads();
+        const int j = base + threadIdx.x;
+        if (j < C.end) {
+            const double4 a = rec[2 * j], b = rec[2 * j + 1];
+            tp[threadIdx.x][0] = a.x - C.cx;
+            tp[threadIdx.x][1] = a.y - C.cy;
+            tp[threadIdx.x][2] = a.z - C.cz;
+            gp[threadIdx.x][0] = a.w;
+            gp[threadIdx.x][1] = b.x;
+            gp[threadIdx.x][2] = b.y;
+        }
+        __syncthreads();
+        const int cnt = min(FMM_BLOCK, C.end - base);
+        for (int r = 0; r < 2; r++) {
+            const int k = threadIdx.x + r * FMM_BLOCK;
+            if (k >= K) continue;
+            const int e0 = mi[3 * k], e1 = mi[3 * k + 1], e2 = mi[3 * k + 2];
+            for (int jj = 0; jj < cnt; jj++) {
+                double px = 1.0, py = 1.0, pz = 1.0;
+                for (int e = 0; e < e0; e++) px *= tp[jj][0];
+                for (int e = 0; e < e1; e++) py *= tp[jj][1];
+                for (int e = 0; e < e2; e++) pz *= tp[jj][2];
+                const double mono = px * py * pz;
+                acc[r][0] += gp[jj][0] * mono;
+                acc[r][1] += gp[jj][1] * mono;
+                acc[r][2] += gp[jj][2] * mono;
+            }
+        }
+    }
+    for (int r = 0; r < 2; r++) {
+        const int k = threadIdx.x + r * FMM_BLOCK;
+        if (k >= K) continue;
+        for (int q = 0; q < 3; q++) M[((size_t)c * 3 + q) * K + k] = acc[r][q];
+    }
+}
+
+// monomials d^a for |a| <= P into LDS (thread-parallel over a)
+__device__ __forceinline__ void lds_monomials(const double d[3], int K, const int *__restrict__ mi,
+                                              double *out)
+{
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        double px = 1.0, py = 1.0, pz = 1.0;
+        for (int e = 0; e < mi[3 * k]; e++) px *= d[0];
+        for (int e = 0; e < mi[3 * k + 1]; e++) py *= d[1];
+        for (int e = 0; e < mi[3 * k + 2]; e++) pz *= d[2];
+        out[k] = px * py * pz;
+    }
+}
+
+// M2M for the parents listed in `parents` (one level): m_a(c) += sum_child
+// sum_{b <= a} C(a,b) m_b(child) d^{a-b}, d = c_child - c.
+__global__ __launch_bounds__(FMM_BLOCK) void m2m_kernel(const int32_t *__restrict__ parents,
+                                                        const FmmCellDev *__restrict__ cells,
+                                                        int K, const int *__restrict__ mi,
+                                                        const int *__restrict__ toff,
+                                                        const int2 *__restrict__ tidx,
+                                                        const double *__restrict__ tcoef,
+                                                        double *__restrict__ M)
+{
+    extern __shared__ double sh[];
+    double *mono = sh;           // K
+    double *mc = sh + K;         // 3K child multipole
+    const int c = parents[blockIdx.x];
+    const FmmCellDev C = cells[c];
+    double acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    for (int ch = 0; ch < C.nchild; ch++) {
+        const int d = C.child0 + ch;
+        const FmmCellDev D = cells[d];
+        __syncthreads();
+        const double dd[3] = {D.cx - C.cx, D.cy - C.cy, D.cz - C.cz};
+        lds_monomials(dd, K, mi, mono);
+        for (int k = threadIdx.x; k < 3 * K; k += blockDim.x) mc[k] = M[(size_t)d * 3 * K + k];
+        __syncthreads();
+        for (int r = 0; r < 2; r++) {
+            const int k = threadIdx.x + r * FMM_BLOCK;
+            if (k >= K) continue;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (int e = toff[k]; e < toff[k + 1]; e++) {
+                const int2 ix = tidx[e];
+                const double w = tcoef[e] * mono[ix.y];
+                s0 += mc[ix.x] * w;
+                s1 += mc[K + ix.x] * w;
+                s2 += mc[2 * K + ix.x] * w;
+            }
+            acc[r][0] += s0;
+            acc[r][1] += s1;
+            acc[r][2] += s2;
+        }
+    }
+    for (int r = 0; r < 2; r++) {
+        const int k = threadIdx.x + r * FMM_BLOCK;
+        if (k >= K) continue;
+        for (int q = 0; q < 3; q++) M[((size_t)c * 3 + q) * K + k] = acc[r][q];
+    }
+}
+
+// M2L: one block per target cell; l_b = sum_sources sum_a coef m_a b_{a+b}.
+// Writes (overwrites) the cell's local expansion.
+__global__ __launch_bounds__(FMM_BLOCK) void m2l_kernel(const FmmCellDev *__restrict__ cells,
+                                                        const int64_t *__restrict__ off,
+                                                        const int32_t *__restrict__ src, int K,
+                                                        int P, const int *__restrict__ mi,
+                                                        const int4 *__restrict__ rec1,
+                                                        const int4 *__restrict__ rec2,
+                                                        const int *__restrict__ toff,
+                                                        const int2 *__restrict__ tidx,
+                                                        const double *__restrict__ tcoef,
+                                                        const double *__restrict__ M,
+                                                        double *__restrict__ L)
+{
+    extern __shared__ double sh[];
+    double *bb = sh;            // K
+    double *ms = sh + K;        // 3K
+    const int c = blockIdx.x;
+    const FmmCellDev A = cells[c];
+    double acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    for (int64_t e = off[c]; e < off[c + 1]; e++) {
+        const int s = src[e];
+        const FmmCellDev B = cells[s];
+        const double r0 = A.cx - B.cx, r1 = A.cy - B.cy, r2 = A.cz - B.cz;
+        const double R2 = r0 * r0 + r1 * r1 + r2 * r2;
+        __syncthreads();
+        for (int k = threadIdx.x; k < 3 * K; k += blockDim.x) ms[k] = M[(size_t)s * 3 * K + k];
+        if (threadIdx.x == 0) bb[0] = 1.0 / sqrt(R2);
+        __syncthreads();
+        // derivatives of 1/r, one degree at a time
+        int kbeg = 1;
+        for (int n = 1; n <= P; n++) {
+            const int kend = (n + 1) * (n + 2) * (n + 3) / 6;
+            for (int k = kbeg + threadIdx.x; k < kend; k += blockDim.x) {
+                const int4 j1 = rec1[k];   // idx(k - e_i) for i = 0..2, -1 if none
+                const int4 j2 = rec2[k];   // idx(k - 2 e_i)
+                double s1 = 0.0, s2 = 0.0;
+                if (j1.x >= 0) s1 += r0 * bb[j1.x];
+                if (j1.y >= 0) s1 += r1 * bb[j1.y];
+                if (j1.z >= 0) s1 += r2 * bb[j1.z];
+                if (j2.x >= 0) s2 += bb[j2.x];
+                if (j2.y >= 0) s2 += bb[j2.y];
+                if (j2.z >= 0) s2 += bb[j2.z];
+                bb[k] = -((2.0 * n - 1.0) * s1 + (n - 1.0) * s2) / (n * R2);
+            }
+            kbeg = kend;
+            __syncthreads();
+        }
+        for (int r = 0; r < 2; r++) {
+            const int k = threadIdx.x + r * FMM_BLOCK;
+            if (k >= K) continue;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (int t = toff[k]; t < toff[k + 1]; t++) {
+                const int2 ix = tidx[t];   // (a, a+b)
+                const double w = tcoef[t] * bb[ix.y];
+                s0 += w * ms[ix.x];
+                s1 += w * ms[K + ix.x];
+                s2 += w * ms[2 * K + ix.x];
+            }
+            acc[r][0] += s0;
+            acc[r][1] += s1;
+            acc[r][2] += s2;
+        }
+    }
+    for (int r = 0; r < 2; r++) {
+        const int k = threadIdx.x + r * FMM_BLOCK;
+        if (k >= K) continue;
+        for (int q = 0; q < 3; q++) L[((size_t)c * 3 + q) * K + k] = acc[r][q];
+    }
+}
+
+// L2L for the children listed in `kids` (one level): l_g(child) +=
+// sum_{b >= g} C(b,g) l_b(parent) e^{b-g}, e = c_child - c_parent.
+__global__ __launch_bounds__(FMM_BLOCK) void l2l_kernel(const int32_t *__restrict__ kids,
+                                                        const int32_t *__restrict__ parent,
+                                                        const FmmCellDev *__restrict__ cells,
+                                                        int K, const int *__restrict__ mi,
+                                                        const int *__restrict__ toff,
+                                                        const int2 *__restrict__ tidx,
+                                                        const double *__restrict__ tcoef,
+                                                        double *__restrict__ L)
+{
+    extern __shared__ double sh[];
+    double *mono = sh;
+    double *lp = sh + K;
+    const int d = kids[blockIdx.x];
+    const int c = parent[d];
+    const FmmCellDev D = cells[d], C = cells[c];
+    const double e[3] = {D.cx - C.cx, D.cy - C.cy, D.cz - C.cz};
+    lds_monomials(e, K, mi, mono);
+    for (int k = threadIdx.x; k < 3 * K; k += blockDim.x) lp[k] = L[(size_t)c * 3 * K + k];
+    __syncthreads();
+    for (int r = 0; r < 2; r++) {
+        const int k = threadIdx.x + r * FMM_BLOCK;
+        if (k >= K) continue;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int t = toff[k]; t < toff[k + 1]; t++) {
+            const int2 ix = tidx[t];   // (b, b-g)
+            const double w = tcoef[t] * mono[ix.y];
+            s0 += lp[ix.x] * w;
+            s1 += lp[K + ix.x] * w;
+            s2 += lp[2 * K + ix.x] * w;
+        }
+        L[((size_t)d * 3 + 0) * K + k] += s0;
+        L[((size_t)d * 3 + 1) * K + k] += s1;
+        L[((size_t)d * 3 + 2) * K + k] += s2;
+    }
+}
+
+// L2P: one block per leaf, one particle per thread; far-field U and J.
+// Writes U, J at the particle's original index (overwrite, or add when
+// reset == 0).
+__global__ __launch_bounds__(P2P_BLOCK) void l2p_kernel(const int32_t *__restrict__ leaves,
+                                                        const FmmCellDev *__restrict__ cells,
+                                                        const double4 *__restrict__ rec,
+                                                        const int32_t *__restrict__ perm, int K,
+                                                        int P, const int *__restrict__ mi,
+                                                        const int4 *__restrict__ d1idx,
+                                                        const int4 *__restrict__ d2idx_a,
+                                                        const int4 *__restrict__ d2idx_b,
+                                                        const double *__restrict__ L,
+                                                        int64_t cap, double *__restrict__ U,
+                                                        double *__restrict__ J, int reset)
+{
+    extern __shared__ double sh[];
+    double *lcell = sh;                         // 3K
+    double *pw = sh + 3 * K;                    // P2P_BLOCK x 3 x (P+1)
+    const int c = leaves[blockIdx.x];
+    const FmmCellDev C = cells[c];
+    for (int k = threadIdx.x; k < 3 * K; k += blockDim.x) lcell[k] = L[(size_t)c * 3 * K + k];
+    __syncthreads();
+    const int stride = 3 * (P + 1);
+    double *mp = pw + threadIdx.x * stride;
+    for (int base = C.begin; base < C.end; base += P2P_BLOCK) {
+        const int j = base + threadIdx.x;
+        if (j >= C.end) continue;
+        const double4 a = rec[2 * j];
+        const double t[3] = {a.x - C.cx, a.y - C.cy, a.z - C.cz};
+        for (int i = 0; i < 3; i++) {
+            mp[i * (P + 1)] = 1.0;
+            for (int e = 1; e <= P; e++) mp[i * (P + 1) + e] = mp[i * (P + 1) + e - 1] * t[i];
+        }
+        double d1[3][3], d2[3][6];
+        for (int q = 0; q < 3; q++) {
+            for (int i = 0; i < 3; i++) d1[q][i] = 0.0;
+            for (int i = 0; i < 6; i++) d2[q][i] = 0.0;
+        }
+        for (int kb = 0; kb < K; kb++) {
+            const int e0 = mi[3 * kb], e1 = mi[3 * kb + 1], e2 = mi[3 * kb + 2];
+            const double mono = mp[e0] * mp[(P + 1) + e1] * mp[2 * (P + 1) + e2];
+            const int4 k1 = d1idx[kb];    // idx(b + e_j), j = 0..2
+            const int4 ka = d2idx_a[kb];  // idx(b + e_j + e_l) for (0,0) (0,1) (0,2)
+            const int4 kc = d2idx_b[kb];  // for (1,1) (1,2) (2,2)
+            const int bexp[3] = {e0, e1, e2};
+            const int k1a[3] = {k1.x, k1.y, k1.z};
+            for (int jj = 0; jj < 3; jj++) {
+                if (k1a[jj] < 0) continue;
+                const double f1 = (double)(bexp[jj] + 1) * mono;
+                for (int q = 0; q < 3; q++) d1[q][jj] += lcell[q * K + k1a[jj]] * f1;
+            }
+            // second derivatives: (j,l) pairs in order (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)
+            const int k2a[6] = {ka.x, ka.y, ka.z, kc.x, kc.y, kc.z};
+            const int pj[6] = {0, 0, 0, 1, 1, 2}, pl[6] = {0, 1, 2, 1, 2, 2};
+            for (int m = 0; m < 6; m++) {
+                if (k2a[m] < 0) continue;
+                const int jj = pj[m], ll = pl[m];
+                const double f2 = (double)(bexp[jj] + 1) * (double)(bexp[ll] + (jj == ll) + 1) * mono;
+                for (int q = 0; q < 3; q++) d2[q][m] += lcell[q * K + k2a[m]] * f2;
+            }
+        }
+        // symmetric second derivatives h[q][j][l]
+        auto h = [&](int q, int jj, int ll) -> double {
+            if (jj > ll) { const int t2 = jj; jj = ll; ll = t2; }
+            const int m = jj == 0 ? ll : (jj == 1 ? 2 + ll : 5);
+            return d2[q][m];
+        };
+        double u[3], Jm[9];
+        u[0] = d1[2][1] - d1[1][2];
+        u[1] = d1[0][2] - d1[2][0];
+        u[2] = d1[1][0] - d1[0][1];
+        for (int l = 0; l < 3; l++) {
+            Jm[0 + 3 * l] = h(2, 1, l) - h(1, 2, l);
+            Jm[1 + 3 * l] = h(0, 2, l) - h(2, 0, l);
+            Jm[2 + 3 * l] = h(1, 0, l) - h(0, 1, l);
+        }
+        const int64_t i = perm[j];
+        for (int k = 0; k < 3; k++) {
+            const double v = INV4PI * u[k];
+            U[k * cap + i] = reset ? v : U[k * cap + i] + v;
+        }
+        for (int k = 0; k < 9; k++) {
+            const double v = INV4PI * Jm[k];
+            J[k * cap + i] = reset ? v : J[k * cap + i] + v;
+        }
+    }
+}
+
+// P2P near field, FP64: one block (one wave) per target leaf, one target per
+// lane, source leaves of the list staged through LDS.  Adds to U, J.
+template <int KK>
+__global__ __launch_bounds__(P2P_BLOCK) void p2p_f64_kernel(const int32_t *__restrict__ leaves,
+                                                            const FmmCellDev *__restrict__ cells,
+                                                            const int64_t *__restrict__ off,
+                                                            const int32_t *__restrict__ src,
+                                                            const double4 *__restrict__ rec,
+                                                            const int32_t *__restrict__ perm,
+                                                            int64_t cap, double *__restrict__ U,
+                                                            double *__restrict__ J)
+{
+    __shared__ double4 tile[2 * P2P_BLOCK];
+    const int c = leaves[blockIdx.x];
+    const FmmCellDev A = cells[c];
+    for (int tb = A.begin; tb < A.end; tb += P2P_BLOCK) {
+        const int it = tb + threadIdx.x;
+        const bool active = it < A.end;
+        double xi = 0, yi = 0, zi = 0;
+        if (active) {
+            const double4 a = rec[2 * it];
+            xi = a.x; yi = a.y; zi = a.z;
+        }
+        double u0 = 0, u1 = 0, u2 = 0, j0 = 0, j1 = 0, j2 = 0, j3 = 0, j4 = 0, j5 = 0, j6 = 0,
+               j7 = 0, j8 = 0;
+        for (int64_t e = off[c]; e < off[c + 1]; e++) {
+            const FmmCellDev B = cells[src[e]];
+            for (int sb = B.begin; sb < B.end; sb += P2P_BLOCK) {
+                __syncthreads();
+                const int js = sb + threadIdx.x;
+                if (js < B.end) {
+                    tile[2 * threadIdx.x] = rec[2 * js];
+                    tile[2 * threadIdx.x + 1] = rec[2 * js + 1];
+                }
+                __syncthreads();
+                const int cnt = min(P2P_BLOCK, B.end - sb);
+                for (int jj = 0; jj < cnt; ++jj) {
+                    const double4 a = tile[2 * jj], b = tile[2 * jj + 1];
+                    const double dX1 = xi - a.x, dX2 = yi - a.y, dX3 = zi - a.z;
+                    const double r2 = dX1 * dX1 + dX2 * dX2 + dX3 * dX3;
+                    if (r2 > 0.0) {
+                        const double r = sqrt(r2);
+                        double g, dg;
+                        g_dg_f64<KK>(r * b.w, g, dg);
+                        const double rinv = 1.0 / r;
+                        const double r3i = rinv * rinv * rinv;
+                        const double G1 = a.w, G2 = b.x, G3 = b.y;
+                        const double c1 = -INV4PI * r3i * (dX2 * G3 - dX3 * G2);
+                        const double c2 = -INV4PI * r3i * (dX3 * G1 - dX1 * G3);
+                        const double c3 = -INV4PI * r3i * (dX1 * G2 - dX2 * G1);
+                        u0 += g * c1; u1 += g * c2; u2 += g * c3;
+                        const double aux = dg * b.w * rinv - 3.0 * g * rinv * rinv;
+                        j0 += aux * c1 * dX1; j1 += aux * c2 * dX1; j2 += aux * c3 * dX1;
+                        j3 += aux * c1 * dX2; j4 += aux * c2 * dX2; j5 += aux * c3 * dX2;
+                        j6 += aux * c1 * dX3; j7 += aux * c2 * dX3; j8 += aux * c3 * dX3;
+                        const double a2 = -INV4PI * g * r3i;
+                        j1 -= a2 * G3; j2 += a2 * G2; j3 += a2 * G3;
+                        j5 -= a2 * G1; j6 -= a2 * G2; j7 += a2 * G1;
+                    }
+                }
+            }
+        }
+        if (active) {
+            const int64_t i = perm[it];
+            U[i] += u0; U[cap + i] += u1; U[2 * cap + i] += u2;
+            J[i] += j0; J[cap + i] += j1; J[2 * cap + i] += j2;
+            J[3 * cap + i] += j3; J[4 * cap + i] += j4; J[5 * cap + i] += j5;
+            J[6 * cap + i] += j6; J[7 * cap + i] += j7; J[8 * cap + i] += j8;
+        }
+    }
+}
+
+// P2P near field, mixed: FP32 pair math on coordinates relative to the
+// target leaf's centre, FP32 partials per source leaf folded into FP64.
+template <int KK>
+__global__ __launch_bounds__(P2P_BLOCK) void p2p_mixed_kernel(const int32_t *__restrict__ leaves,
+                                                              const FmmCellDev *__restrict__ cells,
+                                                              const int64_t *__restrict__ off,
+                                                              const int32_t *__restrict__ src,
+                                                              const double4 *__restrict__ rec,
+                                                              const int32_t *__restrict__ perm,
+                                                              int64_t cap, double *__restrict__ U,
+                                                              double *__restrict__ J)
+{
+    __shared__ float4 tpos[P2P_BLOCK];
+    __shared__ float4 tgam[P2P_BLOCK];
+    const int c = leaves[blockIdx.x];
+    const FmmCellDev A = cells[c];
+    for (int tb = A.begin; tb < A.end; tb += P2P_BLOCK) {
+        const int it = tb + threadIdx.x;
+        const bool active = it < A.end;
+        float xi = 0.f, yi = 0.f, zi = 0.f;
+        if (active) {
+            const double4 a = rec[2 * it];
+            xi = (float)(a.x - A.cx); yi = (float)(a.y - A.cy); zi = (float)(a.z - A.cz);
+        }
+        double dU0 = 0, dU1 = 0, dU2 = 0, dW0 = 0, dW1 = 0, dW2 = 0;
+        double dJ0 = 0, dJ1 = 0, dJ2 = 0, dJ3 = 0, dJ4 = 0, dJ5 = 0, dJ6 = 0, dJ7 = 0;
+        for (int64_t e = off[c]; e < off[c + 1]; e++) {
+            const FmmCellDev B = cells[src[e]];
+            for (int sb = B.begin; sb < B.end; sb += P2P_BLOCK) {
+                __syncthreads();
+                const int js = sb + threadIdx.x;
+                if (js < B.end) {
+                    const double4 a = rec[2 * js], b = rec[2 * js + 1];
+                    float p0 = 0.f, p1 = 0.f;
+                    if constexpr (KK == GAUSSIANERF || KK == GAUSSIAN) {
+                        p0 = (float)b.w;
+                        p1 = (float)(b.w * b.w);
+                    } else if constexpr (KK == WINCKELMANS) {
+                        p0 = (float)(b.z * b.z);
+                    }
+                    tpos[threadIdx.x] = make_float4((float)(a.x - A.cx), (float)(a.y - A.cy),
+                                                    (float)(a.z - A.cz), p0);
+                    tgam[threadIdx.x] = make_float4((float)a.w, (float)b.x, (float)b.y, p1);
+                }
+                __syncthreads();
+                const int cnt = min(P2P_BLOCK, B.end - sb);
+                MixedAcc acc;
+                for (int jj = 0; jj < cnt; ++jj) mixed_pair<KK>(tpos[jj], tgam[jj], xi, yi, zi, acc);
+                dU0 += acc.U0; dU1 += acc.U1; dU2 += acc.U2;
+                dW0 += acc.W0; dW1 += acc.W1; dW2 += acc.W2;
+                dJ0 += acc.J0; dJ1 += acc.J1; dJ2 += acc.J2; dJ3 += acc.J3;
+                dJ4 += acc.J4; dJ5 += acc.J5; dJ6 += acc.J6; dJ7 += acc.J7;
+            }
+        }
+        if (active) {
+            const int64_t i = perm[it];
+            const double k = -INV4PI;
+            U[i] += k * dU0; U[cap + i] += k * dU1; U[2 * cap + i] += k * dU2;
+            J[i] += k * dJ0;
+            J[cap + i] += k * (dJ1 - dW2);
+            J[2 * cap + i] += k * (dJ2 + dW1);
+            J[3 * cap + i] += k * (dJ3 + dW2);
+            J[4 * cap + i] += k * dJ4;
+            J[5 * cap + i] += k * (dJ5 - dW0);
+            J[6 * cap + i] += k * (dJ6 - dW1);
+            J[7 * cap + i] += k * (dJ7 + dW0);
+            J[8 * cap + i] += -k * (dJ0 + dJ4);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host: tree build and traversal (same rules as the oracle's CPU FMM)
+// ---------------------------------------------------------------------------
+
+static int64_t lower_bound_key(const uint64_t *key, int64_t b, int64_t e, uint64_t v)
+{
+    while (b < e) {
+        int64_t m = b + (e - b) / 2;
+        if (key[m] < v) b = m + 1; else e = m;
+    }
+    return b;
+}
+
+void fmm_build_tree(FmmPlan &pl, const uint64_t *key, const double *ssig, int64_t n, int ncrit)
+{
+    pl.cells.clear();
+    std::vector<uint64_t> prefix;
+    FmmCellHost root{};
+    root.begin = 0; root.end = n; root.level = 0; root.child0 = -1; root.nchild = 0;
+    pl.cells.push_back(root);
+    prefix.push_back(0);
+    for (size_t ci = 0; ci < pl.cells.size(); ci++) {
+        const FmmCellHost C = pl.cells[ci];
+        if (C.end - C.begin <= ncrit || C.level >= KEYBITS) continue;
+        const int shift = 3 * (KEYBITS - C.level - 1);
+        const int64_t first = (int64_t)pl.cells.size();
+        int nch = 0;
+        for (uint64_t o = 0; o < 8; o++) {
+            const uint64_t cp = (prefix[ci] << 3) | o;
+            const int64_t b = lower_bound_key(key, C.begin, C.end, cp << shift);
+            const int64_t e = lower_bound_key(key, C.begin, C.end, (cp + 1) << shift);
+            if (e == b) continue;
+            FmmCellHost D{};
+            D.begin = b; D.end = e; D.level = C.level + 1; D.child0 = -1; D.nchild = 0;
+            D.prefix = cp;
+            pl.cells.push_back(D);
+            prefix.push_back(cp);
+            nch++;
+        }
+        pl.cells[ci].child0 = first;
+        pl.cells[ci].nchild = nch;
+    }
+    // geometry
+    for (size_t ci = 0; ci < pl.cells.size(); ci++) {
+        FmmCellHost &C = pl.cells[ci];
+        const uint64_t pre = prefix[ci];
+        uint64_t ix = 0, iy = 0, iz = 0;
+        for (int l = 0; l < C.level; l++) {
+            const uint64_t o = (pre >> (3 * (C.level - 1 - l))) & 7u;
+            ix = (ix << 1) | (o & 1u);
+            iy = (iy << 1) | ((o >> 1) & 1u);
+            iz = (iz << 1) | ((o >> 2) & 1u);
+        }
+        const double h = pl.ext / (double)((uint64_t)1 << C.level);
+        C.c[0] = pl.lo[0] + ((double)ix + 0.5) * h;
+        C.c[1] = pl.lo[1] + ((double)iy + 0.5) * h;
+        C.c[2] = pl.lo[2] + ((double)iz + 0.5) * h;
+        C.R = 0.8660254037844386 * h;
+    }
+    // sigma max, bottom-up
+    for (int64_t ci = (int64_t)pl.cells.size() - 1; ci >= 0; ci--) {
+        FmmCellHost &C = pl.cells[ci];
+        double s = 0.0;
+        if (C.nchild == 0) {
+            for (int64_t j = C.begin; j < C.end; j++) s = fmax(s, ssig[j]);
+        } else {
+            for (int k = 0; k < C.nchild; k++) s = fmax(s, pl.cells[C.child0 + k].smax);
+        }
+        C.smax = s;
+    }
+}
+
+static bool mac(const FmmCellHost &A, const FmmCellHost &B, double theta, double sigcut)
+{
+    const double dx = A.c[0] - B.c[0], dy = A.c[1] - B.c[1], dz = A.c[2] - B.c[2];
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    return (A.R + B.R) < theta * d && (d - A.R - B.R) >= sigcut * B.smax;
+}
+
+void fmm_traverse(FmmPlan &pl, double theta, double sigcut)
+{
+    std::vector<std::pair<int32_t, int32_t>> m2l, p2p, st;
+    st.reserve(1 << 16);
+    if (!pl.cells.empty()) st.push_back({0, 0});
+    const auto &cells = pl.cells;
+    while (!st.empty()) {
+        const auto pr = st.back();
+        st.pop_back();
+        const int32_t a = pr.first, b = pr.second;
+        const FmmCellHost &A = cells[a], &B = cells[b];
+        if (a == b) {
+            if (A.nchild == 0) {
+                p2p.push_back({a, b});
+            } else {
+                for (int i = A.nchild - 1; i >= 0; i--)
+                    for (int j = A.nchild - 1; j >= 0; j--)
+                        st.push_back({(int32_t)(A.child0 + i), (int32_t)(A.child0 + j)});
+            }
+        } else if (mac(A, B, theta, sigcut)) {
+            m2l.push_back({a, b});
+        } else if (A.nchild == 0 && B.nchild == 0) {
+            p2p.push_back({a, b});
+        } else if (B.nchild == 0 || (A.nchild != 0 && A.R >= B.R)) {
+            for (int i = A.nchild - 1; i >= 0; i--) st.push_back({(int32_t)(A.child0 + i), b});
+        } else {
+            for (int j = B.nchild - 1; j >= 0; j--) st.push_back({a, (int32_t)(B.child0 + j)});
+        }
+    }
+    std::sort(m2l.begin(), m2l.end());
+    std::sort(p2p.begin(), p2p.end());
+    const int64_t nc = (int64_t)cells.size();
+    auto csr = [nc](const std::vector<std::pair<int32_t, int32_t>> &v, std::vector<int64_t> &off,
+                    std::vector<int32_t> &src) {
+        off.assign(nc + 1, 0);
+        src.resize(v.size());
+        for (size_t i = 0; i < v.size(); i++) {
+            off[v[i].first + 1]++;
+            src[i] = v[i].second;
+        }
+        for (int64_t c = 0; c < nc; c++) off[c + 1] += off[c];
+    };
+    csr(m2l, pl.m2l_off, pl.m2l_src);
+    csr(p2p, pl.p2p_off, pl.p2p_src);
+}
+
+// Operator tables for expansion order P (host build, uploaded once per P).
+void fmm_build_tables(FmmTables &T, int P)
+{
+    MIndex M;
+    M.build(P);
+    const int K = M.K;
+    T.P = P;
+    T.K = K;
+    T.mi = M.mi;
+    // derivative recurrence: idx(k - e_i), idx(k - 2 e_i)
+    T.rec1.assign(4 * K, -1);
+    T.rec2.assign(4 * K, -1);
+    for (int k = 0; k < K; k++) {
+        const int *a = &M.mi[3 * k];
+        for (int i = 0; i < 3; i++) {
+            int d[3] = {a[0], a[1], a[2]};
+            d[i] -= 1;
+            T.rec1[4 * k + i] = M.idx(d[0], d[1], d[2]);
+            d[i] -= 1;
+            T.rec2[4 * k + i] = M.idx(d[0], d[1], d[2]);
+        }
+    }
+    // M2M: for each a, (b, a-b, C(a,b))
+    T.m2m_off.assign(K + 1, 0);
+    T.m2m_idx.clear();
+    T.m2m_coef.clear();
+    for (int k = 0; k < K; k++) {
+        const int *a = &M.mi[3 * k];
+        for (int b0 = 0; b0 <= a[0]; b0++)
+            for (int b1 = 0; b1 <= a[1]; b1++)
+                for (int b2 = 0; b2 <= a[2]; b2++) {
+                    T.m2m_idx.push_back(M.idx(b0, b1, b2));
+                    T.m2m_idx.push_back(M.idx(a[0] - b0, a[1] - b1, a[2] - b2));
+                    T.m2m_coef.push_back(binom(a[0], b0) * binom(a[1], b1) * binom(a[2], b2));
+                }
+        T.m2m_off[k + 1] = (int)T.m2m_coef.size();
+    }
+    // L2L: for each g, (b, b-g, C(b,g)) over b >= g in index order
+    T.l2l_off.assign(K + 1, 0);
+    T.l2l_idx.clear();
+    T.l2l_coef.clear();
+    for (int kg = 0; kg < K; kg++) {
+        const int *g = &M.mi[3 * kg];
+        for (int kb = 0; kb < K; kb++) {
+            const int *b = &M.mi[3 * kb];
+            if (b[0] < g[0] || b[1] < g[1] || b[2] < g[2]) continue;
+            T.l2l_idx.push_back(kb);
+            T.l2l_idx.push_back(M.idx(b[0] - g[0], b[1] - g[1], b[2] - g[2]));
+            T.l2l_coef.push_back(binom(b[0], g[0]) * binom(b[1], g[1]) * binom(b[2], g[2]));
+        }
+        T.l2l_off[kg + 1] = (int)T.l2l_coef.size();
+    }
+    // M2L: for each b, (a, a+b, (-1)^|a| C(a+b, a)), |a| + |b| <= P, a in index order
+    T.m2l_off.assign(K + 1, 0);
+    T.m2l_idx.clear();
+    T.m2l_coef.clear();
+    for (int kb = 0; kb < K; kb++) {
+        const int *b = &M.mi[3 * kb];
+        for (int ka = 0; ka < K; ka++) {
+            const int *a = &M.mi[3 * ka];
+            const int na = a[0] + a[1] + a[2];
+            if (na + b[0] + b[1] + b[2] > P) continue;
+            T.m2l_idx.push_back(ka);
+            T.m2l_idx.push_back(M.idx(a[0] + b[0], a[1] + b[1], a[2] + b[2]));
+            T.m2l_coef.push_back((na % 2 ? -1.0 : 1.0) * binom(a[0] + b[0], a[0]) *
+                                 binom(a[1] + b[1], a[1]) * binom(a[2] + b[2], a[2]));
+        }
+        T.m2l_off[kb + 1] = (int)T.m2l_coef.size();
+    }
+    // L2P: idx(b + e_j) and idx(b + e_j + e_l)
+    T.d1.assign(4 * K, -1);
+    T.d2a.assign(4 * K, -1);
+    T.d2b.assign(4 * K, -1);
+    const int pj[6] = {0, 0, 0, 1, 1, 2}, plx[6] = {0, 1, 2, 1, 2, 2};
+    for (int k = 0; k < K; k++) {
+        const int *b = &M.mi[3 * k];
+        for (int j = 0; j < 3; j++) {
+            int e[3] = {b[0], b[1], b[2]};
+            e[j]++;
+            T.d1[4 * k + j] = M.idx(e[0], e[1], e[2]);
+        }
+        for (int m = 0; m < 6; m++) {
+            int e[3] = {b[0], b[1], b[2]};
+            e[pj[m]]++;
+            e[plx[m]]++;
+            const int v = M.idx(e[0], e[1], e[2]);
+            if (m < 3) T.d2a[4 * k + m] = v; else T.d2b[4 * k + m - 3] = v;
+        }
+    }
+}
+
+// Host version of the key computation and the whole plan, for checking the
+// tree and traversal against an independent CPU implementation.
+static uint64_t spread21_host(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+void fmm_plan_host(FmmPlan &pl, int64_t n, const double *X, const double *sig, int ncrit,
+                   double theta, double sigcut)
+{
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fmin(lo[k], X[3 * i + k]);
+            hi[k] = fmax(hi[k], X[3 * i + k]);
+        }
+    double ext = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    if (!(ext > 0.0)) ext = 1.0;
+    ext *= 1.0000001;
+    const double scale = 2097152.0 / ext;
+    std::vector<std::pair<uint64_t, int64_t>> ki(n);
+    for (int64_t i = 0; i < n; i++) {
+        uint64_t q[3];
+        for (int k = 0; k < 3; k++) {
+            double v = (X[3 * i + k] - lo[k]) * scale;
+            v = fmin(fmax(v, 0.0), 2097151.0);
+            q[k] = (uint64_t)v;
+        }
+        ki[i] = {spread21_host(q[0]) | (spread21_host(q[1]) << 1) | (spread21_host(q[2]) << 2), i};
+    }
+    std::sort(ki.begin(), ki.end());
+    std::vector<uint64_t> key(n);
+    std::vector<double> ss(n);
+    for (int64_t i = 0; i < n; i++) {
+        key[i] = ki[i].first;
+        ss[i] = sig[ki[i].second];
+    }
+    for (int k = 0; k < 3; k++) pl.lo[k] = lo[k];
+    pl.ext = ext;
+    fmm_build_tree(pl, key.data(), ss.data(), n, ncrit);
+    fmm_traverse(pl, theta, sigcut);
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+hipError_t launch_fmm_keys(hipStream_t st, const DeviceField &f, const double lo[3], double scale,
+                           uint64_t *kin, int32_t *vin)
+{
+    if (f.np == 0) return hipSuccess;
+    morton63_kernel<<<gridn(f.np, FMM_BLOCK), FMM_BLOCK, 0, st>>>(f.X, f.np, f.cap, lo[0], lo[1],
+                                                                   lo[2], scale, kin, vin);
+    return hipGetLastError();
+}
+
+size_t fmm_sort_temp_bytes(int64_t n)
+{
+    size_t bytes = 0;
+    (void)rocprim::radix_sort_pairs((void *)nullptr, bytes, (const uint64_t *)nullptr,
+                                    (uint64_t *)nullptr, (const int32_t *)nullptr,
+                                    (int32_t *)nullptr, (unsigned int)n, 0, 63, (hipStream_t)0);
+    return bytes;
+}
+
+hipError_t launch_fmm_sort(hipStream_t st, void *tmp, size_t bytes, const uint64_t *kin,
+                           uint64_t *kout, const int32_t *vin, int32_t *perm, int64_t n)
+{
+    if (n == 0) return hipSuccess;
+    return rocprim::radix_sort_pairs(tmp, bytes, kin, kout, vin, perm, (unsigned int)n, 0, 63, st);
+}
+
+hipError_t launch_fmm_gather(hipStream_t st, const DeviceField &f, const int32_t *perm,
+                             double4 *rec, double *ssig)
+{
+    if (f.np == 0) return hipSuccess;
+    fmm_gather_kernel<<<gridn(f.np, FMM_BLOCK), FMM_BLOCK, 0, st>>>(f.X, f.G, f.sigma, perm, f.np,
+                                                                     f.cap, rec, ssig);
+    return hipGetLastError();
+}
+
+hipError_t launch_fmm_eval(hipStream_t st, const FmmDev &d, const FmmPlan &pl, int kernel,
+                           int precision, const DeviceField &f, int reset)
+{
+    const int K = d.K, P = d.P;
+    const size_t shK = sizeof(double) * 4 * K;
+    if (d.nleaf > 0) {
+        p2m_kernel<<<d.nleaf, FMM_BLOCK, 0, st>>>(d.leaves, d.cells, d.rec, K, d.mi, d.M);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    // M2M bottom-up: parents level by level
+    for (int lev = (int)pl.level_parents.size() - 1; lev >= 0; lev--) {
+        const int cnt = pl.level_parents[lev].second;
+        if (cnt == 0) continue;
+        m2m_kernel<<<cnt, FMM_BLOCK, shK, st>>>(d.parents + pl.level_parents[lev].first, d.cells, K,
+                                               d.mi, d.m2m_off, d.m2m_idx, d.m2m_coef, d.M);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (d.ncell > 0) {
+        m2l_kernel<<<d.ncell, FMM_BLOCK, shK, st>>>(d.cells, d.m2l_off, d.m2l_src, K, P, d.mi,
+                                                   d.rec1, d.rec2, d.m2l_toff, d.m2l_tidx,
+                                                   d.m2l_tcoef, d.M, d.L);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    for (size_t lev = 1; lev < pl.level_kids.size(); lev++) {
+        const int cnt = pl.level_kids[lev].second;
+        if (cnt == 0) continue;
+        l2l_kernel<<<cnt, FMM_BLOCK, shK, st>>>(d.kids + pl.level_kids[lev].first, d.parent, d.cells,
+                                               K, d.mi, d.l2l_off, d.l2l_idx, d.l2l_coef, d.L);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (d.nleaf > 0) {
+        const size_t sh = sizeof(double) * (3 * K + P2P_BLOCK * 3 * (P + 1));
+        l2p_kernel<<<d.nleaf, P2P_BLOCK, sh, st>>>(d.leaves, d.cells, d.rec, d.perm, K, P, d.mi,
+                                                  d.d1, d.d2a, d.d2b, d.L, f.cap, f.U, f.J, reset);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+#define VPMX_P2P(KK)                                                                              \
+    if (precision == 0)                                                                           \
+        p2p_f64_kernel<KK><<<d.nleaf, P2P_BLOCK, 0, st>>>(d.leaves, d.cells, d.p2p_off, d.p2p_src, \
+                                                         d.rec, d.perm, f.cap, f.U, f.J);          \
+    else                                                                                          \
+        p2p_mixed_kernel<KK><<<d.nleaf, P2P_BLOCK, 0, st>>>(d.leaves, d.cells, d.p2p_off,          \
+                                                           d.p2p_src, d.rec, d.perm, f.cap, f.U,  \
+                                                           f.J);
+        switch (kernel) {
+        case SINGULAR: VPMX_P2P(SINGULAR); break;
+        case GAUSSIAN: VPMX_P2P(GAUSSIAN); break;
+        case GAUSSIANERF: VPMX_P2P(GAUSSIANERF); break;
+        case WINCKELMANS: VPMX_P2P(WINCKELMANS); break;
+        default: return hipErrorInvalidValue;
+        }
+#undef VPMX_P2P
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace vpmx

@@ -12,12 +12,12 @@
 namespace vpmx {
 
 // Timed kernel classes (vpmx_get_timing's `which`).
-enum TimedKind { T_UJ = 0, T_ESTR = 1, T_UPDATE = 2, T_NKIND = 3 };
+enum TimedKind { T_UJ = 0, T_ESTR = 1, T_UPDATE = 2, T_FMM = 3, T_NKIND = 4 };
 
 struct Timer {
     bool on = false;
-    double total_ms[T_NKIND] = {0, 0, 0};
-    int64_t count[T_NKIND] = {0, 0, 0};
+    double total_ms[T_NKIND] = {0, 0, 0, 0};
+    int64_t count[T_NKIND] = {0, 0, 0, 0};
     struct Pending { int kind; hipEvent_t a, b; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
@@ -46,11 +46,19 @@ struct Scratch {
     size_t sort_tmp_bytes = 0;
     double *part = nullptr;          // per-block partials (bbox, reductions)
     int64_t part_cap = 0;
+    // target-subset evaluation (sources = the resident field)
+    int64_t tcap = 0;
+    double *tX = nullptr, *tU = nullptr, *tJ = nullptr;   // SoA, stride = number of targets
+    double4 *tpos = nullptr;         // Morton-sorted targets (x, y, z, 0)
+    int32_t *tperm = nullptr;        // sorted slot -> target index
 };
+
+struct FmmState;   // fmm.h
 
 }  // namespace vpmx
 
 struct vpmx_ctx {
+    vpmx::FmmState *fmm = nullptr;
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -72,6 +80,16 @@ hipError_t launch_uj_f64(hipStream_t st, int kernel, const DeviceField &f, const
 hipError_t launch_sort_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, Scratch &s);
 hipError_t launch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
                            int reset);
+hipError_t launch_uj_f64_targets(hipStream_t st, int kernel, const DeviceField &f,
+                                 const Scratch &s, const double *tX, int64_t nt, double *U,
+                                 double *J);
+hipError_t launch_uj_mixed2_targets(hipStream_t st, int kernel, const DeviceField &f,
+                                    const Scratch &s, const double4 *tgt, int64_t nt,
+                                    const int32_t *tperm, int64_t ldo, double *U, double *J);
+hipError_t launch_uj_mixed_targets(hipStream_t st, int kernel, const DeviceField &f,
+                                   const Scratch &s, const double4 *tgt, int64_t nt,
+                                   const int32_t *tperm, int64_t ldo, double *U, double *J,
+                                   int reset);
 hipError_t launch_estr_f64(hipStream_t st, int kernel, int transposed, const DeviceField &f,
                            Scratch &s);
 hipError_t launch_sfs_coeff(hipStream_t st, const DeviceField &f, double Cs, int clip);
@@ -82,6 +100,7 @@ hipError_t launch_update(hipStream_t st, const DeviceField &f, int transposed, i
 hipError_t launch_relax(hipStream_t st, const DeviceField &f, int kind, double rlxf);
 hipError_t launch_diagnostics(hipStream_t st, const DeviceField &f, Scratch &s, double *out2);
 size_t sort_temp_bytes(int64_t n);
+hipError_t launch_bbox(hipStream_t st, const DeviceField &f, double *part, int *nparts);
 
 double zeta0_of(int kernel);
 

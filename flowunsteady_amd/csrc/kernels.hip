@@ -128,6 +128,31 @@ hipError_t launch_pack_f64(hipStream_t st, const DeviceField &f, Scratch &s)
     return hipGetLastError();
 }
 
+hipError_t launch_uj_f64_targets(hipStream_t st, int kernel, const DeviceField &f,
+                                 const Scratch &s, const double *tX, int64_t nt, double *U,
+                                 double *J)
+{
+    if (nt == 0 || f.np == 0) return hipSuccess;
+    const unsigned g = grid_for(nt, UJ_BLOCK);
+    switch (kernel) {
+    case SINGULAR:
+        uj_f64_kernel<SINGULAR><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, tX, nt, nt, U, J, 1);
+        break;
+    case GAUSSIAN:
+        uj_f64_kernel<GAUSSIAN><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, tX, nt, nt, U, J, 1);
+        break;
+    case GAUSSIANERF:
+        uj_f64_kernel<GAUSSIANERF><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, tX, nt, nt, U, J, 1);
+        break;
+    case WINCKELMANS:
+        uj_f64_kernel<WINCKELMANS><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, tX, nt, nt, U, J, 1);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_uj_f64(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
                          int reset)
 {
@@ -254,68 +279,14 @@ __global__ __launch_bounds__(EW_BLOCK) void pack_mixed_kernel(const double *__re
     sgam[s] = make_float4((float)G[i], (float)G[cap + i], (float)G[2 * cap + i], (float)p1);
 }
 
-// FP32 tile partials of one target: U, W = sum A Gamma (Kronecker part of J)
-// and eight radial entries of J (the ninth follows from trace J = 0).
-struct MixedAcc {
-    float U0 = 0.f, U1 = 0.f, U2 = 0.f, W0 = 0.f, W1 = 0.f, W2 = 0.f;
-    float J0 = 0.f, J1 = 0.f, J2 = 0.f, J3 = 0.f, J4 = 0.f, J5 = 0.f, J6 = 0.f, J7 = 0.f;
-};
-
-// One pair in FP32.  a = (source position relative to the block origin, p0),
-// b = (Gamma, p1).  When every lane of the wave is in the far field
-// (rho^2 >= 49) the gaussian kernels skip exp/erf: g = 1, g' = 0 to FP32
-// rounding.  Coincident pairs contribute nothing.
 template <int K>
-__device__ __forceinline__ void mixed_pair(const float4 a, const float4 b, float xi, float yi,
-                                           float zi, MixedAcc &acc)
-{
-    const float dx = xi - a.x, dy = yi - a.y, dz = zi - a.z;
-    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    const bool self = !(r2 > 0.f);
-    const float rinv = self ? 0.f : frsq(r2);
-    float A, B;
-    if constexpr (K == GAUSSIANERF || K == GAUSSIAN) {
-        const bool near = r2 * b.w < RHO2_FAR;
-        if (__any(near)) {
-            radial_near_f32<K>(r2, rinv, a.w, b.w, A, B);
-        } else {
-            const float ri2 = rinv * rinv;
-            A = ri2 * rinv;
-            B = -3.0f * A * ri2;
-        }
-    } else {
-        radial_near_f32<K>(r2, rinv, a.w, b.w, A, B);
-    }
-    A = self ? 0.f : A;
-    B = self ? 0.f : B;
-    // c = dX x Gamma
-    const float c0 = fmaf(dy, b.z, -dz * b.y);
-    const float c1 = fmaf(dz, b.x, -dx * b.z);
-    const float c2 = fmaf(dx, b.y, -dy * b.x);
-    acc.U0 = fmaf(A, c0, acc.U0);
-    acc.U1 = fmaf(A, c1, acc.U1);
-    acc.U2 = fmaf(A, c2, acc.U2);
-    acc.W0 = fmaf(A, b.x, acc.W0);
-    acc.W1 = fmaf(A, b.y, acc.W1);
-    acc.W2 = fmaf(A, b.z, acc.W2);
-    const float ex = B * dx, ey = B * dy, ez = B * dz;
-    // radial part c_k dX_l B, column-major (k + 3 l)
-    acc.J0 = fmaf(c0, ex, acc.J0);
-    acc.J1 = fmaf(c1, ex, acc.J1);
-    acc.J2 = fmaf(c2, ex, acc.J2);
-    acc.J3 = fmaf(c0, ey, acc.J3);
-    acc.J4 = fmaf(c1, ey, acc.J4);
-    acc.J5 = fmaf(c2, ey, acc.J5);
-    acc.J6 = fmaf(c0, ez, acc.J6);
-    acc.J7 = fmaf(c1, ez, acc.J7);
-}
-
-template <int K>
-__global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_kernel(const double4 *__restrict__ spos,
-                                                            const float4 *__restrict__ sgam,
-                                                            int64_t n,
+__global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_kernel(const double4 *__restrict__ tgt,
+                                                            int64_t nt,
                                                             const int32_t *__restrict__ perm,
-                                                            int64_t cap, double *__restrict__ U,
+                                                            const double4 *__restrict__ spos,
+                                                            const float4 *__restrict__ sgam,
+                                                            int64_t n, int64_t cap,
+                                                            double *__restrict__ U,
                                                             double *__restrict__ J, int reset)
 {
     __shared__ float4 tpos[UJ_BLOCK];
@@ -323,12 +294,12 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_kernel(const double4 *__res
     const int t = threadIdx.x;
     const int64_t s0 = (int64_t)blockIdx.x * UJ_BLOCK;
     const int64_t s = s0 + t;
-    const bool active = s < n;
+    const bool active = s < nt;
     // Block origin: the block's first (Morton-sorted) target, in FP64.
-    const double4 o = spos[s0];
+    const double4 o = tgt[s0];
     float xi = 0.f, yi = 0.f, zi = 0.f;
     if (active) {
-        const double4 p = spos[s];
+        const double4 p = tgt[s];
         xi = (float)(p.x - o.x);
         yi = (float)(p.y - o.y);
         zi = (float)(p.z - o.z);
@@ -381,6 +352,110 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_kernel(const double4 *__res
     J[6 * cap + i] = j6; J[7 * cap + i] = j7; J[8 * cap + i] = j8;
 }
 
+// Target-subset variant (probes): targets need not be spatially compact, so
+// both target and source carry their offset from the block origin as an FP32
+// (hi, lo) pair and dX = (t_hi - s_hi) + (t_lo - s_lo); the hi difference is
+// exact for the nearby pairs that matter, leaving dX accurate to ~1e-15
+// absolute for any block extent.
+template <int K>
+__global__ __launch_bounds__(UJ_BLOCK) void uj_mixed2_kernel(const double4 *__restrict__ tgt,
+                                                             int64_t nt,
+                                                             const int32_t *__restrict__ perm,
+                                                             const double4 *__restrict__ spos,
+                                                             const float4 *__restrict__ sgam,
+                                                             int64_t n, int64_t ldo,
+                                                             double *__restrict__ U,
+                                                             double *__restrict__ J)
+{
+    __shared__ float4 shi[UJ_BLOCK];
+    __shared__ float4 slo[UJ_BLOCK];
+    __shared__ float4 sgm[UJ_BLOCK];
+    const int t = threadIdx.x;
+    const int64_t s0 = (int64_t)blockIdx.x * UJ_BLOCK;
+    const int64_t s = s0 + t;
+    const bool active = s < nt;
+    const double4 o = tgt[s0];
+    float xh = 0.f, yh = 0.f, zh = 0.f, xl = 0.f, yl = 0.f, zl = 0.f;
+    if (active) {
+        const double4 p = tgt[s];
+        const double dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+        xh = (float)dx; yh = (float)dy; zh = (float)dz;
+        xl = (float)(dx - xh); yl = (float)(dy - yh); zl = (float)(dz - zh);
+    }
+    double dU0 = 0, dU1 = 0, dU2 = 0, dW0 = 0, dW1 = 0, dW2 = 0;
+    double dJ0 = 0, dJ1 = 0, dJ2 = 0, dJ3 = 0, dJ4 = 0, dJ5 = 0, dJ6 = 0, dJ7 = 0;
+    for (int64_t base = 0; base < n; base += UJ_BLOCK) {
+        __syncthreads();
+        const int64_t js = base + t;
+        if (js < n) {
+            const double4 p = spos[js];
+            const double dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+            const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
+            shi[t] = make_float4(hx, hy, hz, (float)p.w);
+            slo[t] = make_float4((float)(dx - hx), (float)(dy - hy), (float)(dz - hz), 0.f);
+            sgm[t] = sgam[js];
+        }
+        __syncthreads();
+        const int cnt = (int)((n - base) < UJ_BLOCK ? (n - base) : UJ_BLOCK);
+        MixedAcc acc;
+        for (int jj = 0; jj < cnt; ++jj) {
+            const float4 h = shi[jj], l = slo[jj];
+            mixed_pair_d<K>((xh - h.x) + (xl - l.x), (yh - h.y) + (yl - l.y),
+                            (zh - h.z) + (zl - l.z), h.w, sgm[jj], acc);
+        }
+        dU0 += acc.U0; dU1 += acc.U1; dU2 += acc.U2;
+        dW0 += acc.W0; dW1 += acc.W1; dW2 += acc.W2;
+        dJ0 += acc.J0; dJ1 += acc.J1; dJ2 += acc.J2; dJ3 += acc.J3;
+        dJ4 += acc.J4; dJ5 += acc.J5; dJ6 += acc.J6; dJ7 += acc.J7;
+    }
+    if (!active) return;
+    const int64_t i = perm[s];
+    const double k = -INV4PI;
+    U[i] = k * dU0; U[ldo + i] = k * dU1; U[2 * ldo + i] = k * dU2;
+    J[i] = k * dJ0;
+    J[ldo + i] = k * (dJ1 - dW2);
+    J[2 * ldo + i] = k * (dJ2 + dW1);
+    J[3 * ldo + i] = k * (dJ3 + dW2);
+    J[4 * ldo + i] = k * dJ4;
+    J[5 * ldo + i] = k * (dJ5 - dW0);
+    J[6 * ldo + i] = k * (dJ6 - dW1);
+    J[7 * ldo + i] = k * (dJ7 + dW0);
+    J[8 * ldo + i] = -k * (dJ0 + dJ4);
+}
+
+hipError_t launch_uj_mixed2_targets(hipStream_t st, int kernel, const DeviceField &f,
+                                    const Scratch &s, const double4 *tgt, int64_t nt,
+                                    const int32_t *tperm, int64_t ldo, double *U, double *J)
+{
+    if (nt == 0 || f.np == 0) return hipSuccess;
+    const unsigned g = grid_for(nt, UJ_BLOCK);
+    switch (kernel) {
+    case SINGULAR:
+        uj_mixed2_kernel<SINGULAR><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, f.np, ldo, U, J);
+        break;
+    case GAUSSIAN:
+        uj_mixed2_kernel<GAUSSIAN><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, f.np, ldo, U, J);
+        break;
+    case GAUSSIANERF:
+        uj_mixed2_kernel<GAUSSIANERF><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, f.np, ldo, U, J);
+        break;
+    case WINCKELMANS:
+        uj_mixed2_kernel<WINCKELMANS><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, f.np, ldo, U, J);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bbox(hipStream_t st, const DeviceField &f, double *part, int *nparts)
+{
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, grid_for(f.np, EW_BLOCK)));
+    *nparts = nb;
+    bbox_partial_kernel<<<nb, EW_BLOCK, 0, st>>>(f.X, f.np, f.cap, part);
+    return hipGetLastError();
+}
+
 size_t sort_temp_bytes(int64_t n)
 {
     size_t bytes = 0;
@@ -429,21 +504,29 @@ hipError_t launch_sort_pack_mixed(hipStream_t st, int kernel, const DeviceField 
 hipError_t launch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
                            int reset)
 {
+    return launch_uj_mixed_targets(st, kernel, f, s, s.spos, f.np, s.perm, f.cap, f.U, f.J, reset);
+}
+
+hipError_t launch_uj_mixed_targets(hipStream_t st, int kernel, const DeviceField &f,
+                                   const Scratch &s, const double4 *tgt, int64_t nt,
+                                   const int32_t *tperm, int64_t ldo, double *U, double *J,
+                                   int reset)
+{
     const int64_t n = f.np;
-    if (n == 0) return hipSuccess;
-    const unsigned g = grid_for(n, UJ_BLOCK);
+    if (nt == 0) return hipSuccess;
+    const unsigned g = grid_for(nt, UJ_BLOCK);
     switch (kernel) {
     case SINGULAR:
-        uj_mixed_kernel<SINGULAR><<<g, UJ_BLOCK, 0, st>>>(s.spos, s.sgam, n, s.perm, f.cap, f.U, f.J, reset);
+        uj_mixed_kernel<SINGULAR><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset);
         break;
     case GAUSSIAN:
-        uj_mixed_kernel<GAUSSIAN><<<g, UJ_BLOCK, 0, st>>>(s.spos, s.sgam, n, s.perm, f.cap, f.U, f.J, reset);
+        uj_mixed_kernel<GAUSSIAN><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset);
         break;
     case GAUSSIANERF:
-        uj_mixed_kernel<GAUSSIANERF><<<g, UJ_BLOCK, 0, st>>>(s.spos, s.sgam, n, s.perm, f.cap, f.U, f.J, reset);
+        uj_mixed_kernel<GAUSSIANERF><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset);
         break;
     case WINCKELMANS:
-        uj_mixed_kernel<WINCKELMANS><<<g, UJ_BLOCK, 0, st>>>(s.spos, s.sgam, n, s.perm, f.cap, f.U, f.J, reset);
+        uj_mixed_kernel<WINCKELMANS><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset);
         break;
     default:
         return hipErrorInvalidValue;

@@ -9,10 +9,13 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "device_math.h"
+#include "fmm.h"
 #include "internal.h"
 
 using namespace vpmx;
@@ -153,6 +156,8 @@ void free_scratch(Scratch &s)
     if (s.perm) (void)hipFree(s.perm);
     if (s.sort_tmp) (void)hipFree(s.sort_tmp);
     if (s.part) (void)hipFree(s.part);
+    for (void *p : {(void *)s.tX, (void *)s.tU, (void *)s.tJ, (void *)s.tpos, (void *)s.tperm})
+        if (p) (void)hipFree(p);
     s = Scratch();
 }
 
@@ -240,8 +245,17 @@ int check_uj_params(vpmx_ctx *c, const vpmx_uj_params *P)
         return fail(c, VPMX_EINVAL, "unknown sfs model " + std::to_string(P->sfs));
     if (P->sfs && P->kernel == VPMX_KERNEL_SINGULAR)
         return fail(c, VPMX_EINVAL, "the SFS model needs a smooth kernel (zeta of singular is a delta)");
-    if (P->fmm != 0)
-        return fail(c, VPMX_EUNSUPPORTED, "fmm != 0: only the direct scheme is built");
+    if (P->fmm != 0 && P->fmm != 1)
+        return fail(c, VPMX_EUNSUPPORTED, "fmm must be 0 (direct) or 1 (fast multipole)");
+    if (P->fmm) {
+        if (P->p < 0 || P->p > FMM_MAX_P)
+            return fail(c, VPMX_EINVAL, "FMM expansion order p must be in [0, " +
+                                            std::to_string(FMM_MAX_P) + "]");
+        if (P->ncrit < 1) return fail(c, VPMX_EINVAL, "FMM ncrit must be positive");
+        if (!(P->theta > 0.0) || !isfinite(P->theta))
+            return fail(c, VPMX_EINVAL, "FMM theta must be positive and finite");
+        if (P->sfs) return fail(c, VPMX_EUNSUPPORTED, "Estr with the FMM is not built (use fmm = 0)");
+    }
     return VPMX_OK;
 }
 
@@ -368,10 +382,289 @@ int download(vpmx_ctx *c, double *P, const vpmx_layout *L, int what)
     return VPMX_OK;
 }
 
+// ---- fast multipole UJ ------------------------------------------------------------------
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+double default_sigma_cut(int kernel)
+{
+    switch (kernel) {
+    case VPMX_KERNEL_GAUSSIANERF: return 7.0;   // 1 - g(7) ~ 1e-10
+    case VPMX_KERNEL_GAUSSIAN: return 3.0;      // e^{-27}
+    case VPMX_KERNEL_WINCKELMANS: return 20.0;  // 1 - g ~ 15/(8 rho^4) ~ 1e-5
+    default: return 0.0;
+    }
+}
+
+template <class T>
+int upload_vec(vpmx_ctx *c, T *dst, const std::vector<T> &v)
+{
+    if (v.empty()) return VPMX_OK;
+    VPMX_HIP(c, hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return VPMX_OK;
+}
+
+template <class D, class S>
+int upload_as(vpmx_ctx *c, D **dst, const std::vector<S> &v)
+{
+    const size_t bytes = v.size() * sizeof(S);
+    VPMX_TRY(dalloc(c, (uint8_t **)dst, bytes > 0 ? bytes : 16));
+    if (bytes) VPMX_HIP(c, hipMemcpy(*dst, v.data(), bytes, hipMemcpyHostToDevice));
+    return VPMX_OK;
+}
+
+int fmm_tables_to_device(vpmx_ctx *c, FmmState &S)
+{
+    const FmmTables &T = S.tab;
+    VPMX_TRY(upload_as(c, &S.mi, T.mi));
+    VPMX_TRY(upload_as(c, &S.rec1, T.rec1));
+    VPMX_TRY(upload_as(c, &S.rec2, T.rec2));
+    VPMX_TRY(upload_as(c, &S.d1, T.d1));
+    VPMX_TRY(upload_as(c, &S.d2a, T.d2a));
+    VPMX_TRY(upload_as(c, &S.d2b, T.d2b));
+    VPMX_TRY(upload_as(c, &S.m2m_off, T.m2m_off));
+    VPMX_TRY(upload_as(c, &S.m2m_idx, T.m2m_idx));
+    VPMX_TRY(upload_as(c, &S.m2m_coef, T.m2m_coef));
+    VPMX_TRY(upload_as(c, &S.l2l_off, T.l2l_off));
+    VPMX_TRY(upload_as(c, &S.l2l_idx, T.l2l_idx));
+    VPMX_TRY(upload_as(c, &S.l2l_coef, T.l2l_coef));
+    VPMX_TRY(upload_as(c, &S.m2l_toff, T.m2l_off));
+    VPMX_TRY(upload_as(c, &S.m2l_tidx, T.m2l_idx));
+    VPMX_TRY(upload_as(c, &S.m2l_tcoef, T.m2l_coef));
+    S.dev_tab_p = T.P;
+    return VPMX_OK;
+}
+
+void free_fmm(FmmState *S)
+{
+    if (!S) return;
+    void *ptrs[] = {S->mi, S->m2m_off, S->l2l_off, S->m2l_toff, S->rec1, S->rec2, S->d1, S->d2a,
+                    S->d2b, S->m2m_idx, S->l2l_idx, S->m2l_tidx, S->m2m_coef, S->l2l_coef,
+                    S->m2l_tcoef, S->cells, S->leaves, S->parents, S->kids, S->parent,
+                    S->m2l_off, S->p2p_off, S->m2l_src, S->p2p_src, S->M, S->L, S->kin, S->kout,
+                    S->ssig, S->sort_tmp};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (S->h_keys) (void)hipHostFree(S->h_keys);
+    if (S->h_ssig) (void)hipHostFree(S->h_ssig);
+    delete S;
+}
+
+int fmm_reserve_n(vpmx_ctx *c, FmmState &S, int64_t n)
+{
+    if (n <= S.cap_n) return VPMX_OK;
+    const int64_t cap = std::max<int64_t>(n, S.cap_n + S.cap_n / 2);
+    VPMX_TRY(dalloc(c, &S.kin, cap));
+    VPMX_TRY(dalloc(c, &S.kout, cap));
+    VPMX_TRY(dalloc(c, &S.ssig, cap));
+    S.sort_bytes = fmm_sort_temp_bytes(cap);
+    VPMX_TRY(dalloc(c, &S.sort_tmp, S.sort_bytes));
+    if (S.h_keys) (void)hipHostFree(S.h_keys);
+    if (S.h_ssig) (void)hipHostFree(S.h_ssig);
+    S.h_keys = nullptr;
+    S.h_ssig = nullptr;
+    VPMX_HIP(c, hipHostMalloc((void **)&S.h_keys, cap * sizeof(uint64_t), hipHostMallocDefault));
+    VPMX_HIP(c, hipHostMalloc((void **)&S.h_ssig, cap * sizeof(double), hipHostMallocDefault));
+    S.cap_n = cap;
+    return VPMX_OK;
+}
+
+int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
+{
+    if (!c->fmm) c->fmm = new (std::nothrow) FmmState();
+    if (!c->fmm) return fail(c, VPMX_ENOMEM, "out of host memory");
+    FmmState &S = *c->fmm;
+    DeviceField &f = c->f;
+    const int64_t n = f.np;
+    for (double &v : S.stats) v = 0.0;
+    if (n == 0) return VPMX_OK;
+    const double sigcut = P->sigma_cut > 0.0 ? P->sigma_cut : default_sigma_cut(P->kernel);
+    hipStream_t st = c->stream;
+
+    // bounding cube (exact min/max, same arithmetic as the oracle)
+    int nb = 0;
+    VPMX_HIP(c, launch_bbox(st, f, c->s.part, &nb));
+    std::vector<double> hp(6 * (size_t)nb);
+    VPMX_HIP(c, hipMemcpyAsync(hp.data(), c->s.part, hp.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+    VPMX_TRY(sync(c));
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+        lo[k] = hp[k];
+        hi[k] = hp[3 + k];
+        for (int b = 1; b < nb; b++) {
+            lo[k] = fmin(lo[k], hp[6 * b + k]);
+            hi[k] = fmax(hi[k], hp[6 * b + 3 + k]);
+        }
+    }
+    double ext = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    if (!(ext > 0.0)) ext = 1.0;
+    ext *= 1.0000001;
+    const double scale = 2097152.0 / ext;
+
+    // keys, sort, sorted records
+    VPMX_TRY(fmm_reserve_n(c, S, n));
+    VPMX_HIP(c, launch_fmm_keys(st, f, lo, scale, S.kin, c->s.vin));
+    VPMX_HIP(c, launch_fmm_sort(st, S.sort_tmp, S.sort_bytes, S.kin, S.kout, c->s.vin, c->s.perm, n));
+    VPMX_HIP(c, launch_fmm_gather(st, f, c->s.perm, c->s.src64, S.ssig));
+    VPMX_HIP(c, hipMemcpyAsync(S.h_keys, S.kout, n * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    VPMX_HIP(c, hipMemcpyAsync(S.h_ssig, S.ssig, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    VPMX_TRY(sync(c));
+
+    // host: tree and traversal
+    FmmPlan &pl = S.plan;
+    const double t0 = now_ms();
+    for (int k = 0; k < 3; k++) pl.lo[k] = lo[k];
+    pl.ext = ext;
+    fmm_build_tree(pl, S.h_keys, S.h_ssig, n, P->ncrit);
+    const double t1 = now_ms();
+    fmm_traverse(pl, P->theta, sigcut);
+    const double t2 = now_ms();
+    const int64_t nc = (int64_t)pl.cells.size();
+    int maxlev = 0;
+    for (const auto &C : pl.cells) maxlev = std::max(maxlev, C.level);
+    pl.leaves.clear();
+    pl.parents.clear();
+    pl.kids.clear();
+    pl.parent.assign(nc, -1);
+    pl.level_parents.assign(maxlev + 1, {0, 0});
+    pl.level_kids.assign(maxlev + 1, {0, 0});
+    std::vector<FmmCellDev> hc(nc);
+    for (int64_t i = 0; i < nc; i++) {
+        const FmmCellHost &C = pl.cells[i];
+        hc[i] = FmmCellDev{(int32_t)C.begin, (int32_t)C.end, (int32_t)C.child0, C.nchild,
+                           C.c[0], C.c[1], C.c[2]};
+        if (C.nchild == 0) pl.leaves.push_back((int32_t)i);
+        for (int k = 0; k < C.nchild; k++) pl.parent[C.child0 + k] = (int32_t)i;
+    }
+    for (int lev = 0; lev <= maxlev; lev++) {
+        pl.level_parents[lev].first = (int)pl.parents.size();
+        pl.level_kids[lev].first = (int)pl.kids.size();
+        for (int64_t i = 0; i < nc; i++) {
+            if (pl.cells[i].level != lev) continue;
+            if (pl.cells[i].nchild) pl.parents.push_back((int32_t)i);
+            if (lev > 0) pl.kids.push_back((int32_t)i);
+        }
+        pl.level_parents[lev].second = (int)pl.parents.size() - pl.level_parents[lev].first;
+        pl.level_kids[lev].second = (int)pl.kids.size() - pl.level_kids[lev].first;
+    }
+    int64_t pairs = 0;
+    for (int32_t a : pl.leaves) {
+        int64_t s = 0;
+        for (int64_t e = pl.p2p_off[a]; e < pl.p2p_off[a + 1]; e++) {
+            const FmmCellHost &B = pl.cells[pl.p2p_src[e]];
+            s += B.end - B.begin;
+        }
+        pairs += s * (pl.cells[a].end - pl.cells[a].begin);
+    }
+    pl.p2p_pairs = pairs;
+
+    // device: tables (once per p), tree, lists, expansions
+    if (S.dev_tab_p != P->p) {
+        fmm_build_tables(S.tab, P->p);
+        VPMX_TRY(fmm_tables_to_device(c, S));
+    }
+    const int K = S.tab.K;
+    if (nc > S.cap_cells) {
+        const int64_t cap = std::max<int64_t>(nc, S.cap_cells + S.cap_cells / 2);
+        VPMX_TRY(dalloc(c, &S.cells, cap));
+        VPMX_TRY(dalloc(c, &S.leaves, cap));
+        VPMX_TRY(dalloc(c, &S.parents, cap));
+        VPMX_TRY(dalloc(c, &S.kids, cap));
+        VPMX_TRY(dalloc(c, &S.parent, cap));
+        VPMX_TRY(dalloc(c, &S.m2l_off, cap + 1));
+        VPMX_TRY(dalloc(c, &S.p2p_off, cap + 1));
+        S.cap_cells = cap;
+    }
+    if (nc * 3 * K > S.cap_coef) {
+        const int64_t cap = std::max<int64_t>(nc * 3 * K, S.cap_coef + S.cap_coef / 2);
+        VPMX_TRY(dalloc(c, &S.M, cap));
+        VPMX_TRY(dalloc(c, &S.L, cap));
+        S.cap_coef = cap;
+    }
+    const int64_t nm2l = (int64_t)pl.m2l_src.size(), np2p = (int64_t)pl.p2p_src.size();
+    if (nm2l > S.cap_m2l || !S.m2l_src) {
+        const int64_t cap = std::max<int64_t>({nm2l, S.cap_m2l + S.cap_m2l / 2, 1});
+        VPMX_TRY(dalloc(c, &S.m2l_src, cap));
+        S.cap_m2l = cap;
+    }
+    if (np2p > S.cap_p2p || !S.p2p_src) {
+        const int64_t cap = std::max<int64_t>({np2p, S.cap_p2p + S.cap_p2p / 2, 1});
+        VPMX_TRY(dalloc(c, &S.p2p_src, cap));
+        S.cap_p2p = cap;
+    }
+    VPMX_TRY(upload_vec(c, S.cells, hc));
+    VPMX_TRY(upload_vec(c, S.leaves, pl.leaves));
+    VPMX_TRY(upload_vec(c, S.parents, pl.parents));
+    VPMX_TRY(upload_vec(c, S.kids, pl.kids));
+    VPMX_TRY(upload_vec(c, S.parent, pl.parent));
+    VPMX_TRY(upload_vec(c, S.m2l_off, pl.m2l_off));
+    VPMX_TRY(upload_vec(c, S.p2p_off, pl.p2p_off));
+    VPMX_TRY(upload_vec(c, S.m2l_src, pl.m2l_src));
+    VPMX_TRY(upload_vec(c, S.p2p_src, pl.p2p_src));
+    VPMX_TRY(sync(c));
+    const double t3 = now_ms();
+
+    FmmDev d;
+    d.P = P->p;
+    d.K = K;
+    d.ncell = (int)nc;
+    d.nleaf = (int)pl.leaves.size();
+    d.cells = S.cells;
+    d.leaves = S.leaves;
+    d.parents = S.parents;
+    d.kids = S.kids;
+    d.parent = S.parent;
+    d.m2l_off = S.m2l_off;
+    d.p2p_off = S.p2p_off;
+    d.m2l_src = S.m2l_src;
+    d.p2p_src = S.p2p_src;
+    d.rec = c->s.src64;
+    d.perm = c->s.perm;
+    d.M = S.M;
+    d.L = S.L;
+    d.mi = S.mi;
+    d.rec1 = S.rec1;
+    d.rec2 = S.rec2;
+    d.d1 = S.d1;
+    d.d2a = S.d2a;
+    d.d2b = S.d2b;
+    d.m2m_off = S.m2m_off;
+    d.l2l_off = S.l2l_off;
+    d.m2l_toff = S.m2l_toff;
+    d.m2m_idx = S.m2m_idx;
+    d.l2l_idx = S.l2l_idx;
+    d.m2l_tidx = S.m2l_tidx;
+    d.m2m_coef = S.m2m_coef;
+    d.l2l_coef = S.l2l_coef;
+    d.m2l_tcoef = S.m2l_tcoef;
+    const double ms_before = c->timer.total_ms[T_FMM];
+    {
+        TimedScope ts(c, T_FMM);
+        VPMX_HIP(c, launch_fmm_eval(st, d, pl, P->kernel, P->precision, f, reset));
+    }
+    VPMX_TRY(sync(c));
+    S.stats[0] = t1 - t0;
+    S.stats[1] = t2 - t1;
+    S.stats[2] = t3 - t2;
+    S.stats[3] = (double)nc;
+    S.stats[4] = (double)pl.leaves.size();
+    S.stats[5] = (double)nm2l;
+    S.stats[6] = (double)np2p;
+    S.stats[7] = (double)pairs;
+    S.stats[8] = c->timer.total_ms[T_FMM] - ms_before;
+    return VPMX_OK;
+}
+
 // ---- device sequences (enqueue only; callers sync) ---------------------------------
 int enqueue_uj(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
 {
     const DeviceField &f = c->f;
+    if (P->fmm) return run_fmm(c, P, reset);
     if (P->precision == VPMX_FP64) {
         VPMX_HIP(c, launch_pack_f64(c->stream, f, c->s));
         TimedScope ts(c, T_UJ);
@@ -486,6 +779,8 @@ void vpmx_destroy(vpmx_ctx *c)
     for (auto e : c->timer.pool) (void)hipEventDestroy(e);
     free_field(c->f);
     free_scratch(c->s);
+    free_fmm(c->fmm);
+    c->fmm = nullptr;
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -619,6 +914,131 @@ int vpmx_get_timing(vpmx_ctx *c, int32_t which, double *total_ms, int64_t *count
     if (which < 0 || which >= T_NKIND) return fail(c, VPMX_EINVAL, "unknown timing class");
     if (total_ms) *total_ms = c->timer.total_ms[which];
     if (count) *count = c->timer.count[which];
+    return VPMX_OK;
+}
+
+int vpmx_uj_targets(vpmx_ctx *c, const double *targets, int64_t m, double *Uout, double *Jout,
+                    const vpmx_uj_params *P)
+{
+    VPMX_TRY(entry(c));
+    VPMX_TRY(check_uj_params(c, P));
+    if (P->fmm) return fail(c, VPMX_EUNSUPPORTED, "target evaluation uses the direct scheme (fmm = 0)");
+    if (P->sfs) return fail(c, VPMX_EUNSUPPORTED, "target evaluation has no SFS term");
+    if (m < 0) return fail(c, VPMX_EINVAL, "number of targets is negative");
+    if (m > (int64_t)INT32_MAX) return fail(c, VPMX_EINVAL, "number of targets exceeds 2^31 - 1");
+    if (m > 0 && (!targets || !Uout || !Jout)) return fail(c, VPMX_EINVAL, "targets or outputs NULL");
+    for (int64_t i = 0; i < 3 * m; i++)
+        if (!isfinite(targets[i])) return fail(c, VPMX_EINVAL, "non-finite target position");
+    if (m == 0) return VPMX_OK;
+    const DeviceField &f = c->f;
+    if (f.np == 0) {
+        memset(Uout, 0, sizeof(double) * 3 * m);
+        memset(Jout, 0, sizeof(double) * 9 * m);
+        return VPMX_OK;
+    }
+    Scratch &s = c->s;
+    if (m > s.tcap) {
+        const int64_t cap = std::max<int64_t>(m, s.tcap + s.tcap / 2);
+        VPMX_TRY(dalloc(c, &s.tX, 3 * cap));
+        VPMX_TRY(dalloc(c, &s.tU, 3 * cap));
+        VPMX_TRY(dalloc(c, &s.tJ, 9 * cap));
+        VPMX_TRY(dalloc(c, &s.tpos, cap));
+        VPMX_TRY(dalloc(c, &s.tperm, cap));
+        s.tcap = cap;
+    }
+    VPMX_TRY(ensure_stage(c, (size_t)12 * m * sizeof(double)));
+    hipStream_t st = c->stream;
+    if (P->precision == VPMX_FP64) {
+        double *h = c->h_stage;
+        for (int64_t i = 0; i < m; i++)
+            for (int k = 0; k < 3; k++) h[k * m + i] = targets[3 * i + k];
+        VPMX_HIP(c, hipMemcpyAsync(s.tX, h, 3 * m * sizeof(double), hipMemcpyHostToDevice, st));
+        VPMX_HIP(c, launch_pack_f64(st, f, s));
+        TimedScope ts(c, T_UJ);
+        VPMX_HIP(c, launch_uj_f64_targets(st, P->kernel, f, s, s.tX, m, s.tU, s.tJ));
+    } else {
+        // Morton order of the targets (30-bit keys in their bounding cube), so
+        // each block's targets sit near the block origin.
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = 0; i < m; i++)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = fmin(lo[k], targets[3 * i + k]);
+                hi[k] = fmax(hi[k], targets[3 * i + k]);
+            }
+        const double ext = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+        const double scale = ext > 0.0 ? 1023.999 / ext : 0.0;
+        auto spread = [](uint32_t v) {
+            v &= 0x3ffu;
+            v = (v | (v << 16)) & 0x030000ffu;
+            v = (v | (v << 8)) & 0x0300f00fu;
+            v = (v | (v << 4)) & 0x030c30c3u;
+            v = (v | (v << 2)) & 0x09249249u;
+            return v;
+        };
+        std::vector<std::pair<uint32_t, int32_t>> ki(m);
+        for (int64_t i = 0; i < m; i++) {
+            uint32_t q[3];
+            for (int k = 0; k < 3; k++)
+                q[k] = (uint32_t)fmin(fmax((targets[3 * i + k] - lo[k]) * scale, 0.0), 1023.0);
+            ki[i] = {spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2), (int32_t)i};
+        }
+        std::sort(ki.begin(), ki.end());
+        double4 *hp = (double4 *)c->h_stage;
+        int32_t *hperm = (int32_t *)(c->h_stage + 4 * m);
+        for (int64_t j = 0; j < m; j++) {
+            const int64_t i = ki[j].second;
+            hp[j] = make_double4(targets[3 * i], targets[3 * i + 1], targets[3 * i + 2], 0.0);
+            hperm[j] = (int32_t)i;
+        }
+        VPMX_HIP(c, hipMemcpyAsync(s.tpos, hp, m * sizeof(double4), hipMemcpyHostToDevice, st));
+        VPMX_HIP(c, hipMemcpyAsync(s.tperm, hperm, m * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        VPMX_HIP(c, launch_sort_pack_mixed(st, P->kernel, f, s));
+        TimedScope ts(c, T_UJ);
+        VPMX_HIP(c, launch_uj_mixed2_targets(st, P->kernel, f, s, s.tpos, m, s.tperm, m, s.tU,
+                                             s.tJ));
+    }
+    double *h = c->h_stage;
+    VPMX_HIP(c, hipMemcpyAsync(h, s.tU, 3 * m * sizeof(double), hipMemcpyDeviceToHost, st));
+    VPMX_HIP(c, hipMemcpyAsync(h + 3 * m, s.tJ, 9 * m * sizeof(double), hipMemcpyDeviceToHost, st));
+    VPMX_TRY(sync(c));
+    for (int64_t i = 0; i < m; i++) {
+        for (int k = 0; k < 3; k++) Uout[3 * i + k] = h[k * m + i];
+        for (int k = 0; k < 9; k++) Jout[9 * i + k] = h[(3 + k) * m + i];
+    }
+    return VPMX_OK;
+}
+
+int vpmx_fmm_plan_counts(int64_t n, const double *X, const double *sigma, int32_t ncrit,
+                         double theta, double sigma_cut, int64_t *out)
+{
+    if (n < 0 || (n > 0 && (!X || !sigma)) || !out || ncrit < 1)
+        return fail(nullptr, VPMX_EINVAL, "bad arguments");
+    FmmPlan pl;
+    fmm_plan_host(pl, n, X, sigma, ncrit, theta, sigma_cut);
+    int64_t nleaf = 0, pairs = 0;
+    for (const auto &C : pl.cells) {
+        if (C.nchild) continue;
+        nleaf++;
+    }
+    for (size_t a = 0; a < pl.cells.size(); a++) {
+        if (pl.cells[a].nchild) continue;
+        int64_t s = 0;
+        for (int64_t e = pl.p2p_off[a]; e < pl.p2p_off[a + 1]; e++)
+            s += pl.cells[pl.p2p_src[e]].end - pl.cells[pl.p2p_src[e]].begin;
+        pairs += s * (pl.cells[a].end - pl.cells[a].begin);
+    }
+    out[0] = (int64_t)pl.cells.size();
+    out[1] = nleaf;
+    out[2] = (int64_t)pl.m2l_src.size();
+    out[3] = (int64_t)pl.p2p_src.size();
+    out[4] = pairs;
+    return VPMX_OK;
+}
+
+int vpmx_fmm_stats(vpmx_ctx *c, double *out, int32_t n)
+{
+    if (!c || !out) return fail(c, VPMX_EINVAL, "context or output is NULL");
+    for (int k = 0; k < n && k < 9; k++) out[k] = c->fmm ? c->fmm->stats[k] : 0.0;
     return VPMX_OK;
 }
 

@@ -149,9 +149,12 @@ def SFS_Cs_nobackscatter(Cs: float = 0.15) -> SubFilterScale:
 # ---- FMM settings (src/FLOWUnsteady_simulation.jl:43) -------------------------------
 @dataclass(frozen=True)
 class FMM:
+    """Expansion order p, leaf size ncrit, acceptance theta; sigma_cut <= 0
+    selects the kernel's default far-field separation in core radii."""
     p: int = 4
     ncrit: int = 50
     theta: float = 0.4
+    sigma_cut: float = 0.0
 
 
 # ---- contexts: one per device, created on first use -----------------------------------
@@ -206,15 +209,16 @@ class ParticleField:
     def _ptr(self):
         return self.particles.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
-    def _uj_params(self, *, reset=True, sfs=False, transposed=None) -> N.UJParams:
+    def _uj_params(self, *, reset=True, sfs=False, transposed=None, fmm=None) -> N.UJParams:
         P = N.UJParams()
         P.kernel = self.kernel.id
         P.precision = N.MIXED if self.precision == "mixed" else N.FP64
         P.transposed = int(self.transposed if transposed is None else transposed)
         P.reset = int(reset)
         P.sfs = int(bool(sfs))
-        P.fmm = 0
+        P.fmm = int(self.UJ is UJ_fmm_gpu) if fmm is None else int(fmm)
         P.p, P.ncrit, P.theta = self.fmm.p, self.fmm.ncrit, self.fmm.theta
+        P.sigma_cut = self.fmm.sigma_cut
         return P
 
     def _step_params(self) -> N.StepParams:
@@ -336,12 +340,45 @@ def UJ_gpu(pfield: ParticleField, *, reset: bool = True, reset_sfs: bool = False
     if reset_sfs and sfs:
         pfield.particles[SFS_INDEX, :pfield.np] = 0.0
     ctx = _context(pfield.device)
-    P = pfield._uj_params(reset=reset, sfs=sfs, transposed=transposed)
+    P = pfield._uj_params(reset=reset, sfs=sfs, transposed=transposed, fmm=False)
     ctx.check(N.lib().vpmx_uj(ctx.ptr, pfield._ptr(), pfield.np, ctypes.byref(pfield.layout),
                               ctypes.byref(P)))
 
 
 UJ_direct = UJ_gpu
+
+
+def UJ_fmm_gpu(pfield: ParticleField, *, reset: bool = True, **kwargs):
+    """Drop-in for vpm.UJ_fmm (the default vpm_UJ, src/FLOWUnsteady_simulation.jl:38,129):
+    U and J by the fast multipole method with pfield.fmm's p, ncrit, theta."""
+    ctx = _context(pfield.device)
+    P = pfield._uj_params(reset=reset, fmm=True)
+    ctx.check(N.lib().vpmx_uj(ctx.ptr, pfield._ptr(), pfield.np, ctypes.byref(pfield.layout),
+                              ctypes.byref(P)))
+
+
+UJ_fmm = UJ_fmm_gpu
+
+
+def Vvpm_on_Xs(pfield: ParticleField, Xs) -> np.ndarray:
+    """Velocity induced by the particle field at points Xs (m, 3), as
+    FLOWUnsteady's Vvpm_on_Xs (src/FLOWUnsteady_simulation.jl:494-570) returns
+    it, evaluated without appending probes to the field (the probes carry no
+    circulation, so only the m x N target-source pairs are computed)."""
+    Xs = np.ascontiguousarray(Xs, dtype=np.float64).reshape(-1, 3)
+    m = Xs.shape[0]
+    U = np.zeros((m, 3))
+    J = np.zeros((m, 9))
+    if m == 0:
+        return U
+    ctx = _context(pfield.device)
+    L = N.lib()
+    ctx.check(L.vpmx_upload(ctx.ptr, pfield._ptr(), pfield.np, ctypes.byref(pfield.layout)))
+    P = pfield._uj_params(fmm=False)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ctx.check(L.vpmx_uj_targets(ctx.ptr, Xs.ctypes.data_as(dp), m, U.ctypes.data_as(dp),
+                                J.ctypes.data_as(dp), ctypes.byref(P)))
+    return U
 
 
 def rungekutta3_gpu(pfield: ParticleField, dt: float, *, relax: bool = False, **kwargs):

@@ -96,9 +96,13 @@ typedef struct {
     int32_t transposed;   /* stretching scheme used by the SFS pass         */
     int32_t reset;        /* 1: overwrite U,J; 0: accumulate into them      */
     int32_t sfs;          /* VPMX_SFS_*: also evaluate Estr into SFS rows   */
-    int32_t fmm;          /* 0 direct (the only scheme in this version)     */
-    int32_t p, ncrit;     /* FMM expansion order and leaf size (reserved)   */
-    double theta;         /* FMM multipole acceptance criterion (reserved)  */
+    int32_t fmm;          /* 0 direct all-pairs, 1 fast multipole           */
+    int32_t p, ncrit;     /* FMM expansion order (0..12) and leaf size      */
+    double theta;         /* FMM multipole acceptance: (R_A+R_B) < theta d  */
+    double sigma_cut;     /* FMM: M2L also needs a gap of sigma_cut * max
+                             sigma of the source cell; <= 0 selects the
+                             kernel's default (gaussian-erf 7, gaussian 3,
+                             Winckelmans 20, singular 0)                   */
 } vpmx_uj_params;
 
 /* Settings of one time step (formulation, freestream, SFS, relaxation). */
@@ -158,6 +162,13 @@ int vpmx_rk3_resident(vpmx_ctx *ctx, const vpmx_uj_params *uj, const vpmx_step_p
                       double dt, int32_t relax);
 int vpmx_euler_resident(vpmx_ctx *ctx, const vpmx_uj_params *uj, const vpmx_step_params *step,
                         double dt, int32_t relax);
+/* U (m,3) and J (m,9) (row-major per target, J[k + 3l] = dU_k/dx_l) induced
+ * by the resident particles at m arbitrary points, without adding them to the
+ * field: the probe evaluation of Vvpm_on_Xs (src/FLOWUnsteady_simulation.jl:494-570),
+ * which otherwise appends zero-strength probes and evaluates all (N+m)^2
+ * pairs.  Direct scheme only; coincident points are skipped. */
+int vpmx_uj_targets(vpmx_ctx *ctx, const double *targets, int64_t m, double *U, double *J,
+                    const vpmx_uj_params *params);
 /* Enstrophy 0.5 sum_p Gamma_p . omega(x_p), omega = curl U from the resident J
  * (vpm.monitor_enstrophy, src/FLOWUnsteady_monitors.jl:573-574,614), and the
  * energy sum_p U_p . (x_p x Gamma_p). */
@@ -167,10 +178,22 @@ int64_t vpmx_resident_np(const vpmx_ctx *ctx);
 
 /* ---- instrumentation ----------------------------------------------------------- */
 /* Time every launch of the pairwise kernels with HIP events on the context's
- * stream.  which: 0 = UJ pair kernel, 1 = Estr pair kernel, 2 = fused update.
+ * stream.  which: 0 = UJ pair kernel, 1 = Estr pair kernel, 2 = fused update,
+ * 3 = FMM device evaluation (P2M through P2P).
  * Returns the summed milliseconds and the launch count since the last reset. */
 int vpmx_set_timing(vpmx_ctx *ctx, int32_t on);
 int vpmx_get_timing(vpmx_ctx *ctx, int32_t which, double *total_ms, int64_t *count);
+/* Statistics of the last FMM evaluation on ctx, in this order: host tree
+ * build ms, host traversal ms, list upload ms, cells, leaves, M2L pairs, P2P
+ * leaf pairs, P2P particle pairs, device evaluation ms (when timing is on).
+ * Writes min(n, 9) values. */
+int vpmx_fmm_stats(vpmx_ctx *ctx, double *out, int32_t n);
+/* Host-only: build the FMM tree and traversal lists for positions X (n,3)
+ * and sigma exactly as vpmx_uj does with fmm = 1, and report cells, leaves,
+ * M2L pairs, P2P leaf pairs and P2P particle pairs in out[0..4] (for checking
+ * the plan without a GPU). */
+int vpmx_fmm_plan_counts(int64_t n, const double *X, const double *sigma, int32_t ncrit,
+                         double theta, double sigma_cut, int64_t *out);
 /* The context's HIP stream (hipStream_t), for callers that order work on it. */
 void *vpmx_stream(vpmx_ctx *ctx);
 /* Make the next `count` entry-point calls fail with VPMX_EINJECT before any

@@ -216,3 +216,56 @@ def euler_step(params, field: Field, dt: float, relax: bool = False):
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+class FMMStats(ctypes.Structure):
+    _fields_ = [("t_tree", ctypes.c_double), ("t_traverse", ctypes.c_double),
+                ("t_p2m", ctypes.c_double), ("t_m2m", ctypes.c_double),
+                ("t_m2l", ctypes.c_double), ("t_l2l", ctypes.c_double),
+                ("t_l2p", ctypes.c_double), ("t_p2p", ctypes.c_double),
+                ("ncell", ctypes.c_int64), ("nleaf", ctypes.c_int64),
+                ("nm2l", ctypes.c_int64), ("np2p", ctypes.c_int64),
+                ("p2p_pairs", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def uj_fmm(kernel, X, G, sigma, p=4, ncrit=50, theta=0.4, sigcut=6.0, sample=None):
+    """U, J of all particles by the oracle's CPU FMM (p, ncrit, theta, and the
+    sigma-aware acceptance gap sigcut).  With `sample` (particle indices) only
+    the leaves holding them are evaluated (others stay zero)."""
+    L = lib()
+    if not hasattr(L, "_fmm_ready"):
+        L.orc_uj_fmm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp,
+                                 ctypes.POINTER(ctypes.c_int64), ctypes.c_int64,
+                                 ctypes.POINTER(FMMStats)]
+        L.orc_fmm_deriv.argtypes = [ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
+        L._fmm_ready = True
+    X = _c(X, (-1, 3)); G = _c(G, (-1, 3)); sigma = _c(sigma, (-1,))
+    n = X.shape[0]
+    U = np.zeros((n, 3)); J = np.zeros((n, 9))
+    st = FMMStats()
+    sp, ns = None, 0
+    if sample is not None:
+        sa = np.ascontiguousarray(sample, dtype=np.int64)
+        sp, ns = sa.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(sa)
+    rc = L.orc_uj_fmm(_kid(kernel), int(p), int(ncrit), float(theta), float(sigcut), n, _p(X),
+                      _p(G), _p(sigma), _p(U), _p(J), sp, ns, ctypes.byref(st))
+    if rc != 0:
+        raise ValueError("orc_uj_fmm: bad arguments")
+    return U, J, st.as_dict()
+
+
+def fmm_deriv(P, r):
+    """b_k(r) = d^k(1/|r|)/k! for |k| <= P (graded order) and the multi-indices."""
+    L = lib()
+    if not hasattr(L, "_fmm_ready"):   # sets the FMM argtypes
+        uj_fmm(0, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0))
+    K = (P + 1) * (P + 2) * (P + 3) // 6
+    b = np.zeros(K)
+    mi = np.zeros((K, 3), dtype=np.int32)
+    r = _c(r, (3,))
+    L.orc_fmm_deriv(int(P), _p(r), _p(b), mi.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return b, mi

@@ -113,7 +113,7 @@ double orc_zeta0(int kernel)
  *   J += (g'/(sigma r) - 3 g / r^2) * crss_k * dX_l   (radial part)
  *      + (-g/(4 pi r^3)) * eps_{klm} Gamma_m          (Kronecker part)
  * with crss = -(1/(4 pi r^3)) dX x Gamma. */
-static inline void pair_uj(int kernel, const double *xt, const double *xs, const double *gs,
+void orc_pair_uj(int kernel, const double *xt, const double *xs, const double *gs,
                            double sgm, double *U, double *J)
 {
     double dX1 = xt[0] - xs[0], dX2 = xt[1] - xs[1], dX3 = xt[2] - xs[2];
@@ -155,7 +155,7 @@ void orc_uj_direct(int kernel, int64_t nt, const double *tX, int64_t ns, const d
     for (int64_t i = 0; i < nt; i++) {
         double u[3] = {0, 0, 0}, jj[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int64_t j = 0; j < ns; j++)
-            pair_uj(kernel, tX + 3 * i, sX + 3 * j, sG + 3 * j, ssig[j], u, jj);
+            orc_pair_uj(kernel, tX + 3 * i, sX + 3 * j, sG + 3 * j, ssig[j], u, jj);
         for (int k = 0; k < 3; k++) U[3 * i + k] += u[k];
         for (int k = 0; k < 9; k++) J[9 * i + k] += jj[k];
     }

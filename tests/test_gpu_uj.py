@@ -210,9 +210,13 @@ def test_error_paths():
     code, _ = _ctx_call(L.vpmx_uj, pf._ptr(), pf.np, ctypes.byref(pf.layout), ctypes.byref(P2))
     assert code == N.EINVAL
     P2 = pf._uj_params()
-    P2.fmm = 1
+    P2.fmm = 2
     code, _ = _ctx_call(L.vpmx_uj, pf._ptr(), pf.np, ctypes.byref(pf.layout), ctypes.byref(P2))
     assert code == N.EUNSUPPORTED
+    P2 = pf._uj_params(fmm=True)
+    P2.p = 13
+    code, msg = _ctx_call(L.vpmx_uj, pf._ptr(), pf.np, ctypes.byref(pf.layout), ctypes.byref(P2))
+    assert code == N.EINVAL and "order" in msg
     # injected fault, then recovery
     ctx = vpm._context(0)
     assert L.vpmx_inject_fault(ctx.ptr, 1) == 0
@@ -236,3 +240,33 @@ def test_timing_counts_launches():
     ms, n = ctx.timing(N.T_UJ)
     ctx.set_timing(False)
     assert n == 2 and ms > 0.0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_targets_probe_evaluation(precision, kernel):
+    """vpmx_uj_targets (Vvpm_on_Xs without appending probes) vs the oracle
+    with sources = the field, including a probe that coincides with a particle."""
+    X, G, s, _ = workloads.rotor_wake(n=1 << 14, seed=3)
+    pf = make_field(X, G, s, kernel=kernel, precision=precision)
+    rng = np.random.default_rng(4)
+    P = np.concatenate([rng.uniform(-1.2, 1.2, (700, 3)) * [1, 1, 5], X[:5]])
+    U = vpm.Vvpm_on_Xs(pf, P)
+    Ur, _ = O.uj_direct(kernel, P, X, G, s)
+    tu, _ = TOL[precision]
+    assert normwise(U, Ur)[0] < tu
+    assert vpm.Vvpm_on_Xs(pf, np.zeros((0, 3))).shape == (0, 3)
+
+
+def test_targets_match_appended_probes():
+    """Same probe velocities as the reference's way: append zero-strength,
+    sigma = 1e-6 probes and run UJ over everything (simulation.jl:539-547)."""
+    X, G, s, _ = workloads.vortex_ring(n_stations=20, n_cross=40)
+    P = np.random.default_rng(2).uniform(-1.5, 1.5, (64, 3))
+    pf = make_field(X, G, s, extra=len(P))
+    Ut = vpm.Vvpm_on_Xs(pf, P)
+    for p in P:
+        vpm.add_particle(pf, p, np.zeros(3), 1e-6, vol=0.0)
+    vpm.UJ_gpu(pf)
+    Up = vpm.get_U(pf)[len(X):]
+    assert normwise(Ut, Up)[0] < 1e-13
