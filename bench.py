@@ -102,6 +102,93 @@ class Field:
                                                    dt, 0))
 
 
+def uj_at(ctx, X_targets, kernel, precision):
+    """U, J at target points from the resident field (vpmx_uj_targets)."""
+    from flowunsteady_amd import _native as N
+    T = np.ascontiguousarray(X_targets, dtype=np.float64)
+    m = len(T)
+    U = np.zeros((m, 3))
+    J = np.zeros((m, 9))
+    P = N.UJParams()
+    P.kernel, P.precision, P.transposed, P.reset = kernel, precision, 1, 1
+    dp = ctypes.POINTER(ctypes.c_double)
+    ctx.check(N.lib().vpmx_uj_targets(ctx.ptr, T.ctypes.data_as(dp), m, U.ctypes.data_as(dp),
+                                      J.ctypes.data_as(dp), ctypes.byref(P)))
+    return U, J
+
+
+def fmm_section(ctx, n, orders, log, cpu=True, nref=1024):
+    """UJ by the FMM on the rotor-wake workload (3 helical tip filaments +
+    root line, Gamma = 0.05 t + N(0, 5e-4), sigma = 0.03), ncrit = 50,
+    theta = 0.4, mixed near field, FP64 expansions; errors against the mixed
+    direct sum at `nref` sampled targets."""
+    from flowunsteady_amd import _native as N
+    from flowunsteady_amd import workloads
+    X, G, s, v = workloads.rotor_wake(n=n, seed=3)
+    fld = Field(ctx, X, G, s, v)
+    fld.upload()
+    idx = np.sort(np.random.default_rng(7).choice(n, nref, replace=False))
+    t0 = time.perf_counter()
+    Uref, Jref = uj_at(ctx, X[idx], N.GAUSSIANERF, N.MIXED)
+    log(f"fmm reference: {nref} targets direct mixed in {time.perf_counter() - t0:.2f} s")
+    res = {"workload": f"rotor wake N={n} (3 helices radius 1 pitch 0.3 40 turns + root line), "
+                       "gaussian-erf, ncrit=50, theta=0.4, mixed P2P, FP64 expansions",
+           "N": n, "error_reference": f"direct mixed UJ at {nref} sampled targets", "orders": {}}
+    P = N.UJParams()
+    P.kernel, P.precision, P.transposed, P.reset = N.GAUSSIANERF, N.MIXED, 1, 1
+    P.fmm, P.ncrit, P.theta, P.sigma_cut = 1, 50, 0.4, 0.0
+    for p in orders:
+        P.p = p
+        ctx.check(N.lib().vpmx_uj_resident(ctx.ptr, ctypes.byref(P)))   # warm (tables)
+        ctx.set_timing(True)
+        reps = 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.check(N.lib().vpmx_uj_resident(ctx.ptr, ctypes.byref(P)))
+        wall = (time.perf_counter() - t0) / reps
+        st = ctx.fmm_stats()
+        ctx.set_timing(False)
+        U, J = fld.download()
+        eu, ej = normwise(U[idx], Uref), normwise(J[idx], Jref)
+        dev_s = st["ms_eval"] / 1e3
+        res["orders"][str(p)] = {
+            "ms_per_uj": 1e3 * wall,
+            "direct_equivalent_interactions_per_s": float(n) * n / wall,
+            "device_ms": st["ms_eval"], "host_tree_ms": st["ms_tree"],
+            "host_traverse_ms": st["ms_traverse"], "list_upload_ms": st["ms_upload"],
+            "cells": int(st["cells"]), "leaves": int(st["leaves"]),
+            "m2l_pairs": int(st["m2l_pairs"]), "p2p_pairs": int(st["p2p_pairs"]),
+            "p2p_pairs_per_device_s": st["p2p_pairs"] / dev_s if dev_s > 0 else None,
+            "rel_err_U": {"max": eu[0], "rms": eu[1]}, "rel_err_J": {"max": ej[0], "rms": ej[1]}}
+        log(f"fmm p={p}: {1e3 * wall:.1f} ms/UJ (device {st['ms_eval']:.1f} ms, tree "
+            f"{st['ms_tree']:.1f}, traverse {st['ms_traverse']:.1f}), err U {eu}, J {ej}")
+    if cpu:
+        from oracle import oracle as O
+        p = 8 if 8 in orders else orders[-1]
+        sample = np.random.default_rng(11).choice(n, 256, replace=False)
+        t0 = time.perf_counter()
+        _, _, st = O.uj_fmm("gaussianerf", X, G, s, p=p, ncrit=50, theta=0.4, sigcut=7.0,
+                            sample=sample)
+        wall = time.perf_counter() - t0
+        setup = st["t_tree"] + st["t_traverse"] + st["t_p2m"] + st["t_m2m"]
+        full = (setup + st["t_m2l"] * st["nm2l"] / max(st["nm2l_active"], 1)
+                + st["t_l2l"] * st["ncell"] / max(st["ncell_active"], 1)
+                + st["t_l2p"] * st["nleaf"] / max(st["nactive_leaf"], 1)
+                + st["t_p2p"] * st["p2p_pairs_all"] / max(st["p2p_pairs"], 1))
+        gpu = res["orders"][str(p)]["ms_per_uj"] / 1e3
+        res["cpu_baseline"] = {
+            "value": float(n) * n / full, "unit": "direct-equivalent interactions/s",
+            "seconds_per_uj": full, "cores": O.num_threads(), "kind": "port", "p": p,
+            "sample": f"oracle CPU FMM (same tree/lists, FP64 OpenMP): full tree build, "
+                      f"traversal, P2M, M2M; M2L/L2L/L2P/P2P timed on the {st['nactive_leaf']} "
+                      f"leaves holding 256 sampled targets (and their ancestors) and each "
+                      f"scaled by its work count (M2L pairs, cells, leaves, P2P pairs) to the "
+                      f"full {st['nleaf']}-leaf tree; {wall:.1f} s of CPU work",
+            "gpu_speedup": full / gpu}
+        log("cpu fmm baseline:", res["cpu_baseline"])
+    return res
+
+
 def cpu_baseline(X, G, s, budget_s=12.0):
     """Oracle UJ_direct (OpenMP, FP64) on a bounded target sample against all N
     sources; interactions/s = sample * N / t."""
@@ -131,6 +218,9 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--no-fmm", action="store_true", help="skip the FMM section")
+    ap.add_argument("--fmm-n", type=int, default=1 << 22)
+    ap.add_argument("--fmm-orders", type=str, default="4,8,10")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
@@ -233,6 +323,12 @@ def main():
                                   "J_max": normwise(Jmx[idx], Jr)[0], "targets": 256}
         log("oracle checks:", out["fp64_vs_oracle"], out["mixed_vs_oracle"])
         out["cpu_baseline"] = cpu_baseline(X, G, s)
+    if not args.no_fmm:
+        try:
+            orders = [int(v) for v in args.fmm_orders.split(",") if v]
+            out["fmm"] = fmm_section(ctx, args.fmm_n, orders, log, cpu=not args.no_cpu)
+        except Exception as exc:   # the direct-UJ line is still reported
+            out["fmm"] = {"error": repr(exc)}
     print(json.dumps(out), flush=True)
 
 

@@ -360,7 +360,9 @@ static int64_t *csr_offsets(const pairlist *L, int64_t ncell)
 /* ---- FMM evaluation --------------------------------------------------------------------- */
 typedef struct {
     double t_tree, t_traverse, t_p2m, t_m2m, t_m2l, t_l2l, t_l2p, t_p2p;
-    int64_t ncell, nleaf, nm2l, np2p, p2p_pairs;
+    int64_t ncell, nleaf, nm2l, np2p, p2p_pairs, nactive_leaf;
+    /* work of the full evaluation vs the evaluated (active) part */
+    int64_t p2p_pairs_all, nm2l_active, ncell_active;
 } orc_fmm_stats;
 
 static double wtime(void)
@@ -635,7 +637,18 @@ int orc_uj_fmm(int kernel, int p, int ncrit, double theta, double sigcut, int64_
     st.t_p2p = t8 - t7;
 
     st.ncell = nc;
-    for (int64_t c = 0; c < nc; c++) st.nleaf += T.cells[c].nchild == 0;
+    for (int64_t c = 0; c < nc; c++) {
+        st.nleaf += T.cells[c].nchild == 0;
+        st.nactive_leaf += T.cells[c].nchild == 0 && active[c];
+        st.ncell_active += active[c] != 0;
+        if (active[c]) st.nm2l_active += m2l_off[c + 1] - m2l_off[c];
+        if (T.cells[c].nchild == 0) {
+            int64_t s = 0;
+            for (int64_t e = p2p_off[c]; e < p2p_off[c + 1]; e++)
+                s += T.cells[p2p.a[2 * e + 1]].end - T.cells[p2p.a[2 * e + 1]].begin;
+            st.p2p_pairs_all += s * (T.cells[c].end - T.cells[c].begin);
+        }
+    }
     st.nm2l = m2l.n;
     st.np2p = p2p.n;
     st.p2p_pairs = pairs;

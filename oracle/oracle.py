@@ -225,7 +225,9 @@ class FMMStats(ctypes.Structure):
                 ("t_l2p", ctypes.c_double), ("t_p2p", ctypes.c_double),
                 ("ncell", ctypes.c_int64), ("nleaf", ctypes.c_int64),
                 ("nm2l", ctypes.c_int64), ("np2p", ctypes.c_int64),
-                ("p2p_pairs", ctypes.c_int64)]
+                ("p2p_pairs", ctypes.c_int64), ("nactive_leaf", ctypes.c_int64),
+                ("p2p_pairs_all", ctypes.c_int64), ("nm2l_active", ctypes.c_int64),
+                ("ncell_active", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
