@@ -15,7 +15,26 @@ constexpr int FMM_MAX_P = 12;   // K = 455 <= 2 coefficients per thread of 256
 
 struct FmmCellDev {
     int32_t begin, end, child0, nchild;
-    double cx, cy, cz;
+    double cx, cy, cz, R, smax;
+};
+
+// Scratch of the device dual-tree traversal.
+struct DttBufs {
+    int64_t cap_front = 0, cap_next = 0, cap_m2l = 0, cap_p2p = 0, cap_count = 0, cap_tmpk = 0;
+    int2 *front = nullptr, *next = nullptr;
+    int *cnt_next = nullptr, *cnt_m2l = nullptr, *cnt_p2p = nullptr;
+    int64_t *off_next = nullptr, *off_m2l = nullptr, *off_p2p = nullptr;
+    uint64_t *m2l_keys = nullptr, *p2p_keys = nullptr, *keys_tmp = nullptr;
+    int64_t *totals = nullptr;               // 3 counts + pair total
+    int64_t *h_totals = nullptr;             // pinned
+    uint8_t *tmp = nullptr;
+    size_t tmp_bytes = 0;
+};
+
+// Result sizes of a device traversal.
+struct DttResult {
+    int64_t nm2l = 0, np2p = 0, p2p_pairs = 0;
+    int steps = 0;
 };
 
 struct FmmCellHost {
@@ -90,7 +109,13 @@ struct FmmState {
     int64_t h_cap = 0;
     // statistics of the last evaluation
     double stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    DttBufs dtt;
 };
+
+// Dual tree traversal on the device: fills m2l_off/m2l_src and p2p_off/p2p_src
+// (CSR by target cell, sources ascending), growing the buffers in S.
+hipError_t fmm_traverse_device(hipStream_t st, FmmState &S, int64_t ncell, double theta,
+                               double sigcut, DttResult &res);
 
 void fmm_build_tree(FmmPlan &pl, const uint64_t *key, const double *ssig, int64_t n, int ncrit);
 void fmm_traverse(FmmPlan &pl, double theta, double sigcut);

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "device_math.h"
 #include "fmm.h"
@@ -796,6 +797,283 @@ void fmm_build_tables(FmmTables &T, int P)
         }
     }
 }
+
+// ---------------------------------------------------------------------------
+// Device dual tree traversal
+// ---------------------------------------------------------------------------
+// Level-synchronous: every frontier pair (target a, source b) is classified by
+// the same rules as fmm_traverse (M2L, P2P, or a split into child pairs);
+// exclusive scans of the per-pair output counts give each pair fixed output
+// slots, so the lists are deterministic without atomics.  The M2L and P2P
+// lists are then radix-sorted by (target, source), which is exactly the
+// host traversal's order.
+
+// Multipole acceptance with IEEE-rounded operations in the host's order (no
+// FMA contraction), so device and host decisions agree bit for bit.
+__device__ __forceinline__ bool mac_dev(const FmmCellDev &A, const FmmCellDev &B, double theta,
+                                        double sigcut)
+{
+#pragma clang fp contract(off)
+    const double dx = A.cx - B.cx, dy = A.cy - B.cy, dz = A.cz - B.cz;
+    const double d = __dsqrt_rn(dx * dx + dy * dy + dz * dz);
+    return (A.R + B.R) < theta * d && (d - A.R - B.R) >= sigcut * B.smax;
+}
+
+// kind: 0 M2L, 1 P2P, 2 split both (a == b), 3 split target, 4 split source
+__device__ __forceinline__ int dtt_action(const int2 pr, const FmmCellDev *__restrict__ cells,
+                                          double theta, double sigcut, int &nsplit)
+{
+    const FmmCellDev A = cells[pr.x], B = cells[pr.y];
+    nsplit = 0;
+    if (pr.x == pr.y) {
+        if (A.nchild == 0) return 1;
+        nsplit = A.nchild * A.nchild;
+        return 2;
+    }
+    if (mac_dev(A, B, theta, sigcut)) return 0;
+    if (A.nchild == 0 && B.nchild == 0) return 1;
+    if (B.nchild == 0 || (A.nchild != 0 && A.R >= B.R)) {
+        nsplit = A.nchild;
+        return 3;
+    }
+    nsplit = B.nchild;
+    return 4;
+}
+
+__global__ __launch_bounds__(FMM_BLOCK) void dtt_classify_kernel(const int2 *__restrict__ fr,
+                                                                 int64_t nf,
+                                                                 const FmmCellDev *__restrict__ cells,
+                                                                 double theta, double sigcut,
+                                                                 int *__restrict__ cnext,
+                                                                 int *__restrict__ cm2l,
+                                                                 int *__restrict__ cp2p)
+{
+    const int64_t i = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (i >= nf) return;
+    int ns;
+    const int kind = dtt_action(fr[i], cells, theta, sigcut, ns);
+    cnext[i] = ns;
+    cm2l[i] = kind == 0;
+    cp2p[i] = kind == 1;
+}
+
+__global__ __launch_bounds__(FMM_BLOCK) void dtt_emit_kernel(
+    const int2 *__restrict__ fr, int64_t nf, const FmmCellDev *__restrict__ cells, double theta,
+    double sigcut, const int64_t *__restrict__ onext, const int64_t *__restrict__ om2l,
+    const int64_t *__restrict__ op2p, int2 *__restrict__ next, uint64_t *__restrict__ m2l,
+    uint64_t *__restrict__ p2p, int64_t m2l_base, int64_t p2p_base, int nb)
+{
+    const int64_t i = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (i >= nf) return;
+    const int2 pr = fr[i];
+    int ns;
+    const int kind = dtt_action(pr, cells, theta, sigcut, ns);
+    const uint64_t key = ((uint64_t)pr.x << nb) | (uint64_t)pr.y;
+    if (kind == 0) {
+        m2l[m2l_base + om2l[i]] = key;
+    } else if (kind == 1) {
+        p2p[p2p_base + op2p[i]] = key;
+    } else {
+        const FmmCellDev A = cells[pr.x], B = cells[pr.y];
+        int64_t o = onext[i];
+        if (kind == 2) {
+            for (int a = 0; a < A.nchild; a++)
+                for (int b = 0; b < A.nchild; b++) next[o++] = make_int2(A.child0 + a, A.child0 + b);
+        } else if (kind == 3) {
+            for (int a = 0; a < A.nchild; a++) next[o++] = make_int2(A.child0 + a, pr.y);
+        } else {
+            for (int b = 0; b < B.nchild; b++) next[o++] = make_int2(pr.x, B.child0 + b);
+        }
+    }
+}
+
+// totals[k] = off_k[n-1] + cnt_k[n-1] for the three output kinds
+__global__ void dtt_totals_kernel(int64_t n, const int *cnext, const int *cm2l, const int *cp2p,
+                                  const int64_t *onext, const int64_t *om2l, const int64_t *op2p,
+                                  int64_t *totals)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    totals[0] = onext[n - 1] + cnext[n - 1];
+    totals[1] = om2l[n - 1] + cm2l[n - 1];
+    totals[2] = op2p[n - 1] + cp2p[n - 1];
+}
+
+// CSR offsets from keys sorted by target: off[c] = first key with target >= c.
+__global__ __launch_bounds__(FMM_BLOCK) void csr_kernel(const uint64_t *__restrict__ keys,
+                                                        int64_t nkeys, int64_t ncell, int nb,
+                                                        int64_t *__restrict__ off,
+                                                        int32_t *__restrict__ src)
+{
+    const int64_t t = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (t <= ncell) {
+        const uint64_t v = (uint64_t)t << nb;
+        int64_t lo = 0, hi = nkeys;
+        while (lo < hi) {
+            const int64_t m = lo + (hi - lo) / 2;
+            if (keys[m] < v) lo = m + 1; else hi = m;
+        }
+        off[t] = lo;
+    }
+    const uint64_t mask = ((uint64_t)1 << nb) - 1;
+    for (int64_t i = t; i < nkeys; i += (int64_t)gridDim.x * FMM_BLOCK) src[i] = (int32_t)(keys[i] & mask);
+}
+
+// Particle pairs of the P2P lists (integer atomics: order-independent).
+__global__ __launch_bounds__(FMM_BLOCK) void p2p_count_kernel(const int64_t *__restrict__ off,
+                                                              const int32_t *__restrict__ src,
+                                                              const FmmCellDev *__restrict__ cells,
+                                                              int64_t ncell,
+                                                              unsigned long long *__restrict__ total)
+{
+    const int64_t c = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (c >= ncell) return;
+    const FmmCellDev A = cells[c];
+    unsigned long long s = 0;
+    for (int64_t e = off[c]; e < off[c + 1]; e++) {
+        const FmmCellDev B = cells[src[e]];
+        s += (unsigned long long)(B.end - B.begin);
+    }
+    s *= (unsigned long long)(A.end - A.begin);
+    if (s) atomicAdd(total, s);
+}
+
+template <class T>
+static hipError_t grow(T **p, int64_t &cap, int64_t need, hipStream_t st, int64_t keep = 0)
+{
+    if (need <= cap && *p) return hipSuccess;
+    const int64_t nc = std::max<int64_t>({need, cap + cap / 2, 1024});
+    T *q = nullptr;
+    hipError_t e = hipMalloc((void **)&q, nc * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (*p && keep > 0) {
+        e = hipMemcpyAsync(q, *p, keep * sizeof(T), hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+    }
+    if (*p) (void)hipFree(*p);
+    *p = q;
+    cap = nc;
+    return hipSuccess;
+}
+
+#define VPMX_CHK(x)                              \
+    do {                                         \
+        hipError_t e_ = (x);                     \
+        if (e_ != hipSuccess) return e_;         \
+    } while (0)
+
+hipError_t fmm_traverse_device(hipStream_t st, FmmState &S, int64_t ncell, double theta,
+                               double sigcut, DttResult &res)
+{
+    DttBufs &B = S.dtt;
+    res = DttResult();
+    int nb = 1;
+    while (((int64_t)1 << nb) <= ncell) nb++;
+    if (!B.totals) {
+        VPMX_CHK(hipMalloc((void **)&B.totals, 4 * sizeof(int64_t)));
+        VPMX_CHK(hipHostMalloc((void **)&B.h_totals, 4 * sizeof(int64_t), hipHostMallocDefault));
+    }
+    VPMX_CHK(grow(&B.front, B.cap_front, 1, st));
+    VPMX_CHK(grow(&B.next, B.cap_next, 1, st));
+    const int2 root = make_int2(0, 0);
+    VPMX_CHK(hipMemcpyAsync(B.front, &root, sizeof(int2), hipMemcpyHostToDevice, st));
+    int64_t nf = 1, nm2l = 0, np2p = 0;
+    while (nf > 0) {
+        res.steps++;
+        if (nf > B.cap_count) {
+            int64_t c1 = B.cap_count, c2 = B.cap_count, c3 = B.cap_count, c4 = B.cap_count,
+                    c5 = B.cap_count, c6 = B.cap_count;
+            VPMX_CHK(grow(&B.cnt_next, c1, nf, st));
+            VPMX_CHK(grow(&B.cnt_m2l, c2, nf, st));
+            VPMX_CHK(grow(&B.cnt_p2p, c3, nf, st));
+            VPMX_CHK(grow(&B.off_next, c4, nf, st));
+            VPMX_CHK(grow(&B.off_m2l, c5, nf, st));
+            VPMX_CHK(grow(&B.off_p2p, c6, nf, st));
+            B.cap_count = c1;
+            size_t need = 0;
+            VPMX_CHK(rocprim::exclusive_scan(nullptr, need, B.cnt_next, B.off_next, (int64_t)0,
+                                             (size_t)B.cap_count, rocprim::plus<int64_t>(), st));
+            if (need > B.tmp_bytes) {
+                if (B.tmp) (void)hipFree(B.tmp);
+                B.tmp = nullptr;
+                VPMX_CHK(hipMalloc((void **)&B.tmp, need));
+                B.tmp_bytes = need;
+            }
+        }
+        dtt_classify_kernel<<<gridn(nf, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
+            B.front, nf, S.cells, theta, sigcut, B.cnt_next, B.cnt_m2l, B.cnt_p2p);
+        VPMX_CHK(hipGetLastError());
+        size_t tb = B.tmp_bytes;
+        VPMX_CHK(rocprim::exclusive_scan(B.tmp, tb, B.cnt_next, B.off_next, (int64_t)0, (size_t)nf,
+                                         rocprim::plus<int64_t>(), st));
+        tb = B.tmp_bytes;
+        VPMX_CHK(rocprim::exclusive_scan(B.tmp, tb, B.cnt_m2l, B.off_m2l, (int64_t)0, (size_t)nf,
+                                         rocprim::plus<int64_t>(), st));
+        tb = B.tmp_bytes;
+        VPMX_CHK(rocprim::exclusive_scan(B.tmp, tb, B.cnt_p2p, B.off_p2p, (int64_t)0, (size_t)nf,
+                                         rocprim::plus<int64_t>(), st));
+        dtt_totals_kernel<<<1, 64, 0, st>>>(nf, B.cnt_next, B.cnt_m2l, B.cnt_p2p, B.off_next,
+                                            B.off_m2l, B.off_p2p, B.totals);
+        VPMX_CHK(hipGetLastError());
+        VPMX_CHK(hipMemcpyAsync(B.h_totals, B.totals, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        VPMX_CHK(hipStreamSynchronize(st));
+        const int64_t nn = B.h_totals[0], am = B.h_totals[1], ap = B.h_totals[2];
+        VPMX_CHK(grow(&B.next, B.cap_next, nn, st));
+        VPMX_CHK(grow(&B.m2l_keys, B.cap_m2l, nm2l + am, st, nm2l));
+        VPMX_CHK(grow(&B.p2p_keys, B.cap_p2p, np2p + ap, st, np2p));
+        dtt_emit_kernel<<<gridn(nf, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
+            B.front, nf, S.cells, theta, sigcut, B.off_next, B.off_m2l, B.off_p2p, B.next,
+            B.m2l_keys, B.p2p_keys, nm2l, np2p, nb);
+        VPMX_CHK(hipGetLastError());
+        nm2l += am;
+        np2p += ap;
+        std::swap(B.front, B.next);
+        std::swap(B.cap_front, B.cap_next);
+        nf = nn;
+    }
+    // sort both lists by (target, source) and build CSR
+    const int64_t kmax = std::max(nm2l, np2p);
+    VPMX_CHK(grow(&B.keys_tmp, B.cap_tmpk, std::max<int64_t>(kmax, 1), st));
+    size_t need = 0;
+    VPMX_CHK(rocprim::radix_sort_keys(nullptr, need, B.m2l_keys, B.keys_tmp, (unsigned int)kmax, 0,
+                                      2 * nb, st));
+    if (need > B.tmp_bytes) {
+        if (B.tmp) (void)hipFree(B.tmp);
+        B.tmp = nullptr;
+        VPMX_CHK(hipMalloc((void **)&B.tmp, need));
+        B.tmp_bytes = need;
+    }
+    // lists into FmmState (off: ncell + 1, src: count)
+    if (ncell + 1 > S.cap_cells + 1 || !S.m2l_off) return hipErrorInvalidValue;
+    VPMX_CHK(grow(&S.m2l_src, S.cap_m2l, std::max<int64_t>(nm2l, 1), st));
+    VPMX_CHK(grow(&S.p2p_src, S.cap_p2p, std::max<int64_t>(np2p, 1), st));
+    struct L { uint64_t *keys; int64_t n; int64_t *off; int32_t *src; } lists[2] = {
+        {B.m2l_keys, nm2l, S.m2l_off, S.m2l_src}, {B.p2p_keys, np2p, S.p2p_off, S.p2p_src}};
+    for (auto &li : lists) {
+        if (li.n > 0) {
+            size_t tb = B.tmp_bytes;
+            VPMX_CHK(rocprim::radix_sort_keys(B.tmp, tb, li.keys, B.keys_tmp, (unsigned int)li.n, 0,
+                                              2 * nb, st));
+        }
+        const int64_t work = std::max(li.n, ncell + 1);
+        csr_kernel<<<gridn(work, FMM_BLOCK), FMM_BLOCK, 0, st>>>(B.keys_tmp, li.n, ncell, nb, li.off,
+                                                                li.src);
+        VPMX_CHK(hipGetLastError());
+    }
+    VPMX_CHK(hipMemsetAsync(B.totals + 3, 0, sizeof(int64_t), st));
+    p2p_count_kernel<<<gridn(ncell, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
+        S.p2p_off, S.p2p_src, S.cells, ncell, (unsigned long long *)(B.totals + 3));
+    VPMX_CHK(hipGetLastError());
+    VPMX_CHK(hipMemcpyAsync(B.h_totals + 3, B.totals + 3, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    VPMX_CHK(hipStreamSynchronize(st));
+    res.nm2l = nm2l;
+    res.np2p = np2p;
+    res.p2p_pairs = B.h_totals[3];
+    return hipSuccess;
+}
+
+#undef VPMX_CHK
 
 // Host version of the key computation and the whole plan, for checking the
 // tree and traversal against an independent CPU implementation.

@@ -449,6 +449,12 @@ void free_fmm(FmmState *S)
                     S->ssig, S->sort_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    const DttBufs &B = S->dtt;
+    void *dptrs[] = {B.front, B.next, B.cnt_next, B.cnt_m2l, B.cnt_p2p, B.off_next, B.off_m2l,
+                     B.off_p2p, B.m2l_keys, B.p2p_keys, B.keys_tmp, B.totals, B.tmp};
+    for (void *p : dptrs)
+        if (p) (void)hipFree(p);
+    if (B.h_totals) (void)hipHostFree(B.h_totals);
     if (S->h_keys) (void)hipHostFree(S->h_keys);
     if (S->h_ssig) (void)hipHostFree(S->h_ssig);
     delete S;
@@ -522,8 +528,6 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
     pl.ext = ext;
     fmm_build_tree(pl, S.h_keys, S.h_ssig, n, P->ncrit);
     const double t1 = now_ms();
-    fmm_traverse(pl, P->theta, sigcut);
-    const double t2 = now_ms();
     const int64_t nc = (int64_t)pl.cells.size();
     int maxlev = 0;
     for (const auto &C : pl.cells) maxlev = std::max(maxlev, C.level);
@@ -537,7 +541,7 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
     for (int64_t i = 0; i < nc; i++) {
         const FmmCellHost &C = pl.cells[i];
         hc[i] = FmmCellDev{(int32_t)C.begin, (int32_t)C.end, (int32_t)C.child0, C.nchild,
-                           C.c[0], C.c[1], C.c[2]};
+                           C.c[0], C.c[1], C.c[2], C.R, C.smax};
         if (C.nchild == 0) pl.leaves.push_back((int32_t)i);
         for (int k = 0; k < C.nchild; k++) pl.parent[C.child0 + k] = (int32_t)i;
     }
@@ -552,17 +556,6 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
         pl.level_parents[lev].second = (int)pl.parents.size() - pl.level_parents[lev].first;
         pl.level_kids[lev].second = (int)pl.kids.size() - pl.level_kids[lev].first;
     }
-    int64_t pairs = 0;
-    for (int32_t a : pl.leaves) {
-        int64_t s = 0;
-        for (int64_t e = pl.p2p_off[a]; e < pl.p2p_off[a + 1]; e++) {
-            const FmmCellHost &B = pl.cells[pl.p2p_src[e]];
-            s += B.end - B.begin;
-        }
-        pairs += s * (pl.cells[a].end - pl.cells[a].begin);
-    }
-    pl.p2p_pairs = pairs;
-
     // device: tables (once per p), tree, lists, expansions
     if (S.dev_tab_p != P->p) {
         fmm_build_tables(S.tab, P->p);
@@ -586,27 +579,17 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
         VPMX_TRY(dalloc(c, &S.L, cap));
         S.cap_coef = cap;
     }
-    const int64_t nm2l = (int64_t)pl.m2l_src.size(), np2p = (int64_t)pl.p2p_src.size();
-    if (nm2l > S.cap_m2l || !S.m2l_src) {
-        const int64_t cap = std::max<int64_t>({nm2l, S.cap_m2l + S.cap_m2l / 2, 1});
-        VPMX_TRY(dalloc(c, &S.m2l_src, cap));
-        S.cap_m2l = cap;
-    }
-    if (np2p > S.cap_p2p || !S.p2p_src) {
-        const int64_t cap = std::max<int64_t>({np2p, S.cap_p2p + S.cap_p2p / 2, 1});
-        VPMX_TRY(dalloc(c, &S.p2p_src, cap));
-        S.cap_p2p = cap;
-    }
     VPMX_TRY(upload_vec(c, S.cells, hc));
     VPMX_TRY(upload_vec(c, S.leaves, pl.leaves));
     VPMX_TRY(upload_vec(c, S.parents, pl.parents));
     VPMX_TRY(upload_vec(c, S.kids, pl.kids));
     VPMX_TRY(upload_vec(c, S.parent, pl.parent));
-    VPMX_TRY(upload_vec(c, S.m2l_off, pl.m2l_off));
-    VPMX_TRY(upload_vec(c, S.p2p_off, pl.p2p_off));
-    VPMX_TRY(upload_vec(c, S.m2l_src, pl.m2l_src));
-    VPMX_TRY(upload_vec(c, S.p2p_src, pl.p2p_src));
     VPMX_TRY(sync(c));
+    const double t2 = now_ms();
+    // device: dual tree traversal -> M2L and P2P lists (CSR by target)
+    DttResult dr;
+    VPMX_HIP(c, fmm_traverse_device(st, S, nc, P->theta, sigcut, dr));
+    const int64_t nm2l = dr.nm2l, np2p = dr.np2p, pairs = dr.p2p_pairs;
     const double t3 = now_ms();
 
     FmmDev d;
@@ -648,9 +631,9 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
         VPMX_HIP(c, launch_fmm_eval(st, d, pl, P->kernel, P->precision, f, reset));
     }
     VPMX_TRY(sync(c));
-    S.stats[0] = t1 - t0;
-    S.stats[1] = t2 - t1;
-    S.stats[2] = t3 - t2;
+    S.stats[0] = t1 - t0;   // host tree build
+    S.stats[1] = t3 - t2;   // device traversal (incl. list sort)
+    S.stats[2] = t2 - t1;   // tree arrays to the device
     S.stats[3] = (double)nc;
     S.stats[4] = (double)pl.leaves.size();
     S.stats[5] = (double)nm2l;
