@@ -21,7 +21,7 @@ FP64, MIXED = 0, 1
 # relaxation
 RELAX_NONE, RELAX_PEDRIZZETTI, RELAX_CORRECTED = 0, 1, 2
 # SFS
-SFS_NONE, SFS_CONSTANT = 0, 1
+SFS_NONE, SFS_CONSTANT, SFS_DYNAMIC = 0, 1, 2
 # timing classes
 T_UJ, T_ESTR, T_UPDATE, T_FMM = 0, 1, 2, 3
 FMM_STATS = ("ms_tree", "ms_traverse", "ms_upload", "cells", "leaves", "m2l_pairs",
@@ -47,7 +47,9 @@ class StepParams(ctypes.Structure):
     _fields_ = [("f", ctypes.c_double), ("g", ctypes.c_double), ("Uinf", ctypes.c_double * 3),
                 ("sfs", ctypes.c_int32), ("Cs", ctypes.c_double),
                 ("clip_backscatter", ctypes.c_int32), ("relax_kind", ctypes.c_int32),
-                ("rlxf", ctypes.c_double)]
+                ("rlxf", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("sfs_rlxf", ctypes.c_double), ("minC", ctypes.c_double),
+                ("maxC", ctypes.c_double)]
 
 
 class VPMXError(RuntimeError):

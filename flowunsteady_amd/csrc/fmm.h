@@ -112,6 +112,10 @@ struct FmmState {
     DttBufs dtt;
 };
 
+// Estr over the near-field lists of the last FMM evaluation (J must be current).
+hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int transposed,
+                           const DeviceField &f, Scratch &s);
+
 // Dual tree traversal on the device: fills m2l_off/m2l_src and p2p_off/p2p_src
 // (CSR by target cell, sources ascending), growing the buffers in S.
 hipError_t fmm_traverse_device(hipStream_t st, FmmState &S, int64_t ncell, double theta,

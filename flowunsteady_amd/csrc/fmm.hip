@@ -799,6 +799,129 @@ void fmm_build_tables(FmmTables &T, int P)
 }
 
 // ---------------------------------------------------------------------------
+// SFS stretching over the near-field lists (Estr with the FMM)
+// ---------------------------------------------------------------------------
+// Estr(x_i) = sum_j zeta_sigma_j(x_i - x_j) (Gamma_j . grad)(U_i - U_j)
+// (docs/src/theory/rvpm.md:251-262) decays like zeta: every pair the FMM
+// treats as far field has a gap of at least sigcut * sigma_j, where zeta is
+// below e^{-24.5} zeta(0) for the gaussian kernels, so the sum over the P2P
+// lists is the whole sum to that precision.
+
+// Sorted Estr source records (x, y, z, 1/sigma) (Gamma, 0) (J0..3) (J4..7) (J8, 0, 0, 0).
+__global__ __launch_bounds__(FMM_BLOCK) void fmm_estr_gather_kernel(
+    const double *__restrict__ X, const double *__restrict__ G, const double *__restrict__ sg,
+    const double *__restrict__ Jm, const int32_t *__restrict__ perm, int64_t n, int64_t cap,
+    double4 *__restrict__ out)
+{
+    const int64_t s = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = perm[s];
+    out[5 * s] = make_double4(X[i], X[cap + i], X[2 * cap + i], 1.0 / sg[i]);
+    out[5 * s + 1] = make_double4(G[i], G[cap + i], G[2 * cap + i], 0.0);
+    out[5 * s + 2] = make_double4(Jm[i], Jm[cap + i], Jm[2 * cap + i], Jm[3 * cap + i]);
+    out[5 * s + 3] = make_double4(Jm[4 * cap + i], Jm[5 * cap + i], Jm[6 * cap + i], Jm[7 * cap + i]);
+    out[5 * s + 4] = make_double4(Jm[8 * cap + i], 0.0, 0.0, 0.0);
+}
+
+template <int KK, bool TRANSPOSED>
+__global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_kernel(const int32_t *__restrict__ leaves,
+                                                             const FmmCellDev *__restrict__ cells,
+                                                             const int64_t *__restrict__ off,
+                                                             const int32_t *__restrict__ src,
+                                                             const double4 *__restrict__ rec,
+                                                             const int32_t *__restrict__ perm,
+                                                             int64_t cap, double *__restrict__ E)
+{
+    __shared__ double4 tile[5 * P2P_BLOCK];
+    const int c = leaves[blockIdx.x];
+    const FmmCellDev A = cells[c];
+    for (int tb = A.begin; tb < A.end; tb += P2P_BLOCK) {
+        const int it = tb + threadIdx.x;
+        const bool active = it < A.end;
+        double xi = 0, yi = 0, zi = 0, Ji[9];
+        for (int k = 0; k < 9; ++k) Ji[k] = 0.0;
+        if (active) {
+            const double4 p = rec[5 * it], q2 = rec[5 * it + 2], q3 = rec[5 * it + 3],
+                          q4 = rec[5 * it + 4];
+            xi = p.x; yi = p.y; zi = p.z;
+            Ji[0] = q2.x; Ji[1] = q2.y; Ji[2] = q2.z; Ji[3] = q2.w;
+            Ji[4] = q3.x; Ji[5] = q3.y; Ji[6] = q3.z; Ji[7] = q3.w; Ji[8] = q4.x;
+        }
+        double e0 = 0, e1 = 0, e2 = 0;
+        for (int64_t e = off[c]; e < off[c + 1]; e++) {
+            const FmmCellDev B = cells[src[e]];
+            for (int sb = B.begin; sb < B.end; sb += P2P_BLOCK) {
+                __syncthreads();
+                const int js = sb + threadIdx.x;
+                if (js < B.end)
+                    for (int k = 0; k < 5; ++k) tile[5 * threadIdx.x + k] = rec[5 * js + k];
+                __syncthreads();
+                const int cnt = min(P2P_BLOCK, B.end - sb);
+                for (int jj = 0; jj < cnt; ++jj) {
+                    const double4 a = tile[5 * jj], gm = tile[5 * jj + 1];
+                    const double4 q2 = tile[5 * jj + 2], q3 = tile[5 * jj + 3], q4 = tile[5 * jj + 4];
+                    const double d1 = xi - a.x, d2 = yi - a.y, d3 = zi - a.z;
+                    const double r = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+                    const double zs = zeta_f64<KK>(r * a.w) * (a.w * a.w * a.w);
+                    double dJ[9];
+                    dJ[0] = Ji[0] - q2.x; dJ[1] = Ji[1] - q2.y; dJ[2] = Ji[2] - q2.z;
+                    dJ[3] = Ji[3] - q2.w; dJ[4] = Ji[4] - q3.x; dJ[5] = Ji[5] - q3.y;
+                    dJ[6] = Ji[6] - q3.z; dJ[7] = Ji[7] - q3.w; dJ[8] = Ji[8] - q4.x;
+                    double S0, S1, S2;
+                    if constexpr (TRANSPOSED) {
+                        S0 = dJ[0] * gm.x + dJ[1] * gm.y + dJ[2] * gm.z;
+                        S1 = dJ[3] * gm.x + dJ[4] * gm.y + dJ[5] * gm.z;
+                        S2 = dJ[6] * gm.x + dJ[7] * gm.y + dJ[8] * gm.z;
+                    } else {
+                        S0 = dJ[0] * gm.x + dJ[3] * gm.y + dJ[6] * gm.z;
+                        S1 = dJ[1] * gm.x + dJ[4] * gm.y + dJ[7] * gm.z;
+                        S2 = dJ[2] * gm.x + dJ[5] * gm.y + dJ[8] * gm.z;
+                    }
+                    e0 += zs * S0;
+                    e1 += zs * S1;
+                    e2 += zs * S2;
+                }
+            }
+        }
+        if (active) {
+            const int64_t i = perm[it];
+            E[i] = e0;
+            E[cap + i] = e1;
+            E[2 * cap + i] = e2;
+        }
+    }
+}
+
+hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int transposed,
+                           const DeviceField &f, Scratch &s)
+{
+    if (f.np == 0) return hipSuccess;
+    fmm_estr_gather_kernel<<<gridn(f.np, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
+        f.X, f.G, f.sigma, f.J, s.perm, f.np, f.cap, s.srcE);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int nleaf = (int)S.plan.leaves.size();
+    if (nleaf == 0) return hipSuccess;
+#define VPMX_FESTR(KK)                                                                          \
+    if (transposed)                                                                             \
+        fmm_estr_kernel<KK, true><<<nleaf, P2P_BLOCK, 0, st>>>(S.leaves, S.cells, S.p2p_off,     \
+                                                             S.p2p_src, s.srcE, s.perm, f.cap,  \
+                                                             f.E);                               \
+    else                                                                                        \
+        fmm_estr_kernel<KK, false><<<nleaf, P2P_BLOCK, 0, st>>>(S.leaves, S.cells, S.p2p_off,    \
+                                                              S.p2p_src, s.srcE, s.perm, f.cap, \
+                                                              f.E);
+    switch (kernel) {
+    case GAUSSIAN: VPMX_FESTR(GAUSSIAN); break;
+    case GAUSSIANERF: VPMX_FESTR(GAUSSIANERF); break;
+    case WINCKELMANS: VPMX_FESTR(WINCKELMANS); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef VPMX_FESTR
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Device dual tree traversal
 // ---------------------------------------------------------------------------
 // Level-synchronous: every frontier pair (target a, source b) is classified by

@@ -28,7 +28,9 @@ struct Timer {
 struct DeviceField {
     int64_t np = 0, cap = 0;
     double *X = nullptr, *G = nullptr, *sigma = nullptr, *U = nullptr, *J = nullptr;
-    double *E = nullptr, *C = nullptr, *M = nullptr;
+    double *E = nullptr, *C = nullptr, *M = nullptr;   // C: 3 components
+    double *D = nullptr;       // dynamic SFS: test-filter stretching and Estr (6)
+    double *sig0 = nullptr;    // dynamic SFS: saved sigma
     uint8_t *is_static = nullptr;
     bool any_static = false;
 };
@@ -93,6 +95,15 @@ hipError_t launch_uj_mixed_targets(hipStream_t st, int kernel, const DeviceField
 hipError_t launch_estr_f64(hipStream_t st, int kernel, int transposed, const DeviceField &f,
                            Scratch &s);
 hipError_t launch_sfs_coeff(hipStream_t st, const DeviceField &f, double Cs, int clip);
+// dynamic SFS: sig0 <- sigma, sigma <- alpha sigma
+hipError_t launch_sigma_scale(hipStream_t st, const DeviceField &f, double alpha);
+// sigma <- sig0
+hipError_t launch_sigma_restore(hipStream_t st, const DeviceField &f);
+// D[0..2] <- stretching(J, Gamma), D[3..5] <- E   (test filter)
+hipError_t launch_dyn_stash(hipStream_t st, const DeviceField &f, int transposed);
+// C from the test/domain differences, Lagrangian average, clipping
+hipError_t launch_dyn_coeff(hipStream_t st, const DeviceField &f, int transposed, double zeta0,
+                            double rlxf, double minC, double maxC, int clip);
 hipError_t launch_fill(hipStream_t st, double *p, int64_t n, double v);
 hipError_t launch_update(hipStream_t st, const DeviceField &f, int transposed, int sfs,
                          double fcoef, double gcoef, double zeta0, const double Uinf[3],

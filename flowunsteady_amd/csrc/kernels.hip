@@ -655,6 +655,127 @@ __global__ __launch_bounds__(EW_BLOCK) void sfs_coeff_kernel(int64_t n, int64_t 
     C[i] = c;
 }
 
+// ---- dynamic SFS procedure (docs/src/theory/rvpm.md:264-296) ---------------
+// C_d = <Gamma . L> / <Gamma . m> with the sigma-derivatives of the resolved
+// stretching (L) and of the SFS model (m) taken as differences between a test
+// filter alpha*sigma and the domain filter sigma; <.> is a per-particle
+// Lagrangian average  a <- w a_new + (1 - w) a_old  (first sample taken as is).
+__global__ __launch_bounds__(EW_BLOCK) void sigma_scale_kernel(int64_t n, double *__restrict__ sg,
+                                                               double *__restrict__ sig0,
+                                                               double alpha)
+{
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double s = sg[i];
+    sig0[i] = s;
+    sg[i] = alpha * s;
+}
+
+__global__ __launch_bounds__(EW_BLOCK) void sigma_restore_kernel(int64_t n, double *__restrict__ sg,
+                                                                 const double *__restrict__ sig0)
+{
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i < n) sg[i] = sig0[i];
+}
+
+__device__ __forceinline__ void stretch3(const double *__restrict__ J, int64_t cap, int64_t i,
+                                         double g0, double g1, double g2, int transposed,
+                                         double &s0, double &s1, double &s2)
+{
+    double Jm[9];
+    for (int k = 0; k < 9; ++k) Jm[k] = J[k * cap + i];
+    if (transposed) {
+        s0 = Jm[0] * g0 + Jm[1] * g1 + Jm[2] * g2;
+        s1 = Jm[3] * g0 + Jm[4] * g1 + Jm[5] * g2;
+        s2 = Jm[6] * g0 + Jm[7] * g1 + Jm[8] * g2;
+    } else {
+        s0 = Jm[0] * g0 + Jm[3] * g1 + Jm[6] * g2;
+        s1 = Jm[1] * g0 + Jm[4] * g1 + Jm[7] * g2;
+        s2 = Jm[2] * g0 + Jm[5] * g1 + Jm[8] * g2;
+    }
+}
+
+__global__ __launch_bounds__(EW_BLOCK) void dyn_stash_kernel(int64_t n, int64_t cap,
+                                                             const double *__restrict__ G,
+                                                             const double *__restrict__ J,
+                                                             const double *__restrict__ E,
+                                                             double *__restrict__ D, int transposed)
+{
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double s0, s1, s2;
+    stretch3(J, cap, i, G[i], G[cap + i], G[2 * cap + i], transposed, s0, s1, s2);
+    D[i] = s0;
+    D[cap + i] = s1;
+    D[2 * cap + i] = s2;
+    D[3 * cap + i] = E[i];
+    D[4 * cap + i] = E[cap + i];
+    D[5 * cap + i] = E[2 * cap + i];
+}
+
+__global__ __launch_bounds__(EW_BLOCK) void dyn_coeff_kernel(
+    int64_t n, int64_t cap, const double *__restrict__ G, const double *__restrict__ sg,
+    const double *__restrict__ J, const double *__restrict__ E, const double *__restrict__ D,
+    double *__restrict__ C, const uint8_t *__restrict__ st, int transposed, double zeta0,
+    double w, double minC, double maxC, int clip)
+{
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (st && st[i]) return;
+    const double g0 = G[i], g1 = G[cap + i], g2 = G[2 * cap + i];
+    double s0, s1, s2;
+    stretch3(J, cap, i, g0, g1, g2, transposed, s0, s1, s2);
+    const double e0 = E[i], e1 = E[cap + i], e2 = E[2 * cap + i];
+    const double nume = g0 * (D[i] - s0) + g1 * (D[cap + i] - s1) + g2 * (D[2 * cap + i] - s2);
+    const double sig = sg[i];
+    const double deno = (g0 * (D[3 * cap + i] - e0) + g1 * (D[4 * cap + i] - e1) +
+                         g2 * (D[5 * cap + i] - e2)) * (sig * sig * sig / zeta0);
+    const double n_old = C[cap + i], d_old = C[2 * cap + i];
+    double na = nume, da = deno;
+    if (n_old != 0.0 || d_old != 0.0) {
+        na = w * nume + (1.0 - w) * n_old;
+        da = w * deno + (1.0 - w) * d_old;
+    }
+    C[cap + i] = na;
+    C[2 * cap + i] = da;
+    double c = da != 0.0 ? na / da : 0.0;
+    c = fmin(fmax(c, minC), maxC);
+    if (clip && c * (g0 * e0 + g1 * e1 + g2 * e2) < 0.0) c = 0.0;
+    C[i] = c;
+}
+
+hipError_t launch_sigma_scale(hipStream_t st, const DeviceField &f, double alpha)
+{
+    if (f.np == 0) return hipSuccess;
+    sigma_scale_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(f.np, f.sigma, f.sig0, alpha);
+    return hipGetLastError();
+}
+
+hipError_t launch_sigma_restore(hipStream_t st, const DeviceField &f)
+{
+    if (f.np == 0) return hipSuccess;
+    sigma_restore_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(f.np, f.sigma, f.sig0);
+    return hipGetLastError();
+}
+
+hipError_t launch_dyn_stash(hipStream_t st, const DeviceField &f, int transposed)
+{
+    if (f.np == 0) return hipSuccess;
+    dyn_stash_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(f.np, f.cap, f.G, f.J, f.E, f.D,
+                                                                    transposed);
+    return hipGetLastError();
+}
+
+hipError_t launch_dyn_coeff(hipStream_t st, const DeviceField &f, int transposed, double zeta0,
+                            double rlxf, double minC, double maxC, int clip)
+{
+    if (f.np == 0) return hipSuccess;
+    dyn_coeff_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(
+        f.np, f.cap, f.G, f.sigma, f.J, f.E, f.D, f.C, f.any_static ? f.is_static : nullptr,
+        transposed, zeta0, rlxf, minC, maxC, clip);
+    return hipGetLastError();
+}
+
 hipError_t launch_sfs_coeff(hipStream_t st, const DeviceField &f, double Cs, int clip)
 {
     if (f.np == 0) return hipSuccess;

@@ -134,7 +134,7 @@ int dalloc(vpmx_ctx *c, T **p, size_t count)
 
 void free_field(DeviceField &f)
 {
-    for (double **p : {&f.X, &f.G, &f.sigma, &f.U, &f.J, &f.E, &f.C, &f.M}) {
+    for (double **p : {&f.X, &f.G, &f.sigma, &f.U, &f.J, &f.E, &f.C, &f.M, &f.D, &f.sig0}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
@@ -177,7 +177,9 @@ int ensure_capacity(vpmx_ctx *c, int64_t np)
     VPMX_TRY(dalloc(c, &f.U, 3 * cap));
     VPMX_TRY(dalloc(c, &f.J, 9 * cap));
     VPMX_TRY(dalloc(c, &f.E, 3 * cap));
-    VPMX_TRY(dalloc(c, &f.C, cap));
+    VPMX_TRY(dalloc(c, &f.C, 3 * cap));
+    VPMX_TRY(dalloc(c, &f.D, 6 * cap));
+    VPMX_TRY(dalloc(c, &f.sig0, cap));
     VPMX_TRY(dalloc(c, &f.M, 7 * cap));
     VPMX_TRY(dalloc(c, &f.is_static, cap));
     f.cap = cap;
@@ -225,7 +227,7 @@ int check_layout(vpmx_ctx *c, const vpmx_layout *L)
     struct { const char *name; int32_t off; int w; bool opt; } rows[] = {
         {"X", L->X, 3, false},         {"Gamma", L->Gamma, 3, false}, {"sigma", L->sigma, 1, false},
         {"vol", L->vol, 1, true},      {"U", L->U, 3, false},         {"J", L->J, 9, false},
-        {"SFS", L->SFS, 3, true},      {"C", L->C, 1, true},          {"static", L->static_, 1, true}};
+        {"SFS", L->SFS, 3, true},      {"C", L->C, 3, true},          {"static", L->static_, 1, true}};
     for (auto &r : rows)
         if (!row_ok(r.off, r.w, L->nrows, r.opt))
             return fail(c, VPMX_EINVAL,
@@ -254,7 +256,6 @@ int check_uj_params(vpmx_ctx *c, const vpmx_uj_params *P)
         if (P->ncrit < 1) return fail(c, VPMX_EINVAL, "FMM ncrit must be positive");
         if (!(P->theta > 0.0) || !isfinite(P->theta))
             return fail(c, VPMX_EINVAL, "FMM theta must be positive and finite");
-        if (P->sfs) return fail(c, VPMX_EUNSUPPORTED, "Estr with the FMM is not built (use fmm = 0)");
     }
     return VPMX_OK;
 }
@@ -265,15 +266,23 @@ int check_step_params(vpmx_ctx *c, const vpmx_step_params *S, double dt)
     if (!isfinite(dt)) return fail(c, VPMX_EINVAL, "dt is not finite");
     if (S->relax_kind < 0 || S->relax_kind > 2)
         return fail(c, VPMX_EINVAL, "unknown relaxation kind " + std::to_string(S->relax_kind));
-    if (S->sfs != VPMX_SFS_NONE && S->sfs != VPMX_SFS_CONSTANT)
+    if (S->sfs < VPMX_SFS_NONE || S->sfs > VPMX_SFS_DYNAMIC)
         return fail(c, VPMX_EINVAL, "unknown sfs model " + std::to_string(S->sfs));
+    if (S->sfs == VPMX_SFS_DYNAMIC) {
+        if (!(S->alpha > 0.0) || S->alpha == 1.0 || !isfinite(S->alpha))
+            return fail(c, VPMX_EINVAL, "dynamic SFS needs a test filter factor alpha > 0, != 1");
+        if (!(S->sfs_rlxf >= 0.0 && S->sfs_rlxf <= 1.0))
+            return fail(c, VPMX_EINVAL, "dynamic SFS averaging weight must be in [0, 1]");
+        if (!(S->minC <= S->maxC))
+            return fail(c, VPMX_EINVAL, "dynamic SFS needs minC <= maxC");
+    }
     if (!isfinite(S->f) || !isfinite(S->g) || 1.0 + 3.0 * S->f == 0.0)
         return fail(c, VPMX_EINVAL, "invalid formulation coefficients f, g");
     return VPMX_OK;
 }
 
 // ---- host <-> device ------------------------------------------------------------------
-// Staging layout (doubles): X[3 np] G[3 np] sigma[np] | U[3 np] J[9 np] | E[3 np] C[np]
+// Upload staging (doubles): X[3 np] G[3 np] sigma[np] C[3 np] | U[3 np] J[9 np],
 // followed by np bytes of static flags.
 int upload(vpmx_ctx *c, const double *P, int64_t np, const vpmx_layout *L, bool with_uj)
 {
@@ -281,7 +290,7 @@ int upload(vpmx_ctx *c, const double *P, int64_t np, const vpmx_layout *L, bool 
     DeviceField &f = c->f;
     f.np = np;
     if (np == 0) return VPMX_OK;
-    const size_t nd = (size_t)(with_uj ? 19 : 7) * np;
+    const size_t nd = (size_t)(with_uj ? 22 : 10) * np;
     VPMX_TRY(ensure_stage(c, nd * sizeof(double) + np));
     double *h = c->h_stage;
     uint8_t *hs = (uint8_t *)(h + nd);
@@ -302,12 +311,13 @@ int upload(vpmx_ctx *c, const double *P, int64_t np, const vpmx_layout *L, bool 
             return fail(c, VPMX_EINVAL, "particle " + std::to_string(p + 1) + " has sigma = " +
                                             std::to_string(s) + " (must be positive and finite)");
         h[6 * np + p] = s;
+        for (int k = 0; k < 3; ++k) h[(7 + k) * np + p] = L->C >= 0 ? col[L->C + k] : 0.0;
         const uint8_t st = (L->static_ >= 0 && col[L->static_] != 0.0) ? 1 : 0;
         hs[p] = st;
         any_static |= st != 0;
         if (with_uj) {
-            for (int k = 0; k < 3; ++k) h[(7 + k) * np + p] = col[L->U + k];
-            for (int k = 0; k < 9; ++k) h[(10 + k) * np + p] = col[L->J + k];
+            for (int k = 0; k < 3; ++k) h[(10 + k) * np + p] = col[L->U + k];
+            for (int k = 0; k < 9; ++k) h[(13 + k) * np + p] = col[L->J + k];
         }
     }
     f.any_static = any_static;
@@ -318,12 +328,14 @@ int upload(vpmx_ctx *c, const double *P, int64_t np, const vpmx_layout *L, bool 
         VPMX_HIP(c, hipMemcpyAsync(f.G + k * f.cap, h + (3 + k) * np, b, hipMemcpyHostToDevice, st));
     }
     VPMX_HIP(c, hipMemcpyAsync(f.sigma, h + 6 * np, b, hipMemcpyHostToDevice, st));
+    for (int k = 0; k < 3; ++k)
+        VPMX_HIP(c, hipMemcpyAsync(f.C + k * f.cap, h + (7 + k) * np, b, hipMemcpyHostToDevice, st));
     VPMX_HIP(c, hipMemcpyAsync(f.is_static, hs, (size_t)np, hipMemcpyHostToDevice, st));
     if (with_uj) {
         for (int k = 0; k < 3; ++k)
-            VPMX_HIP(c, hipMemcpyAsync(f.U + k * f.cap, h + (7 + k) * np, b, hipMemcpyHostToDevice, st));
+            VPMX_HIP(c, hipMemcpyAsync(f.U + k * f.cap, h + (10 + k) * np, b, hipMemcpyHostToDevice, st));
         for (int k = 0; k < 9; ++k)
-            VPMX_HIP(c, hipMemcpyAsync(f.J + k * f.cap, h + (10 + k) * np, b, hipMemcpyHostToDevice, st));
+            VPMX_HIP(c, hipMemcpyAsync(f.J + k * f.cap, h + (13 + k) * np, b, hipMemcpyHostToDevice, st));
     }
     return sync(c);   // staging is reused: the copies must finish
 }
@@ -335,7 +347,7 @@ int download(vpmx_ctx *c, double *P, const vpmx_layout *L, int what)
     DeviceField &f = c->f;
     const int64_t np = f.np;
     if (np == 0) return VPMX_OK;
-    VPMX_TRY(ensure_stage(c, (size_t)23 * np * sizeof(double)));
+    VPMX_TRY(ensure_stage(c, (size_t)25 * np * sizeof(double)));
     double *h = c->h_stage;
     hipStream_t st = c->stream;
     const size_t b = (size_t)np * sizeof(double);
@@ -356,7 +368,8 @@ int download(vpmx_ctx *c, double *P, const vpmx_layout *L, int what)
     if (sfs) {
         for (int k = 0; k < 3; ++k)
             VPMX_HIP(c, hipMemcpyAsync(h + (19 + k) * np, f.E + k * f.cap, b, hipMemcpyDeviceToHost, st));
-        VPMX_HIP(c, hipMemcpyAsync(h + 22 * np, f.C, b, hipMemcpyDeviceToHost, st));
+        for (int k = 0; k < 3; ++k)
+            VPMX_HIP(c, hipMemcpyAsync(h + (22 + k) * np, f.C + k * f.cap, b, hipMemcpyDeviceToHost, st));
     }
     VPMX_TRY(sync(c));
     const int64_t ld = L->nrows;
@@ -376,7 +389,8 @@ int download(vpmx_ctx *c, double *P, const vpmx_layout *L, int what)
         if (sfs) {
             if (L->SFS >= 0)
                 for (int k = 0; k < 3; ++k) col[L->SFS + k] = h[(19 + k) * np + p];
-            if (L->C >= 0) col[L->C] = h[22 * np + p];
+            if (L->C >= 0)
+                for (int k = 0; k < 3; ++k) col[L->C + k] = h[(22 + k) * np + p];
         }
     }
     return VPMX_OK;
@@ -660,13 +674,29 @@ int enqueue_uj(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
     return VPMX_OK;
 }
 
-int enqueue_sfs(vpmx_ctx *c, int kernel, int transposed, double Cs, int clip, bool coeff)
+// Estr with the scheme of the UJ that just ran (direct all-pairs, or the FMM's
+// near-field lists).
+int enqueue_estr(vpmx_ctx *c, const vpmx_uj_params *U)
 {
-    {
-        TimedScope ts(c, T_ESTR);
-        VPMX_HIP(c, launch_estr_f64(c->stream, kernel, transposed, c->f, c->s));
+    TimedScope ts(c, T_ESTR);
+    if (U->fmm) {
+        if (!c->fmm) return fail(c, VPMX_EINVAL, "Estr with the FMM needs an FMM UJ first");
+        VPMX_HIP(c, launch_fmm_estr(c->stream, *c->fmm, U->kernel, U->transposed, c->f, c->s));
+    } else {
+        VPMX_HIP(c, launch_estr_f64(c->stream, U->kernel, U->transposed, c->f, c->s));
     }
-    if (coeff) VPMX_HIP(c, launch_sfs_coeff(c->stream, c->f, Cs, clip));
+    return VPMX_OK;
+}
+
+// Dynamic procedure, before the stage's UJ: UJ and Estr with the test filter
+// alpha*sigma, stretching and Estr stashed in D, sigma restored.
+int enqueue_dynamic_test_filter(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S)
+{
+    VPMX_HIP(c, launch_sigma_scale(c->stream, c->f, S->alpha));
+    VPMX_TRY(enqueue_uj(c, U, 1));
+    VPMX_TRY(enqueue_estr(c, U));
+    VPMX_HIP(c, launch_dyn_stash(c->stream, c->f, U->transposed));
+    VPMX_HIP(c, launch_sigma_restore(c->stream, c->f));
     return VPMX_OK;
 }
 
@@ -686,8 +716,16 @@ int enqueue_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S
     const int nst = rk3 ? 3 : 1;
     if (rk3) VPMX_HIP(c, launch_fill(c->stream, f.M, 7 * f.cap, 0.0));
     for (int s = 0; s < nst; ++s) {
+        if (S->sfs == VPMX_SFS_DYNAMIC) VPMX_TRY(enqueue_dynamic_test_filter(c, U, S));
         VPMX_TRY(enqueue_uj(c, U, 1));
-        if (sfs) VPMX_TRY(enqueue_sfs(c, U->kernel, U->transposed, S->Cs, S->clip_backscatter, true));
+        if (sfs) {
+            VPMX_TRY(enqueue_estr(c, U));
+            if (S->sfs == VPMX_SFS_DYNAMIC)
+                VPMX_HIP(c, launch_dyn_coeff(c->stream, f, U->transposed, zeta0, S->sfs_rlxf,
+                                             S->minC, S->maxC, S->clip_backscatter));
+            else
+                VPMX_HIP(c, launch_sfs_coeff(c->stream, f, S->Cs, S->clip_backscatter));
+        }
         TimedScope ts(c, T_UPDATE);
         VPMX_HIP(c, launch_update(c->stream, f, U->transposed, sfs, S->f, S->g, zeta0, S->Uinf,
                                   rk3 ? RK3_A[s] : 0.0, rk3 ? RK3_B[s] : 1.0, dt));
@@ -781,7 +819,7 @@ int vpmx_uj(vpmx_ctx *c, double *particles, int64_t np, const vpmx_layout *L,
     if (P->sfs && L->SFS < 0) return fail(c, VPMX_EINVAL, "sfs requested but layout.SFS = -1");
     VPMX_TRY(upload(c, particles, np, L, !P->reset));
     VPMX_TRY(enqueue_uj(c, P, P->reset));
-    if (P->sfs) VPMX_TRY(enqueue_sfs(c, P->kernel, P->transposed, 0.0, 0, false));
+    if (P->sfs) VPMX_TRY(enqueue_estr(c, P));
     VPMX_TRY(sync(c));
     return download(c, particles, L, DL_UJ | (P->sfs ? DL_SFS : 0));
 }
@@ -838,7 +876,7 @@ int vpmx_uj_resident(vpmx_ctx *c, const vpmx_uj_params *P)
     VPMX_TRY(entry(c));
     VPMX_TRY(check_uj_params(c, P));
     VPMX_TRY(enqueue_uj(c, P, P->reset));
-    if (P->sfs) VPMX_TRY(enqueue_sfs(c, P->kernel, P->transposed, 0.0, 0, false));
+    if (P->sfs) VPMX_TRY(enqueue_estr(c, P));
     return sync(c);
 }
 

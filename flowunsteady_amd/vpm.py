@@ -130,10 +130,25 @@ class SubFilterScale:
     kind: int
     Cs: float = 0.0
     clip_backscatter: bool = True
+    alpha: float = 0.999     # dynamic: test filter width alpha * sigma
+    rlxf: float = 0.005      # dynamic: weight of the newest sample in the Lagrangian average
+    minC: float = 0.0
+    maxC: float = 1.0
 
 
 noSFS = SubFilterScale("noSFS", N.SFS_NONE)
 SFS_none = noSFS
+
+
+def DynamicSFS(alpha: float = 0.999, rlxf: float = 0.005, minC: float = 0.0, maxC: float = 1.0,
+               clip_backscatter: bool = True) -> SubFilterScale:
+    """Dynamic model coefficient (vpm.DynamicSFS(Estr, pseudo3level_positive;
+    alpha, maxC, clippings=[clipping_backscatter]), examples/rotorhover/rotorhover.jl:53-55):
+    C = <Gamma . (S(alpha sigma) - S(sigma))> / <Gamma . (E(alpha sigma) - E(sigma)) sigma^3/zeta0>
+    with a per-particle Lagrangian average, bounded to [minC, maxC] (minC = 0:
+    positive), then backscatter clipping (docs/src/theory/rvpm.md:264-296)."""
+    return SubFilterScale("DynamicSFS", N.SFS_DYNAMIC, 0.0, bool(clip_backscatter),
+                          float(alpha), float(rlxf), float(minC), float(maxC))
 
 
 def ConstantSFS(Cs: float, clip_backscatter: bool = True) -> SubFilterScale:
@@ -232,6 +247,8 @@ class ParticleField:
         S.clip_backscatter = int(self.SFS.clip_backscatter)
         S.relax_kind = self.relaxation.kind
         S.rlxf = self.relaxation.rlxf
+        S.alpha, S.sfs_rlxf = self.SFS.alpha, self.SFS.rlxf
+        S.minC, S.maxC = self.SFS.minC, self.SFS.maxC
         return S
 
 

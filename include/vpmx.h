@@ -68,7 +68,9 @@ extern "C" {
 
 /* SFS models (src/FLOWUnsteady_simulation.jl:39,130). */
 #define VPMX_SFS_NONE 0
-#define VPMX_SFS_CONSTANT 1   /* C = Cs with Estr_direct, optional clipping   */
+#define VPMX_SFS_CONSTANT 1   /* C = Cs with Estr, optional clipping          */
+#define VPMX_SFS_DYNAMIC 2    /* C from the dynamic procedure (test filter
+                                 alpha*sigma, Lagrangian average), clipped   */
 
 /* ---- types ----------------------------------------------------------------- */
 typedef struct vpmx_ctx vpmx_ctx;
@@ -85,7 +87,9 @@ typedef struct {
     int32_t U;            /* 3 rows: velocity (output)                     */
     int32_t J;            /* 9 rows: velocity Jacobian (output)            */
     int32_t SFS;          /* 3 rows: Estr (output when sfs != 0; -1 allowed) */
-    int32_t C;            /* 1 row : SFS coefficient (-1 allowed)          */
+    int32_t C;            /* 3 rows: SFS coefficient and the Lagrangian
+                             averages of the dynamic procedure's numerator
+                             and denominator (-1 allowed)                  */
     int32_t static_;      /* 1 row : static flag, nonzero = static (-1 allowed) */
 } vpmx_layout;
 
@@ -114,6 +118,9 @@ typedef struct {
     int32_t clip_backscatter; /* clip C to 0 where C Gamma.Estr < 0         */
     int32_t relax_kind;   /* VPMX_RELAX_*                                   */
     double rlxf;          /* relaxation factor                              */
+    /* dynamic SFS (sfs = VPMX_SFS_DYNAMIC): test filter width alpha*sigma,
+     * Lagrangian-average weight of the newest sample, bounds of C */
+    double alpha, sfs_rlxf, minC, maxC;
 } vpmx_step_params;
 
 /* ---- context ----------------------------------------------------------------- */

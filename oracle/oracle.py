@@ -26,7 +26,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 class _Field(ctypes.Structure):
     _fields_ = [("np", ctypes.c_int64),
                 ("X", _dp), ("G", _dp), ("sigma", _dp), ("U", _dp), ("J", _dp),
-                ("E", _dp), ("C", _dp), ("M", _dp),
+                ("E", _dp), ("M", _dp), ("C", _dp),
                 ("is_static", ctypes.POINTER(ctypes.c_uint8))]
 
 
@@ -36,7 +36,9 @@ class _Params(ctypes.Structure):
                 ("Uinf", ctypes.c_double * 3),
                 ("sfs", ctypes.c_int), ("Cs", ctypes.c_double),
                 ("clip_backscatter", ctypes.c_int),
-                ("relax_kind", ctypes.c_int), ("rlxf", ctypes.c_double)]
+                ("relax_kind", ctypes.c_int), ("rlxf", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("sfs_rlxf", ctypes.c_double),
+                ("minC", ctypes.c_double), ("maxC", ctypes.c_double)]
 
 
 def build(force: bool = False) -> str:
@@ -173,7 +175,7 @@ class Field:
         self.sigma = _c(sigma, (-1,)).copy()
         n = self.X.shape[0]
         self.U = np.zeros((n, 3)); self.J = np.zeros((n, 9)); self.E = np.zeros((n, 3))
-        self.C = np.zeros(n); self.M = np.zeros((n, 7))
+        self.C = np.zeros((n, 3)); self.M = np.zeros((n, 7))
         self.is_static = None if is_static is None else np.ascontiguousarray(is_static, np.uint8)
 
     def _struct(self):
@@ -187,7 +189,9 @@ class Field:
 
 
 def make_params(kernel="gaussianerf", transposed=True, formulation="rVPM", Uinf=(0, 0, 0),
-                sfs=False, Cs=0.0, clip_backscatter=True, relax_kind=0, rlxf=0.3):
+                sfs=False, Cs=0.0, clip_backscatter=True, relax_kind=0, rlxf=0.3,
+                alpha=0.999, sfs_rlxf=0.005, minC=0.0, maxC=1.0):
+    """sfs: False/0 none, True/1 constant Cs, "dynamic"/2 dynamic procedure."""
     P = _Params()
     P.kernel = _kid(kernel)
     P.transposed = int(transposed)
@@ -199,8 +203,10 @@ def make_params(kernel="gaussianerf", transposed=True, formulation="rVPM", Uinf=
         P.f, P.g = formulation
     for k in range(3):
         P.Uinf[k] = float(Uinf[k])
-    P.sfs = int(bool(sfs)); P.Cs = float(Cs); P.clip_backscatter = int(clip_backscatter)
+    P.sfs = 2 if sfs in ("dynamic", 2) else int(bool(sfs))
+    P.Cs = float(Cs); P.clip_backscatter = int(clip_backscatter)
     P.relax_kind = int(relax_kind); P.rlxf = float(rlxf)
+    P.alpha, P.sfs_rlxf, P.minC, P.maxC = float(alpha), float(sfs_rlxf), float(minC), float(maxC)
     return P
 
 

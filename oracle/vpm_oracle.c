@@ -276,7 +276,8 @@ void orc_relax(int kind, double rlxf, const double *J, double *G)
 /* Particle field state for the integrators: AoS per particle. */
 typedef struct {
     int64_t np;
-    double *X, *G, *sigma, *U, *J, *E, *C, *M; /* M: 7 registers per particle */
+    double *X, *G, *sigma, *U, *J, *E, *M;    /* M: 7 registers per particle */
+    double *C;                                /* 3 per particle: C, <num>, <den> */
     const uint8_t *is_static;                  /* may be NULL: no static particles */
 } orc_field;
 
@@ -284,11 +285,12 @@ typedef struct {
     int kernel, transposed;
     double f, g;               /* formulation: rVPM f=0 g=1/5, cVPM f=g=0 */
     double Uinf[3];
-    int sfs;                   /* 0 none, 1 constant Cs with Estr_direct */
+    int sfs;                   /* 0 none, 1 constant Cs, 2 dynamic; Estr_direct */
     double Cs;
     int clip_backscatter;
     int relax_kind;            /* 0 none, 1 pedrizzetti, 2 corrected */
     double rlxf;
+    double alpha, sfs_rlxf, minC, maxC;   /* dynamic procedure */
 } orc_params;
 
 static void field_uj(const orc_params *P, orc_field *F)
@@ -298,22 +300,85 @@ static void field_uj(const orc_params *P, orc_field *F)
     orc_uj_direct(P->kernel, F->np, F->X, F->np, F->X, F->G, F->sigma, F->U, F->J);
 }
 
-/* Constant-coefficient SFS after UJ: E = Estr_direct, C = Cs, then backscatter
- * clipping C = 0 where C Gamma.E < 0 (docs/src/theory/rvpm.md:296). */
-static void field_sfs(const orc_params *P, orc_field *F)
+static void field_estr(const orc_params *P, orc_field *F)
 {
-    if (!P->sfs) return;
     memset(F->E, 0, sizeof(double) * 3 * F->np);
     orc_estr_direct(P->kernel, P->transposed, F->np, F->X, F->J, F->np, F->X, F->G, F->sigma,
                     F->J, F->E);
-    for (int64_t i = 0; i < F->np; i++) {
-        double c = P->Cs;
-        if (P->clip_backscatter) {
-            const double *g = F->G + 3 * i, *e = F->E + 3 * i;
-            if (c * (g[0] * e[0] + g[1] * e[1] + g[2] * e[2]) < 0.0) c = 0.0;
+}
+
+static int clipped(const orc_params *P, const orc_field *F, int64_t i, double c)
+{
+    const double *g = F->G + 3 * i, *e = F->E + 3 * i;
+    return P->clip_backscatter && c * (g[0] * e[0] + g[1] * e[1] + g[2] * e[2]) < 0.0;
+}
+
+/* One stage's UJ and SFS coefficient.
+ * Constant model: E = Estr_direct, C = Cs, then backscatter clipping C = 0
+ * where C Gamma.E < 0 (docs/src/theory/rvpm.md:296).
+ * Dynamic model (rvpm.md:264-296, C_d = <Gamma.L>/<Gamma.m>): the sigma
+ * derivatives of the resolved stretching (L) and of the SFS model (m) are
+ * taken as the differences between a test filter alpha*sigma and the domain
+ * filter sigma, so per particle
+ *   num = Gamma . (S(alpha sigma) - S(sigma)),
+ *   den = Gamma . (E(alpha sigma) - E(sigma)) sigma^3 / zeta0,
+ * averaged along the particle's trajectory as a <- w a + (1 - w) a_old (the
+ * first sample is taken as is), C = <num>/<den> bounded to [minC, maxC], then
+ * backscatter clipping.  This restates the documented procedure; FLOWVPM's
+ * own implementation is not in the reference tree. */
+static void field_stage(const orc_params *P, orc_field *F)
+{
+    const int64_t n = F->np;
+    double *D = NULL;
+    if (P->sfs == 2) {
+        D = malloc(sizeof(double) * 6 * (n > 0 ? n : 1));
+        double *sig0 = malloc(sizeof(double) * (n > 0 ? n : 1));
+        for (int64_t i = 0; i < n; i++) {
+            sig0[i] = F->sigma[i];
+            F->sigma[i] = P->alpha * sig0[i];
         }
-        F->C[i] = c;
+        field_uj(P, F);
+        field_estr(P, F);
+        for (int64_t i = 0; i < n; i++) {
+            orc_stretching(P->transposed, F->J + 9 * i, F->G + 3 * i, D + 6 * i);
+            for (int k = 0; k < 3; k++) D[6 * i + 3 + k] = F->E[3 * i + k];
+            F->sigma[i] = sig0[i];
+        }
+        free(sig0);
     }
+    field_uj(P, F);
+    if (!P->sfs) return;
+    field_estr(P, F);
+    const double zeta0 = orc_zeta0(P->kernel);
+    for (int64_t i = 0; i < n; i++) {
+        if (P->sfs == 1) {
+            double c = P->Cs;
+            if (clipped(P, F, i, c)) c = 0.0;
+            F->C[3 * i] = c;
+            continue;
+        }
+        if (F->is_static && F->is_static[i]) continue;
+        double S[3];
+        const double *g = F->G + 3 * i, *e = F->E + 3 * i, *d = D + 6 * i;
+        orc_stretching(P->transposed, F->J + 9 * i, g, S);
+        double num = g[0] * (d[0] - S[0]) + g[1] * (d[1] - S[1]) + g[2] * (d[2] - S[2]);
+        double sg = F->sigma[i];
+        double den = (g[0] * (d[3] - e[0]) + g[1] * (d[4] - e[1]) + g[2] * (d[5] - e[2])) *
+                     (sg * sg * sg / zeta0);
+        double *C = F->C + 3 * i;
+        double na = num, da = den;
+        if (C[1] != 0.0 || C[2] != 0.0) {
+            na = P->sfs_rlxf * num + (1.0 - P->sfs_rlxf) * C[1];
+            da = P->sfs_rlxf * den + (1.0 - P->sfs_rlxf) * C[2];
+        }
+        C[1] = na;
+        C[2] = da;
+        double c = da != 0.0 ? na / da : 0.0;
+        c = fmin(fmax(c, P->minC), P->maxC);
+        if (clipped(P, F, i, c)) c = 0.0;
+        C[0] = c;
+    }
+    free(D);
 }
 
 static void field_relax(const orc_params *P, orc_field *F)
@@ -335,13 +400,12 @@ void orc_rk3_step(const orc_params *P, orc_field *F, double dt, int relax)
     memset(F->M, 0, sizeof(double) * 7 * F->np);
     for (int s = 0; s < 3; s++) {
         double a = LSRK3_A[s], b = LSRK3_B[s];
-        field_uj(P, F);
-        field_sfs(P, F);
+        field_stage(P, F);
         for (int64_t i = 0; i < F->np; i++) {
             if (F->is_static && F->is_static[i]) continue;
             double dX[3], dG[3], ds;
             orc_rhs(P->transposed, P->f, P->g, zeta0, P->Uinf, F->U + 3 * i, F->J + 9 * i,
-                    F->G + 3 * i, F->sigma[i], P->sfs ? F->C[i] : 0.0,
+                    F->G + 3 * i, F->sigma[i], P->sfs ? F->C[3 * i] : 0.0,
                     P->sfs ? F->E + 3 * i : NULL, dX, dG, &ds);
             double *M = F->M + 7 * i;
             for (int k = 0; k < 3; k++) {
@@ -364,13 +428,12 @@ void orc_rk3_step(const orc_params *P, orc_field *F, double dt, int relax)
 void orc_euler_step(const orc_params *P, orc_field *F, double dt, int relax)
 {
     double zeta0 = orc_zeta0(P->kernel);
-    field_uj(P, F);
-    field_sfs(P, F);
+    field_stage(P, F);
     for (int64_t i = 0; i < F->np; i++) {
         if (F->is_static && F->is_static[i]) continue;
         double dX[3], dG[3], ds;
         orc_rhs(P->transposed, P->f, P->g, zeta0, P->Uinf, F->U + 3 * i, F->J + 9 * i,
-                F->G + 3 * i, F->sigma[i], P->sfs ? F->C[i] : 0.0, P->sfs ? F->E + 3 * i : NULL,
+                F->G + 3 * i, F->sigma[i], P->sfs ? F->C[3 * i] : 0.0, P->sfs ? F->E + 3 * i : NULL,
                 dX, dG, &ds);
         for (int k = 0; k < 3; k++) {
             F->X[3 * i + k] += dt * dX[k];

@@ -119,13 +119,3 @@ def test_fmm_rk3_step_close_to_direct():
                                         ctypes.byref(P), ctypes.byref(S), 0.01, 1))
     assert normwise(vpm.get_X(b), vpm.get_X(a))[0] < 1e-6
     assert normwise(vpm.get_Gamma(b), vpm.get_Gamma(a))[0] < 1e-5
-
-
-def test_fmm_sfs_rejected():
-    X, G, s, _ = workloads.uniform_cube(n=100, seed=1)
-    pf = fmm_field(X, G, s)
-    import ctypes
-    P = pf._uj_params(sfs=True, fmm=True)
-    ctx = vpm._context(0)
-    code = N.lib().vpmx_uj(ctx.ptr, pf._ptr(), pf.np, ctypes.byref(pf.layout), ctypes.byref(P))
-    assert code == N.EUNSUPPORTED

@@ -1,0 +1,86 @@
+"""GPU parity of the SFS paths: the dynamic procedure (direct UJ) against the
+oracle's, and the SFS stretching over the FMM's near-field lists against the
+direct oracle."""
+import numpy as np
+import pytest
+
+from flowunsteady_amd import vpm, workloads
+from helpers import make_field, normwise
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def blob(n=3000, seed=4):
+    X, G, s, _ = workloads.gaussian_blob(n=n, seed=seed, std=0.3)
+    return X, G, s * 3
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_dynamic_sfs_rk3_vs_oracle(kernel):
+    X, G, s = blob()
+    sfs = vpm.DynamicSFS(alpha=0.999, rlxf=0.005, minC=0.0, maxC=1.0)
+    pf = make_field(X, G, s, kernel=kernel, SFS=sfs, relaxation=vpm.pedrizzetti)
+    P = O.make_params(kernel, True, "rVPM", sfs="dynamic", alpha=0.999, sfs_rlxf=0.005,
+                      minC=0.0, maxC=1.0, relax_kind=1, rlxf=0.3)
+    F = O.Field(X, G, s)
+    for i in range(1, 3):
+        vpm.nextstep(pf, 0.002, relax=True)
+        O.rk3_step(P, F, 0.002, True)
+    for get, ref in ((vpm.get_X, F.X), (vpm.get_Gamma, F.G)):
+        assert normwise(get(pf), ref)[0] < 1e-12
+    C = pf.particles[vpm.C_INDEX, :pf.np].T
+    # the alpha = 0.999 differences cancel ~3 digits: compare normwise at 1e-9
+    for k in range(3):
+        scale = np.abs(F.C[:, k]).max()
+        assert np.abs(C[:, k] - F.C[:, k]).max() <= 1e-9 * scale, k
+    assert np.any(C[:, 0] > 0)
+
+
+def test_estr_fmm_vs_direct():
+    X, G, s, _ = workloads.rotor_wake(n=1 << 14, seed=3)
+    pf = make_field(X, G, s, fmm=vpm.FMM(p=10, ncrit=50, theta=0.4))
+    import ctypes
+    from flowunsteady_amd import _native as N
+    P = pf._uj_params(sfs=True, fmm=True)
+    ctx = vpm._context(0)
+    ctx.check(N.lib().vpmx_uj(ctx.ptr, pf._ptr(), pf.np, ctypes.byref(pf.layout), ctypes.byref(P)))
+    U, J = O.uj_direct("gaussianerf", X, X, G, s)
+    E = O.estr_direct("gaussianerf", True, X, J, X, G, s, J)
+    e = normwise(vpm.get_SFS(pf), E)
+    print("Estr fmm vs direct", e)
+    assert e[0] < 1e-5
+
+
+def test_fmm_rk3_with_constant_sfs():
+    X, G, s = blob(4000)
+    pf = make_field(X, G, s, SFS=vpm.ConstantSFS(0.2), relaxation=vpm.pedrizzetti,
+                    fmm=vpm.FMM(p=10, ncrit=50, theta=0.4))
+    pf.UJ = vpm.UJ_fmm_gpu
+    P = O.make_params("gaussianerf", True, "rVPM", sfs=True, Cs=0.2, relax_kind=1, rlxf=0.3)
+    F = O.Field(X, G, s)
+    for i in range(1, 3):
+        vpm.nextstep(pf, 0.002, relax=True)
+        O.rk3_step(P, F, 0.002, True)
+    ex, eg = normwise(vpm.get_X(pf), F.X), normwise(vpm.get_Gamma(pf), F.G)
+    print("fmm rk3 + sfs vs direct oracle", ex, eg)
+    assert ex[0] < 1e-7 and eg[0] < 1e-5
+
+
+def test_dynamic_sfs_mixed_runs():
+    X, G, s = blob()
+    a = make_field(X, G, s, SFS=vpm.DynamicSFS(), precision="fp64")
+    b = make_field(X, G, s, SFS=vpm.DynamicSFS(), precision="mixed")
+    for pf in (a, b):
+        vpm.nextstep(pf, 0.002)
+    Ca = a.particles[vpm.C_INDEX.start, :a.np]
+    Cb = b.particles[vpm.C_INDEX.start, :b.np]
+    assert np.all(np.isfinite(Cb)) and np.all((Cb >= 0) & (Cb <= 1))
+    # Known limitation: the alpha = 0.999 test-filter differences cancel ~3
+    # digits, so FP32 pair errors reach the coefficient ~1e3 amplified and the
+    # [0, 1] bounds / backscatter clipping flip a few particles.  Bound the
+    # population, not the worst particle.
+    e = normwise(vpm.get_Gamma(b), vpm.get_Gamma(a))
+    print("C mixed vs fp64: mean", Ca.mean(), Cb.mean(), "Gamma err", e)
+    assert e[1] < 5e-3
+    assert abs(Cb.mean() - Ca.mean()) < 0.05 * abs(Ca.mean()) + 1e-3
