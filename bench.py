@@ -117,6 +117,103 @@ def uj_at(ctx, X_targets, kernel, precision):
     return U, J
 
 
+def batched_section(ctx, log, nfields=64, seed=5, reps=2, steps=2):
+    """64 independent fields, N_k log-uniform in [5e3, 2.5e5] (rings and helices),
+    mixed gaussian-erf UJ + stretching: per-case sequential launches vs one
+    batched launch under the autotuned schedule, and the batched RK3 step with
+    and without graph replay."""
+    from flowunsteady_amd import _native as N
+    from flowunsteady_amd import vpm, workloads
+    L = N.lib()
+    cases = workloads.batched_fields(nfields=nfields, seed=seed)
+    pfs = []
+    for X, G, s, v in cases:
+        pf = vpm.ParticleField(len(X), precision="mixed", relaxation=vpm.pedrizzetti)
+        vpm.add_particles(pf, X, G, s, vol=v)
+        pfs.append(pf)
+    sizes = np.array([pf.np for pf in pfs], dtype=np.float64)
+    pairs = float(np.sum(sizes ** 2))
+    P = pfs[0]._uj_params(fmm=False)
+    lay = ctypes.byref(pfs[0].layout)
+    # per-case sequential launches (vpmx_uj_resident on each field in turn)
+    order = np.argsort(sizes)
+    ctx.check(L.vpmx_upload(ctx.ptr, pfs[order[0]]._ptr(), pfs[order[0]].np, lay))
+    ctx.check(L.vpmx_uj_resident(ctx.ptr, ctypes.byref(P)))     # warm-up
+    seq_s, seq_U = 0.0, {}
+    for k, pf in enumerate(pfs):
+        ctx.check(L.vpmx_upload(ctx.ptr, pf._ptr(), pf.np, lay))
+        t0 = time.perf_counter()
+        ctx.check(L.vpmx_uj_resident(ctx.ptr, ctypes.byref(P)))
+        seq_s += time.perf_counter() - t0
+        if k < 4:
+            ctx.check(L.vpmx_download(ctx.ptr, pf._ptr(), pf.np, lay))
+            seq_U[k] = vpm.get_U(pf).copy()
+    log(f"batched C5: sequential {seq_s:.3f} s ({pairs / seq_s:.3e} interactions/s)")
+    # one batch, autotuned
+    ptrs = N.pointer_array([pf.particles for pf in pfs])
+    nps = (ctypes.c_int64 * len(pfs))(*[pf.np for pf in pfs])
+    ctx.check(L.vpmx_batch_upload(ctx.ptr, len(pfs), ptrs, nps, lay))
+    tune = ctx.autotune(P, reps=1)
+    sched, block, grid = ctx.schedule()
+    log("autotune:", tune, "->", (sched, block, grid))
+    ctx.check(L.vpmx_uj_resident(ctx.ptr, ctypes.byref(P)))     # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.check(L.vpmx_uj_resident(ctx.ptr, ctypes.byref(P)))
+    bat_s = (time.perf_counter() - t0) / reps
+    ctx.check(L.vpmx_batch_download(ctx.ptr, len(pfs), ptrs, nps, lay))
+    diff = max(normwise(vpm.get_U(pfs[k]), u)[0] for k, u in seq_U.items())
+    log(f"batched C5: one launch {bat_s:.3f} s ({pairs / bat_s:.3e} interactions/s), "
+        f"max U diff vs sequential {diff:.2e}")
+    # batched RK3 steps (3 UJ + fused updates), eager and replayed from a graph
+    S = pfs[0]._step_params()
+    step_ms = {}
+    for graph in (False, True):
+        ctx.set_graph(graph)
+        ctx.check(L.vpmx_batch_upload(ctx.ptr, len(pfs), ptrs, nps, lay))
+        ctx.check(L.vpmx_batch_rk3_resident(ctx.ptr, ctypes.byref(P), ctypes.byref(S), None,
+                                            1e-3, 0))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.check(L.vpmx_batch_rk3_resident(ctx.ptr, ctypes.byref(P), ctypes.byref(S), None,
+                                                1e-3, 0))
+        step_ms["graph" if graph else "eager"] = 1e3 * (time.perf_counter() - t0) / steps
+    ctx.set_graph(False)
+    log("batched RK3 step ms:", step_ms)
+    # launch-bound case: the 8,000-particle ring, FP64 RK3 step, eager vs graph
+    X, G, s, v = workloads.vortex_ring()
+    ring = vpm.ParticleField(len(X), relaxation=vpm.pedrizzetti)
+    vpm.add_particles(ring, X, G, s, vol=v)
+    Pr = ring._uj_params(fmm=False)
+    Sr = ring._step_params()
+    small = {}
+    for graph in (False, True):
+        ctx.set_graph(graph)
+        ctx.check(L.vpmx_upload(ctx.ptr, ring._ptr(), ring.np, ctypes.byref(ring.layout)))
+        for _ in range(3):
+            ctx.check(L.vpmx_rk3_resident(ctx.ptr, ctypes.byref(Pr), ctypes.byref(Sr), 1e-4, 1))
+        t0 = time.perf_counter()
+        for _ in range(20):
+            ctx.check(L.vpmx_rk3_resident(ctx.ptr, ctypes.byref(Pr), ctypes.byref(Sr), 1e-4, 1))
+        small["graph" if graph else "eager"] = 1e3 * (time.perf_counter() - t0) / 20
+    ctx.set_graph(False)
+    log("ring 8k FP64 RK3+relax step ms:", small)
+    names = {N.SCHED_FLAT: "flat", N.SCHED_QUEUE: "queue"}
+    return {"workload": f"{nfields} fields, N_k = floor(exp(U[ln 5e3, ln 2.5e5])) seed {seed}, "
+                        "alternating ring/helix, sigma_k ~ U[0.02, 0.08]; mixed gaussian-erf "
+                        "UJ + stretching",
+            "nfields": nfields, "sum_N": int(sizes.sum()), "sum_N2": pairs,
+            "sequential": {"s": seq_s, "value": pairs / seq_s, "unit": "interactions/s"},
+            "batched": {"s": bat_s, "value": pairs / bat_s, "unit": "interactions/s"},
+            "speedup_batched_vs_sequential": seq_s / bat_s,
+            "max_U_diff_vs_sequential": diff,
+            "schedule": {"kind": names[sched], "block": block, "grid": grid},
+            "autotune_ms": [{"kind": names[a], "block": b, "ms": m} for a, b, m in tune],
+            "rk3_step_ms": step_ms,
+            "rk3_step_interactions_per_s": {k: 3 * pairs / (v / 1e3) for k, v in step_ms.items()},
+            "ring8k_fp64_rk3_relax_step_ms": small}
+
+
 def fmm_section(ctx, n, orders, log, cpu=True, nref=1024):
     """UJ by the FMM on the rotor-wake workload (3 helical tip filaments +
     root line, Gamma = 0.05 t + N(0, 5e-4), sigma = 0.03), ncrit = 50,
@@ -210,6 +307,37 @@ def cpu_baseline(X, G, s, budget_s=12.0):
                       f"(C restatement of UJ_direct), {t:.1f} s"}
 
 
+def traffic_probe(n, copy_bytes=1 << 30):
+    """One launch of the pair kernel on the bench workload and a plain device
+    copy of `copy_bytes`, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs: the
+    copy's counter readings per byte scale the pair kernel's readings to bytes."""
+    import torch
+    from flowunsteady_amd import _native as N
+    from flowunsteady_amd import workloads
+    x = torch.ones(copy_bytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    X, G, s, v = workloads.uniform_cube(n=n, seed=2)
+    ctx = N.Context(0)
+    fld = Field(ctx, X, G, s, v)
+    fld.upload()
+    fld.uj(N.GAUSSIANERF, N.MIXED)
+    fld.substep(N.GAUSSIANERF, N.MIXED, 1e-4)
+    print(json.dumps({"traffic_probe": {"N": n, "copy_bytes": copy_bytes}}), flush=True)
+
+
+def read_traffic(path=os.path.join(ROOT, "profiles", "uj_mixed_cube1m_traffic.md")):
+    """HBM bytes per launch of uj_mixed_kernel<gaussianerf> at N = 1M, from the
+    committed rocprofv3 PMC summary (None when absent)."""
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"HBM bytes per launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,11 +349,18 @@ def main():
     ap.add_argument("--no-fmm", action="store_true", help="skip the FMM section")
     ap.add_argument("--fmm-n", type=int, default=1 << 22)
     ap.add_argument("--fmm-orders", type=str, default="4,8,10")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-fields section")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run one pair-kernel launch and a reference copy (for PMC runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                                    "described in DESIGN.md but not built", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.traffic_probe:
+        traffic_probe(args.n)
+        return
 
     from flowunsteady_amd import _native as N
     from flowunsteady_amd import workloads
@@ -306,7 +441,7 @@ def main():
         "rel_err_mixed_vs_fp64": {k: {"max": e[0], "rms": e[1]} for k, e in err.items()},
         "winckelmans": wk,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": read_traffic() if n == 1 << 20 else None,
                      "kernel": "uj_mixed_kernel<gaussianerf>",
                      "kernel_ms": 1e3 * kernel_s,
                      "flops_per_pair": fl["FLOPS_UJ_PAIR_GAUSSIANERF"]},
@@ -329,6 +464,11 @@ def main():
             out["fmm"] = fmm_section(ctx, args.fmm_n, orders, log, cpu=not args.no_cpu)
         except Exception as exc:   # the direct-UJ line is still reported
             out["fmm"] = {"error": repr(exc)}
+    if not args.no_batch:
+        try:
+            out["batched"] = batched_section(ctx, log)
+        except Exception as exc:
+            out["batched"] = {"error": repr(exc)}
     print(json.dumps(out), flush=True)
 
 

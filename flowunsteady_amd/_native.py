@@ -23,7 +23,9 @@ RELAX_NONE, RELAX_PEDRIZZETTI, RELAX_CORRECTED = 0, 1, 2
 # SFS
 SFS_NONE, SFS_CONSTANT, SFS_DYNAMIC = 0, 1, 2
 # timing classes
-T_UJ, T_ESTR, T_UPDATE, T_FMM = 0, 1, 2, 3
+T_UJ, T_ESTR, T_UPDATE, T_FMM, T_GRAPH = 0, 1, 2, 3, 4
+# schedules of the batched pairwise pass
+SCHED_FLAT, SCHED_QUEUE = 0, 1
 FMM_STATS = ("ms_tree", "ms_traverse", "ms_upload", "cells", "leaves", "m2l_pairs",
              "p2p_leaf_pairs", "p2p_pairs", "ms_eval")
 
@@ -60,6 +62,9 @@ class VPMXError(RuntimeError):
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ctxp = ctypes.c_void_p
+_dpp = ctypes.POINTER(_dp)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
 
 # name -> (restype, argtypes); every function the header declares
 SIGNATURES = {
@@ -97,6 +102,28 @@ SIGNATURES = {
                                             ctypes.c_double, ctypes.c_double,
                                             ctypes.POINTER(ctypes.c_int64)]),
     "vpmx_inject_fault": (ctypes.c_int, [_ctxp, ctypes.c_int32]),
+    "vpmx_batch_upload": (ctypes.c_int, [_ctxp, ctypes.c_int32, _dpp, _i64p,
+                                         ctypes.POINTER(Layout)]),
+    "vpmx_batch_download": (ctypes.c_int, [_ctxp, ctypes.c_int32, _dpp, _i64p,
+                                           ctypes.POINTER(Layout)]),
+    "vpmx_batch_uj": (ctypes.c_int, [_ctxp, ctypes.c_int32, _dpp, _i64p, ctypes.POINTER(Layout),
+                                     ctypes.POINTER(UJParams)]),
+    "vpmx_batch_rk3_step": (ctypes.c_int, [_ctxp, ctypes.c_int32, _dpp, _i64p,
+                                           ctypes.POINTER(Layout), ctypes.POINTER(UJParams),
+                                           ctypes.POINTER(StepParams), _dp, ctypes.c_double,
+                                           ctypes.c_int32]),
+    "vpmx_batch_rk3_resident": (ctypes.c_int, [_ctxp, ctypes.POINTER(UJParams),
+                                               ctypes.POINTER(StepParams), _dp, ctypes.c_double,
+                                               ctypes.c_int32]),
+    "vpmx_batch_nfields": (ctypes.c_int32, [_ctxp]),
+    "vpmx_batch_set_schedule": (ctypes.c_int, [_ctxp, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32]),
+    "vpmx_batch_get_schedule": (ctypes.c_int, [_ctxp, _i32p, _i32p, _i32p]),
+    "vpmx_batch_autotune": (ctypes.c_int, [_ctxp, ctypes.POINTER(UJParams), ctypes.c_int32, _dp,
+                                           ctypes.c_int32, _i32p]),
+    "vpmx_set_graph": (ctypes.c_int, [_ctxp, ctypes.c_int32]),
+    "vpmx_ffv": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64, ctypes.POINTER(Layout), ctypes.c_int32,
+                                _dp]),
     "vpmx_radial_f32": (ctypes.c_int, [ctypes.c_int32, ctypes.c_float, ctypes.c_float,
                                        ctypes.POINTER(ctypes.c_float),
                                        ctypes.POINTER(ctypes.c_float)]),
@@ -166,10 +193,43 @@ class Context:
     def stream(self) -> int:
         return lib().vpmx_stream(self.ptr)
 
+    def set_graph(self, on: bool):
+        self.check(lib().vpmx_set_graph(self.ptr, int(on)))
+
+    def set_schedule(self, schedule: int, block: int, grid: int = 0):
+        self.check(lib().vpmx_batch_set_schedule(self.ptr, schedule, block, grid))
+
+    def schedule(self) -> tuple:
+        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self.check(lib().vpmx_batch_get_schedule(self.ptr, ctypes.byref(a), ctypes.byref(b),
+                                                 ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def autotune(self, params, reps: int = 1) -> list:
+        """Times every candidate schedule of the resident batch; keeps the fastest.
+        Returns [(schedule, block, ms), ...] in candidate order."""
+        ms = (ctypes.c_double * 8)()
+        n = ctypes.c_int32()
+        self.check(lib().vpmx_batch_autotune(self.ptr, ctypes.byref(params), int(reps), ms, 8,
+                                             ctypes.byref(n)))
+        if params.precision == MIXED:
+            cands = [(s, b) for b in (64, 128, 256) for s in (SCHED_FLAT, SCHED_QUEUE)]
+        else:
+            cands = [(SCHED_FLAT, b) for b in (64, 128, 256)]
+        return [(s, b, ms[i]) for i, (s, b) in enumerate(cands[:n.value])]
+
     def fmm_stats(self) -> dict:
         out = (ctypes.c_double * len(FMM_STATS))()
         self.check(lib().vpmx_fmm_stats(self.ptr, out, len(FMM_STATS)))
         return dict(zip(FMM_STATS, list(out)))
+
+
+def pointer_array(mats) -> "ctypes.Array":
+    """double*[] of the column-major float64 matrices in `mats` (kept alive by the caller)."""
+    arr = (_dp * len(mats))()
+    for k, m in enumerate(mats):
+        arr[k] = m.ctypes.data_as(_dp)
+    return arr
 
 
 def fmm_plan_counts(X, sigma, ncrit=50, theta=0.4, sigma_cut=7.0) -> dict:

@@ -320,4 +320,30 @@ __device__ __forceinline__ void mixed_pair_d(float dx, float dy, float dz, float
     acc.J7 = fmaf(c1, ez, acc.J7);
 }
 
+// 30-bit Morton key of a point in the cube of side max(extent) anchored at the
+// box's low corner (box = lo xyz, hi xyz): 10 bits per axis, x in bit 0.
+__device__ __forceinline__ uint32_t spread10(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t morton30(const double *box, double x, double y, double z)
+{
+    const double ext = fmax(fmax(box[3] - box[0], box[4] - box[1]), box[5] - box[2]);
+    const double scale = ext > 0.0 ? 1023.999 / ext : 0.0;
+    const double p[3] = {x, y, z};
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        double v = (p[k] - box[k]) * scale;
+        v = fmin(fmax(v, 0.0), 1023.0);
+        q[k] = (uint32_t)v;
+    }
+    return spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+}
+
 }  // namespace vpmx

@@ -50,6 +50,11 @@ struct FmmTables {
     std::vector<int> mi, rec1, rec2, d1, d2a, d2b;
     std::vector<int> m2m_off, m2m_idx, l2l_off, l2l_idx, m2l_off, m2l_idx;
     std::vector<double> m2m_coef, l2l_coef, m2l_coef;
+    // M2L work split: chunks (b, first term, end term, 0) of at most m2l_chunk_len
+    // terms of one output coefficient b, in b order; m2l_cfirst[b] = first chunk
+    // of b; m2l_rounds = chunks per thread of a 256-thread block (1, 2, 4 or 8).
+    std::vector<int> m2l_chunk, m2l_cfirst;
+    int m2l_nchunk = 0, m2l_rounds = 1, m2l_chunk_len = 0;
 };
 
 // Tree + lists of one evaluation (host).
@@ -79,6 +84,9 @@ struct FmmDev {
     const int *m2m_off = nullptr, *l2l_off = nullptr, *m2l_toff = nullptr;
     const int2 *m2m_idx = nullptr, *l2l_idx = nullptr, *m2l_tidx = nullptr;
     const double *m2m_coef = nullptr, *l2l_coef = nullptr, *m2l_tcoef = nullptr;
+    const int4 *m2l_chunks = nullptr;
+    const int *m2l_cfirst = nullptr;
+    int m2l_nchunk = 0, m2l_rounds = 1;
 };
 
 // Per-context FMM state: host plan, operator tables and device buffers.
@@ -91,6 +99,8 @@ struct FmmState {
     int4 *rec1 = nullptr, *rec2 = nullptr, *d1 = nullptr, *d2a = nullptr, *d2b = nullptr;
     int2 *m2m_idx = nullptr, *l2l_idx = nullptr, *m2l_tidx = nullptr;
     double *m2m_coef = nullptr, *l2l_coef = nullptr, *m2l_tcoef = nullptr;
+    int4 *m2l_chunks = nullptr;
+    int *m2l_cfirst = nullptr;
     // device tree and lists
     int64_t cap_cells = 0, cap_m2l = 0, cap_p2p = 0, cap_coef = 0, cap_n = 0;
     FmmCellDev *cells = nullptr;

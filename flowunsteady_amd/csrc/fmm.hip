@@ -242,74 +242,126 @@ __global__ __launch_bounds__(FMM_BLOCK) void m2m_kernel(const int32_t *__restric
     }
 }
 
-// M2L: one block per target cell; l_b = sum_sources sum_a coef m_a b_{a+b}.
-// Writes (overwrites) the cell's local expansion.
+// M2L: one block per target cell, l_b = sum_sources sum_a coef(a, b) m_a D_{a+b},
+// D_k the derivatives of 1/|r| at the centre separation.  Sources are taken
+// M2L_BATCH at a time: wave w builds the derivative tensor of source w degree
+// by degree (the recurrence of fmm_build_tables' rec1/rec2 tables) while the
+// block stages the batch's multipoles; then each thread contracts its R chunks
+// of the (a, a+b) term list (chunks of one b, evenly sized) against every source
+// of the batch.  At the end the chunk sums of each b are added in chunk order:
+// the summation order is fixed, so results are bitwise repeatable.  Writes
+// (overwrites) the cell's local expansion.
+constexpr int M2L_BATCH = FMM_BLOCK / 64;
+
+template <int R>
 __global__ __launch_bounds__(FMM_BLOCK) void m2l_kernel(const FmmCellDev *__restrict__ cells,
                                                         const int64_t *__restrict__ off,
                                                         const int32_t *__restrict__ src, int K,
-                                                        int P, const int *__restrict__ mi,
-                                                        const int4 *__restrict__ rec1,
+                                                        int P, const int4 *__restrict__ rec1,
                                                         const int4 *__restrict__ rec2,
-                                                        const int *__restrict__ toff,
+                                                        const int4 *__restrict__ chunks,
+                                                        int nchunk,
+                                                        const int *__restrict__ cfirst,
                                                         const int2 *__restrict__ tidx,
                                                         const double *__restrict__ tcoef,
                                                         const double *__restrict__ M,
                                                         double *__restrict__ L)
 {
     extern __shared__ double sh[];
-    double *bb = sh;            // K
-    double *ms = sh + K;        // 3K
+    double *bb = sh;                       // M2L_BATCH x K derivatives
+    double *ms = sh + M2L_BATCH * K;       // M2L_BATCH x 3K multipoles
     const int c = blockIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const FmmCellDev A = cells[c];
-    double acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    for (int64_t e = off[c]; e < off[c + 1]; e++) {
-        const int s = src[e];
-        const FmmCellDev B = cells[s];
-        const double r0 = A.cx - B.cx, r1 = A.cy - B.cy, r2 = A.cz - B.cz;
-        const double R2 = r0 * r0 + r1 * r1 + r2 * r2;
+    const int64_t e0 = off[c], e1 = off[c + 1];
+    double acc[R][3];
+    for (int r = 0; r < R; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
+    for (int64_t eb = e0; eb < e1; eb += M2L_BATCH) {
+        const int nb = (int)min<int64_t>(M2L_BATCH, e1 - eb);
         __syncthreads();
-        for (int k = threadIdx.x; k < 3 * K; k += blockDim.x) ms[k] = M[(size_t)s * 3 * K + k];
-        if (threadIdx.x == 0) bb[0] = 1.0 / sqrt(R2);
+        for (int k = threadIdx.x; k < M2L_BATCH * 3 * K; k += FMM_BLOCK) {
+            const int q = k / (3 * K);
+            ms[k] = q < nb ? M[(size_t)src[eb + q] * 3 * K + (k - q * 3 * K)] : 0.0;
+        }
+        double r0 = 0.0, r1 = 0.0, r2 = 0.0, R2 = 1.0;
+        double *bw = bb + w * K;
+        if (w < nb) {
+            const FmmCellDev B = cells[src[eb + w]];
+            r0 = A.cx - B.cx; r1 = A.cy - B.cy; r2 = A.cz - B.cz;
+            R2 = r0 * r0 + r1 * r1 + r2 * r2;
+            if (lane == 0) bw[0] = 1.0 / sqrt(R2);
+        } else {
+            for (int k = lane; k < K; k += 64) bw[k] = 0.0;
+        }
         __syncthreads();
-        // derivatives of 1/r, one degree at a time
         int kbeg = 1;
         for (int n = 1; n <= P; n++) {
             const int kend = (n + 1) * (n + 2) * (n + 3) / 6;
-            for (int k = kbeg + threadIdx.x; k < kend; k += blockDim.x) {
-                const int4 j1 = rec1[k];   // idx(k - e_i) for i = 0..2, -1 if none
-                const int4 j2 = rec2[k];   // idx(k - 2 e_i)
-                double s1 = 0.0, s2 = 0.0;
-                if (j1.x >= 0) s1 += r0 * bb[j1.x];
-                if (j1.y >= 0) s1 += r1 * bb[j1.y];
-                if (j1.z >= 0) s1 += r2 * bb[j1.z];
-                if (j2.x >= 0) s2 += bb[j2.x];
-                if (j2.y >= 0) s2 += bb[j2.y];
-                if (j2.z >= 0) s2 += bb[j2.z];
-                bb[k] = -((2.0 * n - 1.0) * s1 + (n - 1.0) * s2) / (n * R2);
+            if (w < nb) {
+                for (int k = kbeg + lane; k < kend; k += 64) {
+                    const int4 j1 = rec1[k];   // idx(k - e_i), -1 if none
+                    const int4 j2 = rec2[k];   // idx(k - 2 e_i)
+                    double s1 = 0.0, s2 = 0.0;
+                    if (j1.x >= 0) s1 += r0 * bw[j1.x];
+                    if (j1.y >= 0) s1 += r1 * bw[j1.y];
+                    if (j1.z >= 0) s1 += r2 * bw[j1.z];
+                    if (j2.x >= 0) s2 += bw[j2.x];
+                    if (j2.y >= 0) s2 += bw[j2.y];
+                    if (j2.z >= 0) s2 += bw[j2.z];
+                    bw[k] = -((2.0 * n - 1.0) * s1 + (n - 1.0) * s2) / (n * R2);
+                }
             }
             kbeg = kend;
             __syncthreads();
         }
-        for (int r = 0; r < 2; r++) {
-            const int k = threadIdx.x + r * FMM_BLOCK;
-            if (k >= K) continue;
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-            for (int t = toff[k]; t < toff[k + 1]; t++) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int ch = threadIdx.x + r * FMM_BLOCK;
+            if (ch >= nchunk) continue;
+            const int4 C = chunks[ch];   // (b, first term, end term, -)
+            double s[M2L_BATCH][3];
+#pragma unroll
+            for (int q = 0; q < M2L_BATCH; q++) s[q][0] = s[q][1] = s[q][2] = 0.0;
+            for (int t = C.y; t < C.z; t++) {
                 const int2 ix = tidx[t];   // (a, a+b)
-                const double w = tcoef[t] * bb[ix.y];
-                s0 += w * ms[ix.x];
-                s1 += w * ms[K + ix.x];
-                s2 += w * ms[2 * K + ix.x];
+                const double cf = tcoef[t];
+#pragma unroll
+                for (int q = 0; q < M2L_BATCH; q++) {
+                    const double wq = cf * bb[q * K + ix.y];
+                    const double *mq = ms + q * 3 * K;
+                    s[q][0] += wq * mq[ix.x];
+                    s[q][1] += wq * mq[K + ix.x];
+                    s[q][2] += wq * mq[2 * K + ix.x];
+                }
             }
-            acc[r][0] += s0;
-            acc[r][1] += s1;
-            acc[r][2] += s2;
+#pragma unroll
+            for (int q = 0; q < M2L_BATCH; q++) {
+                acc[r][0] += s[q][0];
+                acc[r][1] += s[q][1];
+                acc[r][2] += s[q][2];
+            }
         }
     }
-    for (int r = 0; r < 2; r++) {
-        const int k = threadIdx.x + r * FMM_BLOCK;
-        if (k >= K) continue;
-        for (int q = 0; q < 3; q++) L[((size_t)c * 3 + q) * K + k] = acc[r][q];
+    // chunk sums -> l_b in chunk order
+    __syncthreads();
+    double *part = sh;   // nchunk x 3
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int ch = threadIdx.x + r * FMM_BLOCK;
+        if (ch < nchunk)
+            for (int q = 0; q < 3; q++) part[3 * ch + q] = acc[r][q];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += FMM_BLOCK) {
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        for (int ch = cfirst[k]; ch < cfirst[k + 1]; ch++) {
+            v0 += part[3 * ch];
+            v1 += part[3 * ch + 1];
+            v2 += part[3 * ch + 2];
+        }
+        L[((size_t)c * 3 + 0) * K + k] = v0;
+        L[((size_t)c * 3 + 1) * K + k] = v1;
+        L[((size_t)c * 3 + 2) * K + k] = v2;
     }
 }
 
@@ -512,7 +564,32 @@ __global__ __launch_bounds__(P2P_BLOCK) void p2p_f64_kernel(const int32_t *__res
 }
 
 // P2P near field, mixed: FP32 pair math on coordinates relative to the
-// target leaf's centre, FP32 partials per source leaf folded into FP64.
+// target leaf's centre, FP32 partials per source tile folded into FP64.
+//
+// One wave per target leaf.  The particles of the leaf's source leaves are
+// staged through LDS as one concatenated stream, 64 records per tile whatever
+// the leaf boundaries (a prefix of the source-leaf sizes, 64 leaves at a time,
+// maps a stream position to its particle).  A leaf of nt <= 32 targets is
+// replicated R = 2, 4 or 8 times across the wave: lane = s*nt + t works on
+// target t with the sources s, s+R, s+2R, ... of every tile, and the R partial
+// sums of a target are added in slice order at the end (deterministic).
+constexpr int P2P_SLOTS = 64;
+
+template <int KK, int R>
+__device__ __forceinline__ void p2p_mixed_tile(const float4 *tpos, const float4 *tgam, int cnt,
+                                               int s, float xi, float yi, float zi, MixedAcc &acc)
+{
+    if (cnt == P2P_BLOCK) {
+#pragma unroll 4
+        for (int i = 0; i < P2P_BLOCK / R; ++i) {
+            const int j = i * R + s;
+            mixed_pair<KK>(tpos[j], tgam[j], xi, yi, zi, acc);
+        }
+    } else {
+        for (int j = s; j < cnt; j += R) mixed_pair<KK>(tpos[j], tgam[j], xi, yi, zi, acc);
+    }
+}
+
 template <int KK>
 __global__ __launch_bounds__(P2P_BLOCK) void p2p_mixed_kernel(const int32_t *__restrict__ leaves,
                                                               const FmmCellDev *__restrict__ cells,
@@ -525,24 +602,51 @@ __global__ __launch_bounds__(P2P_BLOCK) void p2p_mixed_kernel(const int32_t *__r
 {
     __shared__ float4 tpos[P2P_BLOCK];
     __shared__ float4 tgam[P2P_BLOCK];
+    __shared__ int32_t lbeg[P2P_SLOTS], lpre[P2P_SLOTS + 1];
+    __shared__ double red[14][P2P_BLOCK];
+    const int lane = threadIdx.x;
     const int c = leaves[blockIdx.x];
     const FmmCellDev A = cells[c];
+    const int64_t e0 = off[c], e1 = off[c + 1];
     for (int tb = A.begin; tb < A.end; tb += P2P_BLOCK) {
-        const int it = tb + threadIdx.x;
-        const bool active = it < A.end;
+        const int nt = min(P2P_BLOCK, A.end - tb);
+        const int R = nt <= 8 ? 8 : nt <= 16 ? 4 : nt <= 32 ? 2 : 1;
+        const int sl = lane / nt;                  // slice of this lane
+        const int t = lane - sl * nt;              // target of this lane
+        const bool active = sl < R;
         float xi = 0.f, yi = 0.f, zi = 0.f;
-        if (active) {
-            const double4 a = rec[2 * it];
+        {
+            const double4 a = rec[2 * (tb + t)];
             xi = (float)(a.x - A.cx); yi = (float)(a.y - A.cy); zi = (float)(a.z - A.cz);
         }
+        const int s = active ? sl : 0;
         double dU0 = 0, dU1 = 0, dU2 = 0, dW0 = 0, dW1 = 0, dW2 = 0;
         double dJ0 = 0, dJ1 = 0, dJ2 = 0, dJ3 = 0, dJ4 = 0, dJ5 = 0, dJ6 = 0, dJ7 = 0;
-        for (int64_t e = off[c]; e < off[c + 1]; e++) {
-            const FmmCellDev B = cells[src[e]];
-            for (int sb = B.begin; sb < B.end; sb += P2P_BLOCK) {
+        for (int64_t eb = e0; eb < e1; eb += P2P_SLOTS) {
+            const int nl = (int)min<int64_t>(P2P_SLOTS, e1 - eb);
+            __syncthreads();
+            if (lane < nl) {
+                const FmmCellDev B = cells[src[eb + lane]];
+                lbeg[lane] = B.begin;
+                lpre[lane + 1] = B.end - B.begin;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                lpre[0] = 0;
+                for (int k = 1; k <= nl; ++k) lpre[k] += lpre[k - 1];
+            }
+            __syncthreads();
+            const int total = lpre[nl];
+            for (int base = 0; base < total; base += P2P_BLOCK) {
+                const int q = base + lane;
                 __syncthreads();
-                const int js = sb + threadIdx.x;
-                if (js < B.end) {
+                if (q < total) {
+                    int lo = 0, hi = nl;          // lpre[lo] <= q < lpre[hi]
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (lpre[mid] <= q) lo = mid; else hi = mid;
+                    }
+                    const int js = lbeg[lo] + (q - lpre[lo]);
                     const double4 a = rec[2 * js], b = rec[2 * js + 1];
                     float p0 = 0.f, p1 = 0.f;
                     if constexpr (KK == GAUSSIANERF || KK == GAUSSIAN) {
@@ -551,22 +655,44 @@ __global__ __launch_bounds__(P2P_BLOCK) void p2p_mixed_kernel(const int32_t *__r
                     } else if constexpr (KK == WINCKELMANS) {
                         p0 = (float)(b.z * b.z);
                     }
-                    tpos[threadIdx.x] = make_float4((float)(a.x - A.cx), (float)(a.y - A.cy),
-                                                    (float)(a.z - A.cz), p0);
-                    tgam[threadIdx.x] = make_float4((float)a.w, (float)b.x, (float)b.y, p1);
+                    tpos[lane] = make_float4((float)(a.x - A.cx), (float)(a.y - A.cy),
+                                             (float)(a.z - A.cz), p0);
+                    tgam[lane] = make_float4((float)a.w, (float)b.x, (float)b.y, p1);
                 }
                 __syncthreads();
-                const int cnt = min(P2P_BLOCK, B.end - sb);
+                const int cnt = min(P2P_BLOCK, total - base);
                 MixedAcc acc;
-                for (int jj = 0; jj < cnt; ++jj) mixed_pair<KK>(tpos[jj], tgam[jj], xi, yi, zi, acc);
+                switch (R) {
+                case 8: p2p_mixed_tile<KK, 8>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
+                case 4: p2p_mixed_tile<KK, 4>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
+                case 2: p2p_mixed_tile<KK, 2>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
+                default: p2p_mixed_tile<KK, 1>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
+                }
                 dU0 += acc.U0; dU1 += acc.U1; dU2 += acc.U2;
                 dW0 += acc.W0; dW1 += acc.W1; dW2 += acc.W2;
                 dJ0 += acc.J0; dJ1 += acc.J1; dJ2 += acc.J2; dJ3 += acc.J3;
                 dJ4 += acc.J4; dJ5 += acc.J5; dJ6 += acc.J6; dJ7 += acc.J7;
             }
         }
-        if (active) {
-            const int64_t i = perm[it];
+        if (R > 1) {   // add the slices of each target in slice order
+            const double v[14] = {dU0, dU1, dU2, dW0, dW1, dW2, dJ0, dJ1, dJ2, dJ3, dJ4, dJ5, dJ6, dJ7};
+            __syncthreads();
+            for (int k = 0; k < 14; ++k) red[k][lane] = v[k];
+            __syncthreads();
+            if (lane < nt) {
+                double w[14];
+                for (int k = 0; k < 14; ++k) {
+                    double acc2 = red[k][lane];
+                    for (int q = 1; q < R; ++q) acc2 += red[k][lane + q * nt];
+                    w[k] = acc2;
+                }
+                dU0 = w[0]; dU1 = w[1]; dU2 = w[2]; dW0 = w[3]; dW1 = w[4]; dW2 = w[5];
+                dJ0 = w[6]; dJ1 = w[7]; dJ2 = w[8]; dJ3 = w[9]; dJ4 = w[10]; dJ5 = w[11];
+                dJ6 = w[12]; dJ7 = w[13];
+            }
+        }
+        if (lane < nt) {
+            const int64_t i = perm[tb + lane];
             const double k = -INV4PI;
             U[i] += k * dU0; U[cap + i] += k * dU1; U[2 * cap + i] += k * dU2;
             J[i] += k * dJ0;
@@ -775,6 +901,40 @@ void fmm_build_tables(FmmTables &T, int P)
                                  binom(a[1] + b[1], a[1]) * binom(a[2] + b[2], a[2]));
         }
         T.m2l_off[kb + 1] = (int)T.m2l_coef.size();
+    }
+    // M2L chunks: the length that minimises the per-thread work (rounds x length)
+    {
+        auto count = [&](int len) {
+            int n = 0;
+            for (int kb = 0; kb < K; kb++) n += (T.m2l_off[kb + 1] - T.m2l_off[kb] + len - 1) / len;
+            return n;
+        };
+        int best_len = 0, best_rounds = 0;
+        for (int len = 4; len <= 4096; len++) {
+            const int n = count(len);
+            int r = 1;
+            while (r * FMM_BLOCK < n) r *= 2;
+            if (r > 8) continue;
+            if (!best_len || r * len < best_rounds * best_len) {
+                best_len = len;
+                best_rounds = r;
+            }
+        }
+        T.m2l_chunk.clear();
+        T.m2l_cfirst.assign(K + 1, 0);
+        for (int kb = 0; kb < K; kb++) {
+            T.m2l_cfirst[kb] = (int)T.m2l_chunk.size() / 4;
+            for (int t = T.m2l_off[kb]; t < T.m2l_off[kb + 1]; t += best_len) {
+                T.m2l_chunk.push_back(kb);
+                T.m2l_chunk.push_back(t);
+                T.m2l_chunk.push_back(std::min(t + best_len, T.m2l_off[kb + 1]));
+                T.m2l_chunk.push_back(0);
+            }
+        }
+        T.m2l_nchunk = (int)T.m2l_chunk.size() / 4;
+        T.m2l_cfirst[K] = T.m2l_nchunk;
+        T.m2l_rounds = best_rounds;
+        T.m2l_chunk_len = best_len;
     }
     // L2P: idx(b + e_j) and idx(b + e_j + e_l)
     T.d1.assign(4 * K, -1);
@@ -1304,9 +1464,21 @@ hipError_t launch_fmm_eval(hipStream_t st, const FmmDev &d, const FmmPlan &pl, i
         if (e != hipSuccess) return e;
     }
     if (d.ncell > 0) {
-        m2l_kernel<<<d.ncell, FMM_BLOCK, shK, st>>>(d.cells, d.m2l_off, d.m2l_src, K, P, d.mi,
-                                                   d.rec1, d.rec2, d.m2l_toff, d.m2l_tidx,
-                                                   d.m2l_tcoef, d.M, d.L);
+        // derivatives + multipoles of a batch, or the chunk sums at the end
+        const size_t shm = sizeof(double) * std::max<size_t>((size_t)M2L_BATCH * 4 * K,
+                                                             (size_t)3 * d.m2l_nchunk);
+#define VPMX_M2L(RR)                                                                           \
+    m2l_kernel<RR><<<d.ncell, FMM_BLOCK, shm, st>>>(d.cells, d.m2l_off, d.m2l_src, K, P, d.rec1, \
+                                                    d.rec2, d.m2l_chunks, d.m2l_nchunk,         \
+                                                    d.m2l_cfirst, d.m2l_tidx, d.m2l_tcoef, d.M, d.L)
+        switch (d.m2l_rounds) {
+        case 1: VPMX_M2L(1); break;
+        case 2: VPMX_M2L(2); break;
+        case 4: VPMX_M2L(4); break;
+        case 8: VPMX_M2L(8); break;
+        default: return hipErrorInvalidValue;
+        }
+#undef VPMX_M2L
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

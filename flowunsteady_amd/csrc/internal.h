@@ -12,12 +12,12 @@
 namespace vpmx {
 
 // Timed kernel classes (vpmx_get_timing's `which`).
-enum TimedKind { T_UJ = 0, T_ESTR = 1, T_UPDATE = 2, T_FMM = 3, T_NKIND = 4 };
+enum TimedKind { T_UJ = 0, T_ESTR = 1, T_UPDATE = 2, T_FMM = 3, T_GRAPH = 4, T_NKIND = 5 };
 
 struct Timer {
     bool on = false;
-    double total_ms[T_NKIND] = {0, 0, 0, 0};
-    int64_t count[T_NKIND] = {0, 0, 0, 0};
+    double total_ms[T_NKIND] = {0, 0, 0, 0, 0};
+    int64_t count[T_NKIND] = {0, 0, 0, 0, 0};
     struct Pending { int kind; hipEvent_t a, b; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
@@ -53,6 +53,49 @@ struct Scratch {
     double *tX = nullptr, *tU = nullptr, *tJ = nullptr;   // SoA, stride = number of targets
     double4 *tpos = nullptr;         // Morton-sorted targets (x, y, z, 0)
     int32_t *tperm = nullptr;        // sorted slot -> target index
+    // free-vortex force: bound vortex records and per-segment partials
+    int64_t bcap = 0, fpcap = 0;
+    double *bnd = nullptr, *fpart = nullptr, *fout = nullptr;
+};
+
+// One workgroup's work in the batched UJ: targets [t0, t1) against sources
+// [s0, s1), all in slots of the concatenated fields.
+struct BatchItem { int32_t t0, t1, s0, s1; };
+
+// Schedules of the batched UJ.
+enum BatchSchedule { SCHED_FLAT = 0, SCHED_QUEUE = 1 };
+
+// Many independent fields resident as one concatenation: field k owns slots
+// [off[k], off[k+1]) of the DeviceField and of every scratch array.
+struct BatchState {
+    int32_t nfields = 0;             // 0: the resident state is a single field
+    std::vector<int64_t> off;        // nfields + 1
+    int64_t fcap = 0;                // field capacity of box / uinf / doff
+    int64_t *doff = nullptr;         // device copy of off
+    double *box = nullptr;           // per field bounding box (lo xyz, hi xyz)
+    double *uinf = nullptr;          // per field freestream (3)
+    int64_t pcap = 0;                // particle capacity of fid / keys
+    int32_t *fid = nullptr;          // per particle field index
+    uint64_t *kin = nullptr, *kout = nullptr;   // 64-bit sort keys
+    uint8_t *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    // schedule of the pairwise pass and its work list
+    int sched = SCHED_QUEUE, block = 256, grid = 0;   // grid 0: resident occupancy x CUs
+    int grid_eff = 0;                // grid of the last queue launch setup
+    bool tuned = false;
+    int items_block = 0;             // block size the work list was built for
+    std::vector<int64_t> items_off;  // offsets the work list was built for
+    int64_t icap = 0, nitems = 0;
+    BatchItem *items = nullptr;
+    int32_t *counter = nullptr;      // work-queue head
+};
+
+// Instantiated hipGraph of one time step on the resident state and the
+// settings it was captured with.
+struct GraphState {
+    bool on = false;
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned char> key;
 };
 
 struct FmmState;   // fmm.h
@@ -68,8 +111,15 @@ struct vpmx_ctx {
     vpmx::DeviceField f;
     vpmx::Scratch s;
     vpmx::Timer timer;
+    vpmx::BatchState b;
+    vpmx::GraphState g;
+    bool capturing = false;          // enqueueing into a graph: no events, no host syncs
+    uint64_t gen = 0;                // bumped whenever a device buffer is reallocated
+    int ncu = 0;                     // compute units of the device
     double *h_stage = nullptr;       // pinned host staging
     size_t h_stage_bytes = 0;
+    double *h_small = nullptr;       // pinned staging of per-step scalars (freestreams)
+    size_t h_small_bytes = 0;
 };
 
 namespace vpmx {
@@ -112,6 +162,36 @@ hipError_t launch_relax(hipStream_t st, const DeviceField &f, int kind, double r
 hipError_t launch_diagnostics(hipStream_t st, const DeviceField &f, Scratch &s, double *out2);
 size_t sort_temp_bytes(int64_t n);
 hipError_t launch_bbox(hipStream_t st, const DeviceField &f, double *part, int *nparts);
+// Mixed path split: Morton sort (perm), then packing of the sorted records.
+hipError_t launch_sort_mixed(hipStream_t st, const DeviceField &f, Scratch &s);
+hipError_t launch_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, Scratch &s);
+// Freestream per particle from a device table: uinf[3 fid[i] + k] (fid NULL:
+// every particle reads uinf[0..2]).  Replaces the Uinf argument of launch_update.
+hipError_t launch_update_dev(hipStream_t st, const DeviceField &f, int transposed, int sfs,
+                             double fcoef, double gcoef, double zeta0, const double *uinf,
+                             const int32_t *fid, double a, double b, double dt);
+
+// Free-vortex force (force.hip): segments of the free field per bound vortex,
+// then out[6 b + k] for nb bound vortices (bnd: 8 doubles each).
+int ffv_segments(int64_t nb, int64_t nf);
+hipError_t launch_ffv(hipStream_t st, int kernel, const double *bnd, int nb, const DeviceField &f,
+                      int nseg, double *part, double *out);
+
+// Batched fields (batch.hip).
+size_t batch_sort_temp_bytes(int64_t n);
+// per-field bbox, 64-bit keys (field << 30 | Morton), stable radix sort -> s.perm
+hipError_t launch_batch_sort(hipStream_t st, const DeviceField &f, Scratch &s,
+                             const BatchState &b);
+// fills fid from the offsets
+hipError_t launch_batch_fid(hipStream_t st, const BatchState &b, int64_t n);
+// pairwise pass over the work list with the batch's schedule
+hipError_t launch_batch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f,
+                                 const Scratch &s, BatchState &b, int reset, int sched,
+                                 int block, int grid);
+hipError_t launch_batch_uj_f64(hipStream_t st, int kernel, const DeviceField &f,
+                               const Scratch &s, BatchState &b, int reset);
+// resident workgroups per CU of the queue kernel for a block size
+int batch_queue_occupancy(int kernel, int block);
 
 double zeta0_of(int kernel);
 

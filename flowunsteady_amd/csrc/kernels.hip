@@ -16,6 +16,7 @@
 
 #include "device_math.h"
 #include "internal.h"
+#include "uj_block.h"
 
 namespace vpmx {
 
@@ -55,69 +56,8 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_f64_kernel(const double4 *__restr
                                                           double *__restrict__ J, int reset)
 {
     __shared__ double4 tile[2 * UJ_BLOCK];
-    const int t = threadIdx.x;
-    const int64_t i = (int64_t)blockIdx.x * UJ_BLOCK + t;
-    const bool active = i < nt;
-    double xi = 0.0, yi = 0.0, zi = 0.0;
-    if (active) {
-        xi = X[i];
-        yi = X[cap + i];
-        zi = X[2 * cap + i];
-    }
-    double u0 = 0, u1 = 0, u2 = 0;
-    double j0 = 0, j1 = 0, j2 = 0, j3 = 0, j4 = 0, j5 = 0, j6 = 0, j7 = 0, j8 = 0;
-
-    for (int64_t base = 0; base < ns; base += UJ_BLOCK) {
-        __syncthreads();
-        const int64_t js = base + t;
-        if (js < ns) {
-            tile[2 * t] = src[2 * js];
-            tile[2 * t + 1] = src[2 * js + 1];
-        }
-        __syncthreads();
-        const int cnt = (int)((ns - base) < UJ_BLOCK ? (ns - base) : UJ_BLOCK);
-        for (int jj = 0; jj < cnt; ++jj) {
-            const double4 a = tile[2 * jj], b = tile[2 * jj + 1];
-            const double dX1 = xi - a.x, dX2 = yi - a.y, dX3 = zi - a.z;
-            const double r2 = dX1 * dX1 + dX2 * dX2 + dX3 * dX3;
-            if (r2 > 0.0) {   // the reference skips r == 0
-                const double r = sqrt(r2);
-                double g, dg;
-                g_dg_f64<K>(r * b.w, g, dg);
-                const double rinv = 1.0 / r;
-                const double r3i = rinv * rinv * rinv;
-                const double G1 = a.w, G2 = b.x, G3 = b.y;
-                const double c1 = -INV4PI * r3i * (dX2 * G3 - dX3 * G2);
-                const double c2 = -INV4PI * r3i * (dX3 * G1 - dX1 * G3);
-                const double c3 = -INV4PI * r3i * (dX1 * G2 - dX2 * G1);
-                u0 += g * c1;
-                u1 += g * c2;
-                u2 += g * c3;
-                const double aux = dg * b.w * rinv - 3.0 * g * rinv * rinv;
-                j0 += aux * c1 * dX1; j1 += aux * c2 * dX1; j2 += aux * c3 * dX1;
-                j3 += aux * c1 * dX2; j4 += aux * c2 * dX2; j5 += aux * c3 * dX2;
-                j6 += aux * c1 * dX3; j7 += aux * c2 * dX3; j8 += aux * c3 * dX3;
-                const double a2 = -INV4PI * g * r3i;
-                j1 -= a2 * G3;
-                j2 += a2 * G2;
-                j3 += a2 * G3;
-                j5 -= a2 * G1;
-                j6 -= a2 * G2;
-                j7 += a2 * G1;
-            }
-        }
-    }
-    if (!active) return;
-    if (!reset) {
-        u0 += U[i]; u1 += U[cap + i]; u2 += U[2 * cap + i];
-        j0 += J[i]; j1 += J[cap + i]; j2 += J[2 * cap + i];
-        j3 += J[3 * cap + i]; j4 += J[4 * cap + i]; j5 += J[5 * cap + i];
-        j6 += J[6 * cap + i]; j7 += J[7 * cap + i]; j8 += J[8 * cap + i];
-    }
-    U[i] = u0; U[cap + i] = u1; U[2 * cap + i] = u2;
-    J[i] = j0; J[cap + i] = j1; J[2 * cap + i] = j2;
-    J[3 * cap + i] = j3; J[4 * cap + i] = j4; J[5 * cap + i] = j5;
-    J[6 * cap + i] = j6; J[7 * cap + i] = j7; J[8 * cap + i] = j8;
+    const int64_t t0 = (int64_t)blockIdx.x * UJ_BLOCK;
+    uj_f64_block<K, UJ_BLOCK>(src, 0, ns, X, t0, nt, cap, U, J, reset, tile);
 }
 
 hipError_t launch_pack_f64(hipStream_t st, const DeviceField &f, Scratch &s)
@@ -212,16 +152,6 @@ __global__ __launch_bounds__(EW_BLOCK) void bbox_partial_kernel(const double *__
     if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__device__ __forceinline__ uint32_t spread10(uint32_t v)
-{
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000ffu;
-    v = (v | (v << 8)) & 0x0300f00fu;
-    v = (v | (v << 4)) & 0x030c30c3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
 // Reduces the bbox partials (every block redoes the same fixed-order reduction
 // of `nparts` partials, which is cheap) and writes 30-bit Morton keys.
 __global__ __launch_bounds__(EW_BLOCK) void morton_kernel(const double *__restrict__ X,
@@ -241,15 +171,7 @@ __global__ __launch_bounds__(EW_BLOCK) void morton_kernel(const double *__restri
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
-    double ext = fmax(fmax(box[3] - box[0], box[4] - box[1]), box[5] - box[2]);
-    const double scale = ext > 0.0 ? 1023.999 / ext : 0.0;
-    uint32_t q[3];
-    for (int k = 0; k < 3; ++k) {
-        double v = (X[k * cap + i] - box[k]) * scale;
-        v = fmin(fmax(v, 0.0), 1023.0);
-        q[k] = (uint32_t)v;
-    }
-    key[i] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+    key[i] = morton30(box, X[i], X[cap + i], X[2 * cap + i]);
     val[i] = (int32_t)i;
 }
 
@@ -291,65 +213,8 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_kernel(const double4 *__res
 {
     __shared__ float4 tpos[UJ_BLOCK];
     __shared__ float4 tgam[UJ_BLOCK];
-    const int t = threadIdx.x;
-    const int64_t s0 = (int64_t)blockIdx.x * UJ_BLOCK;
-    const int64_t s = s0 + t;
-    const bool active = s < nt;
-    // Block origin: the block's first (Morton-sorted) target, in FP64.
-    const double4 o = tgt[s0];
-    float xi = 0.f, yi = 0.f, zi = 0.f;
-    if (active) {
-        const double4 p = tgt[s];
-        xi = (float)(p.x - o.x);
-        yi = (float)(p.y - o.y);
-        zi = (float)(p.z - o.z);
-    }
-    double dU0 = 0, dU1 = 0, dU2 = 0, dW0 = 0, dW1 = 0, dW2 = 0;
-    double dJ0 = 0, dJ1 = 0, dJ2 = 0, dJ3 = 0, dJ4 = 0, dJ5 = 0, dJ6 = 0, dJ7 = 0;
-
-    for (int64_t base = 0; base < n; base += UJ_BLOCK) {
-        __syncthreads();
-        const int64_t js = base + t;
-        if (js < n) {
-            const double4 p = spos[js];
-            tpos[t] = make_float4((float)(p.x - o.x), (float)(p.y - o.y), (float)(p.z - o.z),
-                                  (float)p.w);
-            tgam[t] = sgam[js];
-        }
-        __syncthreads();
-        const int cnt = (int)((n - base) < UJ_BLOCK ? (n - base) : UJ_BLOCK);
-        MixedAcc acc;
-        if (cnt == UJ_BLOCK) {
-#pragma unroll 4
-            for (int jj = 0; jj < UJ_BLOCK; ++jj)
-                mixed_pair<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
-        } else {
-            for (int jj = 0; jj < cnt; ++jj)
-                mixed_pair<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
-        }
-        dU0 += acc.U0; dU1 += acc.U1; dU2 += acc.U2;
-        dW0 += acc.W0; dW1 += acc.W1; dW2 += acc.W2;
-        dJ0 += acc.J0; dJ1 += acc.J1; dJ2 += acc.J2; dJ3 += acc.J3;
-        dJ4 += acc.J4; dJ5 += acc.J5; dJ6 += acc.J6; dJ7 += acc.J7;
-    }
-    if (!active) return;
-    const int64_t i = perm[s];
-    const double k = -INV4PI;
-    double u0 = k * dU0, u1 = k * dU1, u2 = k * dU2;
-    // J = -1/(4 pi) (radial + eps_klm W_m)
-    double j0 = k * dJ0, j1 = k * (dJ1 - dW2), j2 = k * (dJ2 + dW1);
-    double j3 = k * (dJ3 + dW2), j4 = k * dJ4, j5 = k * (dJ5 - dW0);
-    double j6 = k * (dJ6 - dW1), j7 = k * (dJ7 + dW0), j8 = -k * (dJ0 + dJ4);
-    if (!reset) {
-        u0 += U[i]; u1 += U[cap + i]; u2 += U[2 * cap + i];
-        j0 += J[i]; j1 += J[cap + i]; j2 += J[2 * cap + i];
-        j3 += J[3 * cap + i]; j4 += J[4 * cap + i]; j5 += J[5 * cap + i];
-        j6 += J[6 * cap + i]; j7 += J[7 * cap + i]; j8 += J[8 * cap + i];
-    }
-    U[i] = u0; U[cap + i] = u1; U[2 * cap + i] = u2;
-    J[i] = j0; J[cap + i] = j1; J[2 * cap + i] = j2;
-    J[3 * cap + i] = j3; J[4 * cap + i] = j4; J[5 * cap + i] = j5;
-    J[6 * cap + i] = j6; J[7 * cap + i] = j7; J[8 * cap + i] = j8;
+    const int64_t t0 = (int64_t)blockIdx.x * UJ_BLOCK;
+    uj_mixed_block<K, UJ_BLOCK>(tgt, t0, nt, perm, spos, sgam, 0, n, cap, U, J, reset, tpos, tgam);
 }
 
 // Target-subset variant (probes): targets need not be spatially compact, so
@@ -465,7 +330,7 @@ size_t sort_temp_bytes(int64_t n)
     return bytes;
 }
 
-hipError_t launch_sort_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, Scratch &s)
+hipError_t launch_sort_mixed(hipStream_t st, const DeviceField &f, Scratch &s)
 {
     const int64_t n = f.np;
     if (n == 0) return hipSuccess;
@@ -478,9 +343,14 @@ hipError_t launch_sort_pack_mixed(hipStream_t st, int kernel, const DeviceField 
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     size_t bytes = s.sort_tmp_bytes;
-    e = rocprim::radix_sort_pairs(s.sort_tmp, bytes, s.kin, s.kout, s.vin, s.perm,
-                                  (unsigned int)n, 0, 30, st);
-    if (e != hipSuccess) return e;
+    return rocprim::radix_sort_pairs(s.sort_tmp, bytes, s.kin, s.kout, s.vin, s.perm,
+                                     (unsigned int)n, 0, 30, st);
+}
+
+hipError_t launch_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, Scratch &s)
+{
+    const int64_t n = f.np;
+    if (n == 0) return hipSuccess;
     const unsigned g = grid_for(n, EW_BLOCK);
     switch (kernel) {
     case SINGULAR:
@@ -499,6 +369,12 @@ hipError_t launch_sort_pack_mixed(hipStream_t st, int kernel, const DeviceField 
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_sort_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, Scratch &s)
+{
+    hipError_t e = launch_sort_mixed(st, f, s);
+    return e != hipSuccess ? e : launch_pack_mixed(st, kernel, f, s);
 }
 
 hipError_t launch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
@@ -820,11 +696,20 @@ __global__ __launch_bounds__(EW_BLOCK) void update_kernel(int64_t n, int64_t cap
                                                           const double *__restrict__ E,
                                                           const double *__restrict__ C,
                                                           double *__restrict__ M,
-                                                          const uint8_t *__restrict__ st)
+                                                          const uint8_t *__restrict__ st,
+                                                          const double *__restrict__ uinf,
+                                                          const int32_t *__restrict__ fid)
 {
     const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
     if (st && st[i]) return;
+    double Ux = q.Ux, Uy = q.Uy, Uz = q.Uz;
+    if (uinf) {
+        const int64_t fk = fid ? 3 * (int64_t)fid[i] : 0;
+        Ux = uinf[fk];
+        Uy = uinf[fk + 1];
+        Uz = uinf[fk + 2];
+    }
     double Jm[9];
     for (int k = 0; k < 9; ++k) Jm[k] = J[k * cap + i];
     const double G0 = G[i], G1 = G[cap + i], G2 = G[2 * cap + i];
@@ -850,7 +735,7 @@ __global__ __launch_bounds__(EW_BLOCK) void update_kernel(int64_t n, int64_t cap
     double Z = 0.0;
     if (Gm2 > 0.0)
         Z = ((q.g + q.f) / (1.0 + 3.0 * q.f) * SG - q.f / (1.0 + 3.0 * q.f) * csfs * EG) / Gm2;
-    const double F[7] = {U[i] + q.Ux, U[cap + i] + q.Uy, U[2 * cap + i] + q.Uz,
+    const double F[7] = {U[i] + Ux, U[cap + i] + Uy, U[2 * cap + i] + Uz,
                          S0 - 3.0 * Z * G0 - csfs * e0, S1 - 3.0 * Z * G1 - csfs * e1,
                          S2 - 3.0 * Z * G2 - csfs * e2, -sig * Z};
     double Mq[7];
@@ -873,7 +758,19 @@ hipError_t launch_update(hipStream_t st, const DeviceField &f, int transposed, i
     UpdateArgs q{a, b, dt, fcoef, gcoef, zeta0, Uinf[0], Uinf[1], Uinf[2], transposed, sfs};
     update_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(
         f.np, f.cap, q, f.X, f.G, f.sigma, f.U, f.J, f.E, f.C, f.M,
-        f.any_static ? f.is_static : nullptr);
+        f.any_static ? f.is_static : nullptr, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_dev(hipStream_t st, const DeviceField &f, int transposed, int sfs,
+                             double fcoef, double gcoef, double zeta0, const double *uinf,
+                             const int32_t *fid, double a, double b, double dt)
+{
+    if (f.np == 0) return hipSuccess;
+    UpdateArgs q{a, b, dt, fcoef, gcoef, zeta0, 0.0, 0.0, 0.0, transposed, sfs};
+    update_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(
+        f.np, f.cap, q, f.X, f.G, f.sigma, f.U, f.J, f.E, f.C, f.M,
+        f.any_static ? f.is_static : nullptr, uinf, fid);
     return hipGetLastError();
 }
 

@@ -75,7 +75,7 @@ struct TimedScope {
     hipEvent_t a = nullptr;
     TimedScope(vpmx_ctx *c_, int kind_) : c(c_), kind(kind_)
     {
-        if (c->timer.on) {
+        if (c->timer.on && !c->capturing) {
             a = take_event(c->timer);
             if (a) (void)hipEventRecord(a, c->stream);
         }
@@ -156,7 +156,8 @@ void free_scratch(Scratch &s)
     if (s.perm) (void)hipFree(s.perm);
     if (s.sort_tmp) (void)hipFree(s.sort_tmp);
     if (s.part) (void)hipFree(s.part);
-    for (void *p : {(void *)s.tX, (void *)s.tU, (void *)s.tJ, (void *)s.tpos, (void *)s.tperm})
+    for (void *p : {(void *)s.tX, (void *)s.tU, (void *)s.tJ, (void *)s.tpos, (void *)s.tperm,
+                    (void *)s.bnd, (void *)s.fpart, (void *)s.fout})
         if (p) (void)hipFree(p);
     s = Scratch();
 }
@@ -194,6 +195,117 @@ int ensure_capacity(vpmx_ctx *c, int64_t np)
     s.sort_tmp_bytes = sort_temp_bytes(cap);
     VPMX_TRY(dalloc(c, &s.sort_tmp, s.sort_tmp_bytes));
     s.cap = cap;
+    c->gen++;
+    return VPMX_OK;
+}
+
+// Per-field tables (offsets, boxes, freestreams) for nf fields.
+int ensure_batch_fields(vpmx_ctx *c, int64_t nf)
+{
+    BatchState &B = c->b;
+    if (nf <= B.fcap) return VPMX_OK;
+    const int64_t cap = std::max<int64_t>({nf, 2 * B.fcap, 16});
+    VPMX_TRY(dalloc(c, &B.doff, cap + 1));
+    VPMX_TRY(dalloc(c, &B.box, 6 * cap));
+    VPMX_TRY(dalloc(c, &B.uinf, 3 * cap));
+    B.fcap = cap;
+    c->gen++;
+    return VPMX_OK;
+}
+
+int ensure_batch(vpmx_ctx *c, int64_t nf, int64_t np)
+{
+    BatchState &B = c->b;
+    VPMX_TRY(ensure_batch_fields(c, nf));
+    if (!B.counter) VPMX_TRY(dalloc(c, &B.counter, 1));
+    if (np <= B.pcap) return VPMX_OK;
+    const int64_t cap = std::max<int64_t>({np, B.pcap + B.pcap / 2, 1024});
+    VPMX_TRY(dalloc(c, &B.fid, cap));
+    VPMX_TRY(dalloc(c, &B.kin, cap));
+    VPMX_TRY(dalloc(c, &B.kout, cap));
+    B.sort_tmp_bytes = batch_sort_temp_bytes(cap);
+    VPMX_TRY(dalloc(c, &B.sort_tmp, B.sort_tmp_bytes));
+    B.pcap = cap;
+    c->gen++;
+    return VPMX_OK;
+}
+
+void free_batch(BatchState &B)
+{
+    for (void *p : {(void *)B.doff, (void *)B.box, (void *)B.uinf, (void *)B.fid, (void *)B.kin,
+                    (void *)B.kout, (void *)B.sort_tmp, (void *)B.items, (void *)B.counter})
+        if (p) (void)hipFree(p);
+    B = BatchState();
+}
+
+void drop_graph(GraphState &G)
+{
+    if (G.exec) (void)hipGraphExecDestroy(G.exec);
+    G.exec = nullptr;
+    G.key.clear();
+}
+
+// Work list of the batched pairwise pass for workgroups of `block` targets:
+// one item per block of consecutive targets of one field, against all of the
+// field's sources; fields in decreasing size (longest processing time first),
+// blocks of a field in order.
+int ensure_items(vpmx_ctx *c, int block)
+{
+    BatchState &B = c->b;
+    if (B.items_block == block && B.items_off == B.off) return VPMX_OK;
+    const int nf = B.nfields;
+    std::vector<int> order(nf);
+    for (int k = 0; k < nf; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return B.off[a + 1] - B.off[a] > B.off[b + 1] - B.off[b];
+    });
+    std::vector<BatchItem> items;
+    for (int k : order) {
+        const int64_t o = B.off[k], e = B.off[k + 1];
+        for (int64_t t = o; t < e; t += block)
+            items.push_back({(int32_t)t, (int32_t)std::min<int64_t>(t + block, e), (int32_t)o,
+                             (int32_t)e});
+    }
+    if ((int64_t)items.size() > B.icap) {
+        const int64_t cap = std::max<int64_t>((int64_t)items.size(), 2 * B.icap);
+        VPMX_TRY(dalloc(c, &B.items, cap));
+        B.icap = cap;
+        c->gen++;
+    }
+    if (!items.empty())
+        VPMX_HIP(c, hipMemcpyAsync(B.items, items.data(), items.size() * sizeof(BatchItem),
+                                   hipMemcpyHostToDevice, c->stream));
+    VPMX_TRY(sync(c));
+    B.nitems = (int64_t)items.size();
+    B.items_block = block;
+    B.items_off = B.off;
+    return VPMX_OK;
+}
+
+int cu_count(vpmx_ctx *c)
+{
+    if (c->ncu <= 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || n <= 0)
+            n = 1;
+        c->ncu = n;
+    }
+    return c->ncu;
+}
+
+// Work list and queue grid for the resident batch, before anything is enqueued
+// (the list upload synchronizes, so it never happens inside a capture).
+int prepare_batch(vpmx_ctx *c, int kernel)
+{
+    BatchState &B = c->b;
+    if (!B.nfields) return VPMX_OK;
+    VPMX_TRY(ensure_items(c, B.block));
+    if (B.grid > 0) {
+        B.grid_eff = B.grid;
+    } else {
+        const int occ = batch_queue_occupancy(kernel, B.block);
+        B.grid_eff = std::max(1, occ) * cu_count(c);
+    }
     return VPMX_OK;
 }
 
@@ -282,45 +394,68 @@ int check_step_params(vpmx_ctx *c, const vpmx_step_params *S, double dt)
 }
 
 // ---- host <-> device ------------------------------------------------------------------
-// Upload staging (doubles): X[3 np] G[3 np] sigma[np] C[3 np] | U[3 np] J[9 np],
-// followed by np bytes of static flags.
-int upload(vpmx_ctx *c, const double *P, int64_t np, const vpmx_layout *L, bool with_uj)
+// Upload staging (doubles, stride = total np): X[3] G[3] sigma C[3] | U[3] J[9],
+// followed by total np bytes of static flags.  Field k of a batch fills slots
+// [off[k], off[k+1]).
+std::string where(int nf, int k, int64_t p)
 {
-    VPMX_TRY(ensure_capacity(c, np));
+    std::string s = "particle " + std::to_string(p + 1);
+    if (nf > 1) s = "field " + std::to_string(k + 1) + " " + s;
+    return s;
+}
+
+int upload_fields(vpmx_ctx *c, int nf, const double *const *P, const int64_t *nps,
+                  const vpmx_layout *L, bool with_uj)
+{
+    std::vector<int64_t> off(nf + 1, 0);
+    for (int k = 0; k < nf; ++k) off[k + 1] = off[k] + nps[k];
+    const int64_t np = off[nf];
+    if (np > (int64_t)INT32_MAX) return fail(c, VPMX_EINVAL, "total np exceeds 2^31 - 1");
     DeviceField &f = c->f;
-    f.np = np;
-    if (np == 0) return VPMX_OK;
+    BatchState &B = c->b;
+    // until the upload completes the resident state is one empty field
+    f.np = 0;
+    B.nfields = 0;
+    B.off.assign(2, 0);
+    VPMX_TRY(ensure_capacity(c, np));
     const size_t nd = (size_t)(with_uj ? 22 : 10) * np;
-    VPMX_TRY(ensure_stage(c, nd * sizeof(double) + np));
+    VPMX_TRY(ensure_stage(c, nd * sizeof(double) + np + 8));
     double *h = c->h_stage;
     uint8_t *hs = (uint8_t *)(h + nd);
     const int64_t ld = L->nrows;
     bool any_static = false;
-    for (int64_t p = 0; p < np; ++p) {
-        const double *col = P + p * ld;
-        for (int k = 0; k < 3; ++k) {
-            const double x = col[L->X + k], g = col[L->Gamma + k];
-            if (!isfinite(x) || !isfinite(g))
-                return fail(c, VPMX_EINVAL, "particle " + std::to_string(p + 1) +
-                                                " has a non-finite position or circulation");
-            h[k * np + p] = x;
-            h[(3 + k) * np + p] = g;
-        }
-        const double s = col[L->sigma];
-        if (!(s > 0.0) || !isfinite(s))
-            return fail(c, VPMX_EINVAL, "particle " + std::to_string(p + 1) + " has sigma = " +
-                                            std::to_string(s) + " (must be positive and finite)");
-        h[6 * np + p] = s;
-        for (int k = 0; k < 3; ++k) h[(7 + k) * np + p] = L->C >= 0 ? col[L->C + k] : 0.0;
-        const uint8_t st = (L->static_ >= 0 && col[L->static_] != 0.0) ? 1 : 0;
-        hs[p] = st;
-        any_static |= st != 0;
-        if (with_uj) {
-            for (int k = 0; k < 3; ++k) h[(10 + k) * np + p] = col[L->U + k];
-            for (int k = 0; k < 9; ++k) h[(13 + k) * np + p] = col[L->J + k];
+    for (int fk = 0; fk < nf; ++fk) {
+        for (int64_t p = 0; p < nps[fk]; ++p) {
+            const double *col = P[fk] + p * ld;
+            const int64_t q = off[fk] + p;
+            for (int k = 0; k < 3; ++k) {
+                const double x = col[L->X + k], g = col[L->Gamma + k];
+                if (!isfinite(x) || !isfinite(g))
+                    return fail(c, VPMX_EINVAL,
+                                where(nf, fk, p) + " has a non-finite position or circulation");
+                h[k * np + q] = x;
+                h[(3 + k) * np + q] = g;
+            }
+            const double sg = col[L->sigma];
+            if (!(sg > 0.0) || !isfinite(sg))
+                return fail(c, VPMX_EINVAL, where(nf, fk, p) + " has sigma = " + std::to_string(sg) +
+                                                " (must be positive and finite)");
+            h[6 * np + q] = sg;
+            for (int k = 0; k < 3; ++k) h[(7 + k) * np + q] = L->C >= 0 ? col[L->C + k] : 0.0;
+            const uint8_t st = (L->static_ >= 0 && col[L->static_] != 0.0) ? 1 : 0;
+            hs[q] = st;
+            any_static |= st != 0;
+            if (with_uj) {
+                for (int k = 0; k < 3; ++k) h[(10 + k) * np + q] = col[L->U + k];
+                for (int k = 0; k < 9; ++k) h[(13 + k) * np + q] = col[L->J + k];
+            }
         }
     }
+    f.np = np;
     f.any_static = any_static;
+    B.nfields = nf > 1 ? nf : 0;
+    B.off = off;
+    if (np == 0) return VPMX_OK;
     hipStream_t st = c->stream;
     const size_t b = (size_t)np * sizeof(double);
     for (int k = 0; k < 3; ++k) {
@@ -337,16 +472,28 @@ int upload(vpmx_ctx *c, const double *P, int64_t np, const vpmx_layout *L, bool 
         for (int k = 0; k < 9; ++k)
             VPMX_HIP(c, hipMemcpyAsync(f.J + k * f.cap, h + (13 + k) * np, b, hipMemcpyHostToDevice, st));
     }
+    if (B.nfields) {
+        VPMX_TRY(ensure_batch(c, nf, np));
+        VPMX_HIP(c, hipMemcpyAsync(B.doff, off.data(), off.size() * sizeof(int64_t),
+                                   hipMemcpyHostToDevice, st));
+        VPMX_HIP(c, launch_batch_fid(st, B, np));
+    }
     return sync(c);   // staging is reused: the copies must finish
+}
+
+int upload(vpmx_ctx *c, const double *P, int64_t np, const vpmx_layout *L, bool with_uj)
+{
+    return upload_fields(c, 1, &P, &np, L, with_uj);
 }
 
 enum { DL_STATE = 1, DL_UJ = 2, DL_SFS = 4 };
 
-int download(vpmx_ctx *c, double *P, const vpmx_layout *L, int what)
+int download_fields(vpmx_ctx *c, int nf, double *const *P, const vpmx_layout *L, int what)
 {
     DeviceField &f = c->f;
     const int64_t np = f.np;
     if (np == 0) return VPMX_OK;
+    const std::vector<int64_t> &off = c->b.off;
     VPMX_TRY(ensure_stage(c, (size_t)25 * np * sizeof(double)));
     double *h = c->h_stage;
     hipStream_t st = c->stream;
@@ -373,27 +520,34 @@ int download(vpmx_ctx *c, double *P, const vpmx_layout *L, int what)
     }
     VPMX_TRY(sync(c));
     const int64_t ld = L->nrows;
-    for (int64_t p = 0; p < np; ++p) {
-        double *col = P + p * ld;
-        if (what & DL_STATE) {
-            for (int k = 0; k < 3; ++k) {
-                col[L->X + k] = h[k * np + p];
-                col[L->Gamma + k] = h[(3 + k) * np + p];
+    for (int fk = 0; fk < nf; ++fk) {
+        for (int64_t q = off[fk]; q < off[fk + 1]; ++q) {
+            double *col = P[fk] + (q - off[fk]) * ld;
+            if (what & DL_STATE) {
+                for (int k = 0; k < 3; ++k) {
+                    col[L->X + k] = h[k * np + q];
+                    col[L->Gamma + k] = h[(3 + k) * np + q];
+                }
+                col[L->sigma] = h[6 * np + q];
             }
-            col[L->sigma] = h[6 * np + p];
-        }
-        if (what & DL_UJ) {
-            for (int k = 0; k < 3; ++k) col[L->U + k] = h[(7 + k) * np + p];
-            for (int k = 0; k < 9; ++k) col[L->J + k] = h[(10 + k) * np + p];
-        }
-        if (sfs) {
-            if (L->SFS >= 0)
-                for (int k = 0; k < 3; ++k) col[L->SFS + k] = h[(19 + k) * np + p];
-            if (L->C >= 0)
-                for (int k = 0; k < 3; ++k) col[L->C + k] = h[(22 + k) * np + p];
+            if (what & DL_UJ) {
+                for (int k = 0; k < 3; ++k) col[L->U + k] = h[(7 + k) * np + q];
+                for (int k = 0; k < 9; ++k) col[L->J + k] = h[(10 + k) * np + q];
+            }
+            if (sfs) {
+                if (L->SFS >= 0)
+                    for (int k = 0; k < 3; ++k) col[L->SFS + k] = h[(19 + k) * np + q];
+                if (L->C >= 0)
+                    for (int k = 0; k < 3; ++k) col[L->C + k] = h[(22 + k) * np + q];
+            }
         }
     }
     return VPMX_OK;
+}
+
+int download(vpmx_ctx *c, double *P, const vpmx_layout *L, int what)
+{
+    return download_fields(c, 1, &P, L, what);
 }
 
 // ---- fast multipole UJ ------------------------------------------------------------------
@@ -449,6 +603,8 @@ int fmm_tables_to_device(vpmx_ctx *c, FmmState &S)
     VPMX_TRY(upload_as(c, &S.m2l_toff, T.m2l_off));
     VPMX_TRY(upload_as(c, &S.m2l_tidx, T.m2l_idx));
     VPMX_TRY(upload_as(c, &S.m2l_tcoef, T.m2l_coef));
+    VPMX_TRY(upload_as(c, &S.m2l_chunks, T.m2l_chunk));
+    VPMX_TRY(upload_as(c, &S.m2l_cfirst, T.m2l_cfirst));
     S.dev_tab_p = T.P;
     return VPMX_OK;
 }
@@ -458,7 +614,7 @@ void free_fmm(FmmState *S)
     if (!S) return;
     void *ptrs[] = {S->mi, S->m2m_off, S->l2l_off, S->m2l_toff, S->rec1, S->rec2, S->d1, S->d2a,
                     S->d2b, S->m2m_idx, S->l2l_idx, S->m2l_tidx, S->m2m_coef, S->l2l_coef,
-                    S->m2l_tcoef, S->cells, S->leaves, S->parents, S->kids, S->parent,
+                    S->m2l_tcoef, S->m2l_chunks, S->m2l_cfirst, S->cells, S->leaves, S->parents, S->kids, S->parent,
                     S->m2l_off, S->p2p_off, S->m2l_src, S->p2p_src, S->M, S->L, S->kin, S->kout,
                     S->ssig, S->sort_tmp};
     for (void *p : ptrs)
@@ -639,6 +795,10 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
     d.m2m_coef = S.m2m_coef;
     d.l2l_coef = S.l2l_coef;
     d.m2l_tcoef = S.m2l_tcoef;
+    d.m2l_chunks = S.m2l_chunks;
+    d.m2l_cfirst = S.m2l_cfirst;
+    d.m2l_nchunk = S.tab.m2l_nchunk;
+    d.m2l_rounds = S.tab.m2l_rounds;
     const double ms_before = c->timer.total_ms[T_FMM];
     {
         TimedScope ts(c, T_FMM);
@@ -658,18 +818,44 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
 }
 
 // ---- device sequences (enqueue only; callers sync) ---------------------------------
-int enqueue_uj(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
+// Morton order of the mixed path (per field for a batch).
+int enqueue_sort(vpmx_ctx *c)
+{
+    if (c->b.nfields)
+        VPMX_HIP(c, launch_batch_sort(c->stream, c->f, c->s, c->b));
+    else
+        VPMX_HIP(c, launch_sort_mixed(c->stream, c->f, c->s));
+    return VPMX_OK;
+}
+
+// One UJ evaluation.  `sort` = false reuses the Morton order of the last sort
+// (a time step sorts once and keeps that order through its substeps).
+int enqueue_uj(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool sort = true)
 {
     const DeviceField &f = c->f;
-    if (P->fmm) return run_fmm(c, P, reset);
+    BatchState &B = c->b;
+    if (P->fmm) {
+        if (B.nfields) return fail(c, VPMX_EUNSUPPORTED, "the FMM does not run on batched fields");
+        return run_fmm(c, P, reset);
+    }
+    if (B.nfields && (B.items_block != B.block || B.items_off != B.off))
+        return fail(c, VPMX_EINVAL, "batch work list not prepared");
     if (P->precision == VPMX_FP64) {
         VPMX_HIP(c, launch_pack_f64(c->stream, f, c->s));
         TimedScope ts(c, T_UJ);
-        VPMX_HIP(c, launch_uj_f64(c->stream, P->kernel, f, c->s, reset));
+        if (B.nfields)
+            VPMX_HIP(c, launch_batch_uj_f64(c->stream, P->kernel, f, c->s, B, reset));
+        else
+            VPMX_HIP(c, launch_uj_f64(c->stream, P->kernel, f, c->s, reset));
     } else {
-        VPMX_HIP(c, launch_sort_pack_mixed(c->stream, P->kernel, f, c->s));
+        if (sort) VPMX_TRY(enqueue_sort(c));
+        VPMX_HIP(c, launch_pack_mixed(c->stream, P->kernel, f, c->s));
         TimedScope ts(c, T_UJ);
-        VPMX_HIP(c, launch_uj_mixed(c->stream, P->kernel, f, c->s, reset));
+        if (B.nfields)
+            VPMX_HIP(c, launch_batch_uj_mixed(c->stream, P->kernel, f, c->s, B, reset, B.sched,
+                                              B.block, B.grid_eff));
+        else
+            VPMX_HIP(c, launch_uj_mixed(c->stream, P->kernel, f, c->s, reset));
     }
     return VPMX_OK;
 }
@@ -693,7 +879,7 @@ int enqueue_estr(vpmx_ctx *c, const vpmx_uj_params *U)
 int enqueue_dynamic_test_filter(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S)
 {
     VPMX_HIP(c, launch_sigma_scale(c->stream, c->f, S->alpha));
-    VPMX_TRY(enqueue_uj(c, U, 1));
+    VPMX_TRY(enqueue_uj(c, U, 1, false));
     VPMX_TRY(enqueue_estr(c, U));
     VPMX_HIP(c, launch_dyn_stash(c->stream, c->f, U->transposed));
     VPMX_HIP(c, launch_sigma_restore(c->stream, c->f));
@@ -705,19 +891,22 @@ int enqueue_dynamic_test_filter(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx
 constexpr double RK3_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
 constexpr double RK3_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 
+// The device work of one step after the step's sort: every substep's UJ (and
+// Estr, SFS coefficient) and fused update, then the relaxation.  Enqueues
+// kernels only, so it can be captured into a graph.  The freestream of every
+// particle comes from the device table b.uinf (per field for a batch).
 int enqueue_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S, double dt,
                  int relax, bool rk3)
 {
     DeviceField &f = c->f;
     const int sfs = S->sfs != VPMX_SFS_NONE;
-    if (sfs && U->kernel == VPMX_KERNEL_SINGULAR)
-        return fail(c, VPMX_EINVAL, "the SFS model needs a smooth kernel");
     const double zeta0 = zeta0_of(U->kernel);
     const int nst = rk3 ? 3 : 1;
+    const int32_t *fid = c->b.nfields ? c->b.fid : nullptr;
     if (rk3) VPMX_HIP(c, launch_fill(c->stream, f.M, 7 * f.cap, 0.0));
     for (int s = 0; s < nst; ++s) {
         if (S->sfs == VPMX_SFS_DYNAMIC) VPMX_TRY(enqueue_dynamic_test_filter(c, U, S));
-        VPMX_TRY(enqueue_uj(c, U, 1));
+        VPMX_TRY(enqueue_uj(c, U, 1, false));
         if (sfs) {
             VPMX_TRY(enqueue_estr(c, U));
             if (S->sfs == VPMX_SFS_DYNAMIC)
@@ -727,13 +916,106 @@ int enqueue_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S
                 VPMX_HIP(c, launch_sfs_coeff(c->stream, f, S->Cs, S->clip_backscatter));
         }
         TimedScope ts(c, T_UPDATE);
-        VPMX_HIP(c, launch_update(c->stream, f, U->transposed, sfs, S->f, S->g, zeta0, S->Uinf,
-                                  rk3 ? RK3_A[s] : 0.0, rk3 ? RK3_B[s] : 1.0, dt));
+        VPMX_HIP(c, launch_update_dev(c->stream, f, U->transposed, sfs, S->f, S->g, zeta0,
+                                      c->b.uinf, fid, rk3 ? RK3_A[s] : 0.0,
+                                      rk3 ? RK3_B[s] : 1.0, dt));
     }
     if (relax && S->relax_kind != VPMX_RELAX_NONE) {
-        if (rk3) VPMX_TRY(enqueue_uj(c, U, 1));
+        if (rk3) VPMX_TRY(enqueue_uj(c, U, 1, false));
         VPMX_HIP(c, launch_relax(c->stream, f, S->relax_kind, S->rlxf));
     }
+    return VPMX_OK;
+}
+
+// Settings a captured step depends on, serialized field by field.
+struct KeyWriter {
+    std::vector<unsigned char> v;
+    template <class T> void put(const T &x)
+    {
+        const unsigned char *p = (const unsigned char *)&x;
+        v.insert(v.end(), p, p + sizeof(T));
+    }
+};
+
+std::vector<unsigned char> step_key(vpmx_ctx *c, const vpmx_uj_params *U,
+                                    const vpmx_step_params *S, double dt, int relax, bool rk3)
+{
+    KeyWriter k;
+    k.put(U->kernel); k.put(U->precision); k.put(U->transposed); k.put(U->fmm);
+    k.put(S->f); k.put(S->g); k.put(S->sfs); k.put(S->Cs); k.put(S->clip_backscatter);
+    k.put(S->relax_kind); k.put(S->rlxf); k.put(S->alpha); k.put(S->sfs_rlxf);
+    k.put(S->minC); k.put(S->maxC);
+    k.put(dt); k.put(relax); k.put(rk3);
+    k.put(c->f.np); k.put(c->f.cap); k.put(c->f.any_static); k.put(c->gen);
+    const BatchState &B = c->b;
+    k.put(B.nfields); k.put(B.nitems); k.put(B.items_block); k.put(B.sched); k.put(B.grid_eff);
+    return k.v;
+}
+
+// Runs one step on the resident state: per-step inputs (freestream table,
+// Morton sort), then the step's device work, replayed from a graph when graphs
+// are on and the step has no host-side phase (the FMM builds its tree on the
+// host).  `uinf_fields` holds 3 values per field, or NULL for S->Uinf everywhere.
+int run_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S,
+             const double *uinf_fields, double dt, int relax, bool rk3)
+{
+    BatchState &B = c->b;
+    if (S->sfs != VPMX_SFS_NONE && U->kernel == VPMX_KERNEL_SINGULAR)
+        return fail(c, VPMX_EINVAL, "the SFS model needs a smooth kernel");
+    if (B.nfields && S->sfs != VPMX_SFS_NONE)
+        return fail(c, VPMX_EUNSUPPORTED, "the SFS model does not run on batched fields");
+    if (B.nfields && U->fmm)
+        return fail(c, VPMX_EUNSUPPORTED, "the FMM does not run on batched fields");
+    if (c->f.np == 0) return VPMX_OK;
+    VPMX_TRY(prepare_batch(c, U->kernel));
+    const int nf = std::max(1, B.nfields);
+    VPMX_TRY(ensure_batch_fields(c, nf));
+    if (c->h_small_bytes < (size_t)3 * nf * sizeof(double)) {
+        if (c->h_small) (void)hipHostFree(c->h_small);
+        c->h_small = nullptr;
+        c->h_small_bytes = 0;
+        void *p = nullptr;
+        const size_t want = (size_t)3 * std::max(nf, 64) * sizeof(double);
+        VPMX_HIP(c, hipHostMalloc(&p, want, hipHostMallocDefault));
+        c->h_small = (double *)p;
+        c->h_small_bytes = want;
+    }
+    for (int k = 0; k < nf; ++k)
+        for (int j = 0; j < 3; ++j)
+            c->h_small[3 * k + j] = uinf_fields ? uinf_fields[3 * k + j] : S->Uinf[j];
+    for (int k = 0; k < 3 * nf; ++k)
+        if (!isfinite(c->h_small[k])) return fail(c, VPMX_EINVAL, "freestream is not finite");
+    VPMX_HIP(c, hipMemcpyAsync(B.uinf, c->h_small, (size_t)3 * nf * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+    if (!U->fmm && U->precision == VPMX_MIXED) VPMX_TRY(enqueue_sort(c));
+    if (!c->g.on || U->fmm) return enqueue_step(c, U, S, dt, relax, rk3);
+
+    GraphState &G = c->g;
+    std::vector<unsigned char> key = step_key(c, U, S, dt, relax, rk3);
+    if (!G.exec || key != G.key) {
+        drop_graph(G);
+        VPMX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
+        const int r = enqueue_step(c, U, S, dt, relax, rk3);
+        c->capturing = false;
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (r != VPMX_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return r;
+        }
+        if (e != hipSuccess)
+            return fail(c, VPMX_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            G.exec = nullptr;
+            return fail(c, VPMX_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+        G.key = key;
+    }
+    TimedScope ts(c, T_GRAPH);
+    VPMX_HIP(c, hipGraphLaunch(G.exec, c->stream));
     return VPMX_OK;
 }
 
@@ -798,11 +1080,14 @@ void vpmx_destroy(vpmx_ctx *c)
         (void)hipEventDestroy(p.b);
     }
     for (auto e : c->timer.pool) (void)hipEventDestroy(e);
+    drop_graph(c->g);
     free_field(c->f);
     free_scratch(c->s);
+    free_batch(c->b);
     free_fmm(c->fmm);
     c->fmm = nullptr;
     if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->h_small) (void)hipHostFree(c->h_small);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -834,7 +1119,7 @@ static int step_host(vpmx_ctx *c, double *particles, int64_t np, const vpmx_layo
     VPMX_TRY(check_uj_params(c, U));
     VPMX_TRY(check_step_params(c, S, dt));
     VPMX_TRY(upload(c, particles, np, L, false));
-    VPMX_TRY(enqueue_step(c, U, S, dt, relax, rk3));
+    VPMX_TRY(run_step(c, U, S, nullptr, dt, relax, rk3));
     VPMX_TRY(sync(c));
     return download(c, particles, L, DL_STATE | DL_UJ | (S->sfs ? DL_SFS : 0));
 }
@@ -864,6 +1149,10 @@ int vpmx_download(vpmx_ctx *c, double *particles, int64_t np, const vpmx_layout 
     VPMX_TRY(entry(c));
     VPMX_TRY(check_particles(c, particles, np));
     VPMX_TRY(check_layout(c, L));
+    if (c->b.nfields)
+        return fail(c, VPMX_EINVAL, "the resident state is a batch of " +
+                                        std::to_string(c->b.nfields) +
+                                        " fields: use vpmx_batch_download");
     if (np != c->f.np)
         return fail(c, VPMX_EINVAL, "np = " + std::to_string(np) + " but " +
                                         std::to_string(c->f.np) + " particles are resident");
@@ -875,6 +1164,9 @@ int vpmx_uj_resident(vpmx_ctx *c, const vpmx_uj_params *P)
 {
     VPMX_TRY(entry(c));
     VPMX_TRY(check_uj_params(c, P));
+    if (c->b.nfields && P->sfs)
+        return fail(c, VPMX_EUNSUPPORTED, "the SFS model does not run on batched fields");
+    VPMX_TRY(prepare_batch(c, P->kernel));
     VPMX_TRY(enqueue_uj(c, P, P->reset));
     if (P->sfs) VPMX_TRY(enqueue_estr(c, P));
     return sync(c);
@@ -886,7 +1178,7 @@ int vpmx_rk3_resident(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_para
     VPMX_TRY(entry(c));
     VPMX_TRY(check_uj_params(c, U));
     VPMX_TRY(check_step_params(c, S, dt));
-    VPMX_TRY(enqueue_step(c, U, S, dt, relax, true));
+    VPMX_TRY(run_step(c, U, S, nullptr, dt, relax, true));
     return sync(c);
 }
 
@@ -896,7 +1188,7 @@ int vpmx_euler_resident(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_pa
     VPMX_TRY(entry(c));
     VPMX_TRY(check_uj_params(c, U));
     VPMX_TRY(check_step_params(c, S, dt));
-    VPMX_TRY(enqueue_step(c, U, S, dt, relax, false));
+    VPMX_TRY(run_step(c, U, S, nullptr, dt, relax, false));
     return sync(c);
 }
 
@@ -945,6 +1237,8 @@ int vpmx_uj_targets(vpmx_ctx *c, const double *targets, int64_t m, double *Uout,
     VPMX_TRY(check_uj_params(c, P));
     if (P->fmm) return fail(c, VPMX_EUNSUPPORTED, "target evaluation uses the direct scheme (fmm = 0)");
     if (P->sfs) return fail(c, VPMX_EUNSUPPORTED, "target evaluation has no SFS term");
+    if (c->b.nfields)
+        return fail(c, VPMX_EUNSUPPORTED, "target evaluation needs a single resident field");
     if (m < 0) return fail(c, VPMX_EINVAL, "number of targets is negative");
     if (m > (int64_t)INT32_MAX) return fail(c, VPMX_EINVAL, "number of targets exceeds 2^31 - 1");
     if (m > 0 && (!targets || !Uout || !Jout)) return fail(c, VPMX_EINVAL, "targets or outputs NULL");
@@ -1091,6 +1385,273 @@ int vpmx_radial_f32(int32_t kernel, float r, float sigma, float *A, float *B)
     default: return VPMX_EINVAL;
     }
     return VPMX_OK;
+}
+
+// ---- batched fields ------------------------------------------------------------
+static int check_batch(vpmx_ctx *c, int32_t nf, const void *P, const int64_t *np)
+{
+    if (nf < 1) return fail(c, VPMX_EINVAL, "nfields must be at least 1");
+    if (!P || !np) return fail(c, VPMX_EINVAL, "particles or np is NULL");
+    int64_t tot = 0;
+    for (int32_t k = 0; k < nf; ++k) {
+        if (np[k] < 0) return fail(c, VPMX_EINVAL, "np of field " + std::to_string(k + 1) + " is negative");
+        if (np[k] > 0 && !((const double *const *)P)[k])
+            return fail(c, VPMX_EINVAL, "particles of field " + std::to_string(k + 1) + " is NULL");
+        tot += np[k];
+        if (tot > (int64_t)INT32_MAX) return fail(c, VPMX_EINVAL, "total np exceeds 2^31 - 1");
+    }
+    return VPMX_OK;
+}
+
+static int check_batch_uj(vpmx_ctx *c, int32_t nf, const vpmx_uj_params *P)
+{
+    if (nf > 1 && P->fmm) return fail(c, VPMX_EUNSUPPORTED, "the FMM does not run on batched fields");
+    if (nf > 1 && P->sfs)
+        return fail(c, VPMX_EUNSUPPORTED, "the SFS model does not run on batched fields");
+    return VPMX_OK;
+}
+
+int vpmx_batch_upload(vpmx_ctx *c, int32_t nfields, const double *const *particles,
+                      const int64_t *np, const vpmx_layout *L)
+{
+    VPMX_TRY(entry(c));
+    VPMX_TRY(check_batch(c, nfields, particles, np));
+    VPMX_TRY(check_layout(c, L));
+    return upload_fields(c, nfields, particles, np, L, false);
+}
+
+int vpmx_batch_download(vpmx_ctx *c, int32_t nfields, double *const *particles,
+                        const int64_t *np, const vpmx_layout *L)
+{
+    VPMX_TRY(entry(c));
+    VPMX_TRY(check_batch(c, nfields, particles, np));
+    VPMX_TRY(check_layout(c, L));
+    const int32_t nres = vpmx_batch_nfields(c);
+    if (nfields != nres)
+        return fail(c, VPMX_EINVAL, "nfields = " + std::to_string(nfields) + " but " +
+                                        std::to_string(nres) + " fields are resident");
+    for (int32_t k = 0; k < nfields; ++k)
+        if (np[k] != c->b.off[k + 1] - c->b.off[k])
+            return fail(c, VPMX_EINVAL, "np of field " + std::to_string(k + 1) +
+                                            " does not match the resident field");
+    const int sfs = (L->SFS >= 0 || L->C >= 0) ? DL_SFS : 0;
+    return download_fields(c, nfields, particles, L, DL_STATE | DL_UJ | sfs);
+}
+
+int vpmx_batch_uj(vpmx_ctx *c, int32_t nfields, double *const *particles, const int64_t *np,
+                  const vpmx_layout *L, const vpmx_uj_params *P)
+{
+    VPMX_TRY(entry(c));
+    VPMX_TRY(check_batch(c, nfields, particles, np));
+    VPMX_TRY(check_layout(c, L));
+    VPMX_TRY(check_uj_params(c, P));
+    VPMX_TRY(check_batch_uj(c, nfields, P));
+    if (P->sfs && L->SFS < 0) return fail(c, VPMX_EINVAL, "sfs requested but layout.SFS = -1");
+    VPMX_TRY(upload_fields(c, nfields, (const double *const *)particles, np, L, !P->reset));
+    VPMX_TRY(prepare_batch(c, P->kernel));
+    VPMX_TRY(enqueue_uj(c, P, P->reset));
+    if (P->sfs) VPMX_TRY(enqueue_estr(c, P));
+    VPMX_TRY(sync(c));
+    return download_fields(c, nfields, particles, L, DL_UJ | (P->sfs ? DL_SFS : 0));
+}
+
+int vpmx_batch_rk3_step(vpmx_ctx *c, int32_t nfields, double *const *particles,
+                        const int64_t *np, const vpmx_layout *L, const vpmx_uj_params *U,
+                        const vpmx_step_params *S, const double *Uinf, double dt, int32_t relax)
+{
+    VPMX_TRY(entry(c));
+    VPMX_TRY(check_batch(c, nfields, particles, np));
+    VPMX_TRY(check_layout(c, L));
+    VPMX_TRY(check_uj_params(c, U));
+    VPMX_TRY(check_step_params(c, S, dt));
+    VPMX_TRY(check_batch_uj(c, nfields, U));
+    VPMX_TRY(upload_fields(c, nfields, (const double *const *)particles, np, L, false));
+    VPMX_TRY(run_step(c, U, S, Uinf, dt, relax, true));
+    VPMX_TRY(sync(c));
+    return download_fields(c, nfields, particles, L, DL_STATE | DL_UJ | (S->sfs ? DL_SFS : 0));
+}
+
+int vpmx_batch_rk3_resident(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S,
+                            const double *Uinf, double dt, int32_t relax)
+{
+    VPMX_TRY(entry(c));
+    VPMX_TRY(check_uj_params(c, U));
+    VPMX_TRY(check_step_params(c, S, dt));
+    VPMX_TRY(run_step(c, U, S, Uinf, dt, relax, true));
+    return sync(c);
+}
+
+int32_t vpmx_batch_nfields(const vpmx_ctx *c)
+{
+    if (!c) return -1;
+    if (c->b.nfields) return c->b.nfields;
+    return c->b.off.size() > 1 ? 1 : 0;
+}
+
+int vpmx_batch_set_schedule(vpmx_ctx *c, int32_t schedule, int32_t block, int32_t grid)
+{
+    if (!c) return fail(nullptr, VPMX_EINVAL, "context is NULL");
+    if (schedule != VPMX_SCHED_FLAT && schedule != VPMX_SCHED_QUEUE)
+        return fail(c, VPMX_EINVAL, "unknown schedule " + std::to_string(schedule));
+    if (block != 64 && block != 128 && block != 256)
+        return fail(c, VPMX_EINVAL, "block must be 64, 128 or 256");
+    if (grid < 0) return fail(c, VPMX_EINVAL, "grid must be >= 0");
+    c->b.sched = schedule;
+    c->b.block = block;
+    c->b.grid = grid;
+    c->b.tuned = false;
+    return VPMX_OK;
+}
+
+int vpmx_batch_get_schedule(vpmx_ctx *c, int32_t *schedule, int32_t *block, int32_t *grid)
+{
+    if (!c) return fail(nullptr, VPMX_EINVAL, "context is NULL");
+    if (schedule) *schedule = c->b.sched;
+    if (block) *block = c->b.block;
+    if (grid) *grid = c->b.grid > 0 ? c->b.grid : c->b.grid_eff;
+    return VPMX_OK;
+}
+
+int vpmx_batch_autotune(vpmx_ctx *c, const vpmx_uj_params *P, int32_t reps, double *ms,
+                        int32_t nms, int32_t *ncand)
+{
+    VPMX_TRY(entry(c));
+    VPMX_TRY(check_uj_params(c, P));
+    if (P->fmm || P->sfs)
+        return fail(c, VPMX_EUNSUPPORTED, "the autotuner covers the direct UJ without SFS");
+    BatchState &B = c->b;
+    if (!B.nfields) return fail(c, VPMX_EINVAL, "no batch of several fields is resident");
+    if (nms > 0 && !ms) return fail(c, VPMX_EINVAL, "ms is NULL");
+    if (reps < 1) reps = 1;
+    const bool mixed = P->precision == VPMX_MIXED;
+    struct Cand { int sched, block; };
+    std::vector<Cand> cands;
+    for (int blk : {64, 128, 256}) {
+        cands.push_back({SCHED_FLAT, blk});
+        if (mixed) cands.push_back({SCHED_QUEUE, blk});
+    }
+    if (ncand) *ncand = (int32_t)cands.size();
+    if (mixed) {
+        VPMX_TRY(enqueue_sort(c));
+        VPMX_HIP(c, launch_pack_mixed(c->stream, P->kernel, c->f, c->s));
+    } else {
+        VPMX_HIP(c, launch_pack_f64(c->stream, c->f, c->s));
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    VPMX_HIP(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(c, VPMX_EHIP, "hipEventCreate failed");
+    }
+    auto launch = [&]() -> hipError_t {
+        return mixed ? launch_batch_uj_mixed(c->stream, P->kernel, c->f, c->s, B, 1, B.sched,
+                                             B.block, B.grid_eff)
+                     : launch_batch_uj_f64(c->stream, P->kernel, c->f, c->s, B, 1);
+    };
+    int rc = VPMX_OK, best = -1;
+    double best_ms = INFINITY;
+    for (size_t i = 0; i < cands.size() && rc == VPMX_OK; ++i) {
+        B.sched = cands[i].sched;
+        B.block = cands[i].block;
+        B.grid = 0;
+        rc = prepare_batch(c, P->kernel);
+        if (rc != VPMX_OK) break;
+        if (i == 0 && launch() != hipSuccess) {   // warm-up: first-launch costs
+            rc = fail(c, VPMX_EHIP, "batched UJ launch failed");
+            break;
+        }
+        double tmin = INFINITY;
+        for (int r = 0; r < reps; ++r) {
+            hipError_t e = hipEventRecord(e0, c->stream);
+            if (e == hipSuccess) e = launch();
+            if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float t = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+            if (e != hipSuccess) {
+                rc = fail(c, VPMX_EHIP, std::string("autotune: ") + hipGetErrorString(e));
+                break;
+            }
+            tmin = std::min(tmin, (double)t);
+        }
+        if (i < (size_t)std::max(nms, 0)) ms[i] = tmin;
+        if (tmin < best_ms) {
+            best_ms = tmin;
+            best = (int)i;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    VPMX_TRY(rc);
+    B.sched = cands[best].sched;
+    B.block = cands[best].block;
+    B.grid = 0;
+    VPMX_TRY(prepare_batch(c, P->kernel));
+    B.tuned = true;
+    return sync(c);
+}
+
+int vpmx_set_graph(vpmx_ctx *c, int32_t on)
+{
+    if (!c) return fail(nullptr, VPMX_EINVAL, "context is NULL");
+    c->g.on = on != 0;
+    if (!c->g.on) {
+        (void)hipSetDevice(c->device);
+        drop_graph(c->g);
+    }
+    return VPMX_OK;
+}
+
+// ---- free-vortex force ------------------------------------------------------------
+int vpmx_ffv(vpmx_ctx *c, const double *bound, int64_t nb, const vpmx_layout *L, int32_t kernel,
+             double *out)
+{
+    VPMX_TRY(entry(c));
+    VPMX_TRY(check_particles(c, bound, nb));
+    VPMX_TRY(check_layout(c, L));
+    if (kernel < VPMX_KERNEL_SINGULAR || kernel > VPMX_KERNEL_WINCKELMANS)
+        return fail(c, VPMX_EINVAL, "unknown kernel " + std::to_string(kernel));
+    if (nb > 0 && !out) return fail(c, VPMX_EINVAL, "out is NULL");
+    if (nb == 0) return VPMX_OK;
+    if (c->b.nfields)
+        return fail(c, VPMX_EUNSUPPORTED, "the free-vortex force needs a single resident field");
+    std::vector<double> h(8 * (size_t)nb, 0.0);
+    for (int64_t b = 0; b < nb; ++b) {
+        const double *col = bound + b * L->nrows;
+        for (int k = 0; k < 3; ++k) {
+            h[8 * b + k] = col[L->X + k];
+            h[8 * b + 3 + k] = col[L->Gamma + k];
+        }
+        const double sg = col[L->sigma];
+        if (!(sg > 0.0) || !isfinite(sg))
+            return fail(c, VPMX_EINVAL, "bound particle " + std::to_string(b + 1) +
+                                            " has sigma = " + std::to_string(sg));
+        for (int k = 0; k < 6; ++k)
+            if (!isfinite(h[8 * b + k]))
+                return fail(c, VPMX_EINVAL, "bound particle " + std::to_string(b + 1) +
+                                                " has a non-finite position or circulation");
+        h[8 * b + 6] = 1.0 / sg;
+    }
+    Scratch &s = c->s;
+    if (!s.part) VPMX_TRY(ensure_capacity(c, 0));
+    const int nseg = ffv_segments(nb, c->f.np);
+    if (nb > s.bcap) {
+        const int64_t cap = std::max<int64_t>(nb, 2 * s.bcap);
+        VPMX_TRY(dalloc(c, &s.bnd, 8 * cap));
+        VPMX_TRY(dalloc(c, &s.fout, 6 * cap));
+        s.bcap = cap;
+    }
+    if (nb * nseg > s.fpcap) {
+        const int64_t cap = std::max<int64_t>(nb * nseg, 2 * s.fpcap);
+        VPMX_TRY(dalloc(c, &s.fpart, 6 * cap));
+        s.fpcap = cap;
+    }
+    VPMX_HIP(c, hipMemcpyAsync(s.bnd, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+    VPMX_HIP(c, launch_ffv(c->stream, kernel, s.bnd, (int)nb, c->f, nseg, s.fpart, s.fout));
+    VPMX_HIP(c, hipMemcpyAsync(out, s.fout, 6 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+    return sync(c);
 }
 
 }  // extern "C"

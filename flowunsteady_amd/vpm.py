@@ -432,6 +432,68 @@ def nextstep(pfield: ParticleField, dt: float, *, relax: bool = False, custom_UJ
     pfield.nt += 1
 
 
+# ---- sweeps: many independent fields advanced together ---------------------------------
+def _batch_args(pfields, fmm_ok=False):
+    """Pointer array, sizes and the shared settings of a list of fields.  The
+    fields are independent cases (never interacting) that share the solver
+    settings: kernel, precision, formulation, SFS model, relaxation, device."""
+    pfields = list(pfields)
+    if not pfields:
+        raise ValueError("no particle fields")
+    a = pfields[0]
+    for pf in pfields[1:]:
+        same = (pf.kernel.id == a.kernel.id and pf.precision == a.precision
+                and pf.formulation == a.formulation and pf.SFS.kind == a.SFS.kind
+                and pf.relaxation.kind == a.relaxation.kind
+                and pf.relaxation.rlxf == a.relaxation.rlxf and pf.transposed == a.transposed
+                and pf.device == a.device and pf.layout.nrows == a.layout.nrows)
+        if not same:
+            raise ValueError("batched fields must share kernel, precision, formulation, SFS, "
+                             "relaxation, transposed and device")
+    ptrs = N.pointer_array([pf.particles for pf in pfields])
+    nps = (ctypes.c_int64 * len(pfields))(*[pf.np for pf in pfields])
+    return pfields, a, ptrs, nps
+
+
+def UJ_gpu_batch(pfields, *, reset: bool = True, **kwargs):
+    """UJ_gpu of every field of a sweep in one launch (fields never interact):
+    U and J of each field written into its own particle matrix."""
+    pfields, a, ptrs, nps = _batch_args(pfields)
+    ctx = _context(a.device)
+    P = a._uj_params(reset=reset, fmm=False)
+    ctx.check(N.lib().vpmx_batch_uj(ctx.ptr, len(pfields), ptrs, nps, ctypes.byref(a.layout),
+                                    ctypes.byref(P)))
+
+
+def rungekutta3_gpu_batch(pfields, dt: float, *, relax: bool = False, **kwargs):
+    """rungekutta3_gpu of every field of a sweep as one batched step: the same
+    dt and settings, each case with its own freestream Uinf(t)."""
+    pfields, a, ptrs, nps = _batch_args(pfields)
+    ctx = _context(a.device)
+    U = a._uj_params(sfs=a.SFS.kind != N.SFS_NONE, fmm=False)
+    S = a._step_params()
+    uinf = np.concatenate([np.asarray(pf.Uinf(pf.t), dtype=float).reshape(3) for pf in pfields])
+    ctx.check(N.lib().vpmx_batch_rk3_step(ctx.ptr, len(pfields), ptrs, nps,
+                                          ctypes.byref(a.layout), ctypes.byref(U),
+                                          ctypes.byref(S), uinf.ctypes.data_as(N._dp),
+                                          float(dt), int(bool(relax))))
+
+
+def nextstep_batch(pfields, dt: float, *, relax: bool = False):
+    """vpm.nextstep of every field of a sweep (src/FLOWUnsteady_simulation.jl:358),
+    as one batched step."""
+    rungekutta3_gpu_batch(pfields, dt, relax=relax)
+    for pf in pfields:
+        pf.t += dt
+        pf.nt += 1
+
+
+def set_graphs(on: bool, device: int = 0):
+    """Run time steps on the direct scheme as replayed hipGraphs (bitwise the
+    same results as eager launches)."""
+    _context(device).set_graph(on)
+
+
 def relax_due(pfield: ParticleField, i: int) -> bool:
     """Relaxation cadence of src/FLOWUnsteady_simulation.jl:346-348."""
     r = pfield.relaxation

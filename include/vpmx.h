@@ -72,6 +72,10 @@ extern "C" {
 #define VPMX_SFS_DYNAMIC 2    /* C from the dynamic procedure (test filter
                                  alpha*sigma, Lagrangian average), clipped   */
 
+/* Schedules of the batched pairwise pass (vpmx_batch_set_schedule). */
+#define VPMX_SCHED_FLAT 0     /* one workgroup per work item, largest field first  */
+#define VPMX_SCHED_QUEUE 1    /* persistent grid draining a device work queue      */
+
 /* ---- types ----------------------------------------------------------------- */
 typedef struct vpmx_ctx vpmx_ctx;
 
@@ -183,10 +187,76 @@ int vpmx_diagnostics(vpmx_ctx *ctx, double *enstrophy, double *energy);
 /* Number of particles currently resident. */
 int64_t vpmx_resident_np(const vpmx_ctx *ctx);
 
+/* Free-vortex force of bound vortices on the resident field, FLOWUnsteady's
+ * _Ffv_direct(source, target) (src/FLOWUnsteady_processing_force.jl:860-929)
+ * with the bound particles as `source` and the resident particles (static
+ * included) as `target`: for bound particle b of `bound` (nb columns in
+ * `layout`), with dX = x_f - x_b and g = g(r / sigma_b),
+ *   out[6 b + 0..2] = Ffv/rho  = sum_f U_b(x_f) x Gamma_f,
+ *   out[6 b + 3..5] = Fbxf/rho = sum_f g/(4 pi r^3) dX x (Gamma_b x Gamma_f),
+ * (the values the reference stores in P.M[1:6]); coincident pairs skipped.
+ * FP64, fixed summation order.  kernel: VPMX_KERNEL_*. */
+int vpmx_ffv(vpmx_ctx *ctx, const double *bound, int64_t nb, const vpmx_layout *layout,
+             int32_t kernel, double *out);
+
+/* ---- batched fields ------------------------------------------------------------- */
+/* Many independent particle fields of uneven size (a sweep of rotor or wing
+ * cases, each of which run_simulation advances with its own nextstep,
+ * src/FLOWUnsteady_simulation.jl:339-365) resident as one concatenation and
+ * advanced together: one pairwise launch covers every field, and no field
+ * ever sees another's particles.  particles[k] is field k's host matrix with
+ * np[k] columns, all in the same layout.  A batch of one field is a single
+ * resident field.  The direct scheme only; the SFS model and the FMM are
+ * rejected with VPMX_EUNSUPPORTED for more than one field.  The resident entry
+ * points above (vpmx_uj_resident, vpmx_rk3_resident, vpmx_euler_resident,
+ * vpmx_diagnostics, which then sums over the fields) accept a resident batch. */
+int vpmx_batch_upload(vpmx_ctx *ctx, int32_t nfields, const double *const *particles,
+                      const int64_t *np, const vpmx_layout *layout);
+int vpmx_batch_download(vpmx_ctx *ctx, int32_t nfields, double *const *particles,
+                        const int64_t *np, const vpmx_layout *layout);
+/* UJ of every field: upload, one evaluation, U and J written back (vpmx_uj per field). */
+int vpmx_batch_uj(vpmx_ctx *ctx, int32_t nfields, double *const *particles, const int64_t *np,
+                  const vpmx_layout *layout, const vpmx_uj_params *params);
+/* One RK3 step of every field (vpmx_rk3_step per field).  Uinf: 3 values per
+ * field (the freestream of each case), or NULL for step->Uinf everywhere. */
+int vpmx_batch_rk3_step(vpmx_ctx *ctx, int32_t nfields, double *const *particles,
+                        const int64_t *np, const vpmx_layout *layout,
+                        const vpmx_uj_params *uj, const vpmx_step_params *step,
+                        const double *Uinf, double dt, int32_t relax);
+/* The same step on the resident batch. */
+int vpmx_batch_rk3_resident(vpmx_ctx *ctx, const vpmx_uj_params *uj,
+                            const vpmx_step_params *step, const double *Uinf, double dt,
+                            int32_t relax);
+/* Number of resident fields (0 when nothing was uploaded). */
+int32_t vpmx_batch_nfields(const vpmx_ctx *ctx);
+/* Schedule of the batched pairwise pass: VPMX_SCHED_FLAT or VPMX_SCHED_QUEUE,
+ * workgroups of block = 64, 128 or 256 targets, and for the queue the grid
+ * (0 = resident occupancy x compute units). */
+int vpmx_batch_set_schedule(vpmx_ctx *ctx, int32_t schedule, int32_t block, int32_t grid);
+int vpmx_batch_get_schedule(vpmx_ctx *ctx, int32_t *schedule, int32_t *block, int32_t *grid);
+/* Autotuner: times the pairwise pass of the resident batch under every
+ * candidate schedule (flat and queue at block 64, 128, 256; flat only for
+ * FP64), best of `reps` runs each, keeps the fastest.  Writes the candidates'
+ * milliseconds into ms[0 .. min(nms, ncand) - 1] in that order (flat 64, queue 64,
+ * flat 128, queue 128, flat 256, queue 256; FP64: flat 64, 128, 256) and
+ * their count into *ncand.  Leaves U and J of the resident batch evaluated. */
+int vpmx_batch_autotune(vpmx_ctx *ctx, const vpmx_uj_params *params, int32_t reps, double *ms,
+                        int32_t nms, int32_t *ncand);
+
+/* ---- graphs ---------------------------------------------------------------------- */
+/* on != 0: time steps on the direct scheme (vpmx_rk3_*, vpmx_euler_*,
+ * vpmx_batch_rk3_*) run their device work -- every substep's UJ, SFS and fused
+ * update and the relaxation -- as one instantiated hipGraph, captured on first
+ * use and replayed while the settings, dt, particle count and buffers stay the
+ * same (anything else re-captures).  The freestream and the Morton order are
+ * set before each replay.  Results are bitwise those of the eager path. */
+int vpmx_set_graph(vpmx_ctx *ctx, int32_t on);
+
 /* ---- instrumentation ----------------------------------------------------------- */
 /* Time every launch of the pairwise kernels with HIP events on the context's
  * stream.  which: 0 = UJ pair kernel, 1 = Estr pair kernel, 2 = fused update,
- * 3 = FMM device evaluation (P2M through P2P).
+ * 3 = FMM device evaluation (P2M through P2P), 4 = graph replay of a whole step
+ * (the kernels inside a graph are not timed one by one).
  * Returns the summed milliseconds and the launch count since the last reset. */
 int vpmx_set_timing(vpmx_ctx *ctx, int32_t on);
 int vpmx_get_timing(vpmx_ctx *ctx, int32_t which, double *total_ms, int64_t *count);

@@ -75,6 +75,8 @@ def lib():
         L.orc_rk3_step.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(_Field),
                                    ctypes.c_double, ctypes.c_int]
         L.orc_euler_step.argtypes = L.orc_rk3_step.argtypes
+        L.orc_ffv_direct.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, ctypes.c_int64,
+                                     _dp, _dp, _dp]
         L.orc_enstrophy.argtypes = [ctypes.c_int64, _dp, _dp]
         L.orc_enstrophy.restype = ctypes.c_double
         L.orc_energy.argtypes = [ctypes.c_int64, _dp, _dp, _dp]
@@ -116,6 +118,17 @@ def uj_direct(kernel, tX, sX, sG, ssig):
     U = np.zeros((nt, 3)); J = np.zeros((nt, 9))
     lib().orc_uj_direct(_kid(kernel), nt, _p(tX), ns, _p(sX), _p(sG), _p(ssig), _p(U), _p(J))
     return U, J
+
+
+def ffv_direct(kernel, bX, bG, bsig, fX, fG):
+    """(nb, 6): Ffv/rho and Fbxf/rho of each bound particle from the free
+    particles (src/FLOWUnsteady_processing_force.jl:879-929)."""
+    bX = _c(bX, (-1, 3)); bG = _c(bG, (-1, 3)); bsig = _c(bsig, (-1,))
+    fX = _c(fX, (-1, 3)); fG = _c(fG, (-1, 3))
+    out = np.zeros((bX.shape[0], 6))
+    lib().orc_ffv_direct(_kid(kernel), bX.shape[0], _p(bX), _p(bG), _p(bsig), fX.shape[0],
+                         _p(fX), _p(fG), _p(out))
+    return out
 
 
 def estr_direct(kernel, transposed, tX, tJ, sX, sG, ssig, sJ):

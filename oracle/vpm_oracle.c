@@ -38,6 +38,10 @@
  *                        corrected variant preserves |Gamma|, omega vs curl of U
  *                        by finite differences.
  *   orc_enstrophy     -- 0.5 sum Gamma.omega (src/FLOWUnsteady_monitors.jl:573-574).
+ *   orc_ffv_direct    -- Ffv part vs sum_f U_b(x_f) x Gamma_f with U_b from
+ *                        orc_uj_direct of the single source b; the Gamma_b x
+ *                        Gamma_f part vs a numpy restatement; both vs the
+ *                        antisymmetry of swapping the roles of b and f.
  */
 #include <math.h>
 #include <stdint.h>
@@ -158,6 +162,46 @@ void orc_uj_direct(int kernel, int64_t nt, const double *tX, int64_t ns, const d
             orc_pair_uj(kernel, tX + 3 * i, sX + 3 * j, sG + 3 * j, ssig[j], u, jj);
         for (int k = 0; k < 3; k++) U[3 * i + k] += u[k];
         for (int k = 0; k < 9; k++) J[9 * i + k] += jj[k];
+    }
+}
+
+/* Free-vortex force of bound vortices on the free field, the loop of
+ * _Ffv_direct (src/FLOWUnsteady_processing_force.jl:879-929): for each bound
+ * particle b (sources) and every free particle f (targets, coincident skipped),
+ * with dX = x_f - x_b and g from r / sigma_b,
+ *   U_b(x_f) = -g/(4 pi r^3) dX x Gamma_b,   out[6b+0..2] += U_b(x_f) x Gamma_f,
+ *   out[6b+3..5] += g/(4 pi r^3) dX x (Gamma_b x Gamma_f).
+ * out (nb x 6) is overwritten (the reference flushes P.M first). */
+void orc_ffv_direct(int kernel, int64_t nb, const double *bX, const double *bG,
+                    const double *bsig, int64_t nf, const double *fX, const double *fG,
+                    double *out)
+{
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t b = 0; b < nb; b++) {
+        double m[6] = {0, 0, 0, 0, 0, 0};
+        const double *xb = bX + 3 * b, *gb = bG + 3 * b;
+        for (int64_t f = 0; f < nf; f++) {
+            const double *xf = fX + 3 * f, *gf = fG + 3 * f;
+            const double d1 = xf[0] - xb[0], d2 = xf[1] - xb[1], d3 = xf[2] - xb[2];
+            const double r = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+            if (r == 0.0) continue;
+            double zeta, g, dg;
+            orc_kernel(kernel, r / bsig[b], &zeta, &g, &dg);
+            const double k = g * ORC_CONST4 / (r * r * r);
+            const double u1 = -k * (d2 * gb[2] - d3 * gb[1]);
+            const double u2 = -k * (d3 * gb[0] - d1 * gb[2]);
+            const double u3 = -k * (d1 * gb[1] - d2 * gb[0]);
+            m[0] += u2 * gf[2] - u3 * gf[1];
+            m[1] += u3 * gf[0] - u1 * gf[2];
+            m[2] += u1 * gf[1] - u2 * gf[0];
+            const double c1 = gb[1] * gf[2] - gb[2] * gf[1];
+            const double c2 = gb[2] * gf[0] - gb[0] * gf[2];
+            const double c3 = gb[0] * gf[1] - gb[1] * gf[0];
+            m[3] += k * (d2 * c3 - d3 * c2);
+            m[4] += k * (d3 * c1 - d1 * c3);
+            m[5] += k * (d1 * c2 - d2 * c1);
+        }
+        for (int k = 0; k < 6; k++) out[6 * b + k] = m[k];
     }
 }
 

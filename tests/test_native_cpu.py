@@ -153,3 +153,27 @@ def test_fmm_plan_matches_oracle(geom, ncrit, theta, cut):
     a = N.fmm_plan_counts(X, s, ncrit, theta, cut)
     _, _, st = O.uj_fmm("gaussianerf", X, G, s, p=1, ncrit=ncrit, theta=theta, sigcut=cut)
     assert a == {k: st[k] for k in ("ncell", "nleaf", "nm2l", "np2p", "p2p_pairs")}
+
+
+def test_fluid_domain_grid_and_preprocessing():
+    """Host logic of the fluid-domain helpers (no GPU call)."""
+    from flowunsteady_amd import processing, vpm
+    P = processing.grid_nodes([0, 0, 0], [1, 2, 3], [1, 2, 3])
+    assert P.shape == (2 * 3 * 4, 3)
+    np.testing.assert_array_equal(P[0], [0, 0, 0])
+    np.testing.assert_array_equal(P[1], [1, 0, 0])      # x fastest
+    np.testing.assert_array_equal(P[2], [0, 1, 0])
+    np.testing.assert_array_equal(P[-1], [1, 2, 3])
+    R = np.array([[0, 1, 0], [-1, 0, 0], [0, 0, 1.0]])
+    np.testing.assert_allclose(processing.grid_nodes([0, 0, 0], [1, 1, 1], [1, 1, 1], R)[1],
+                               [0, 1, 0])
+    pf = vpm.ParticleField(4)
+    vpm.add_particles(pf, np.zeros((4, 3)), np.array([[np.nan, 0, 0], [3, 4, 0], [0, 0, 0],
+                                                      [1e-3, 0, 0]]), [0.5, 0.05, 0.2, 0.01])
+    processing.preprocessing_fluiddomain_pfield(0.1, 1.0)(pf)
+    np.testing.assert_array_equal(vpm.get_sigma(pf), [0.1, 0.05, 0.1, 0.01])
+    G = vpm.get_Gamma(pf)
+    np.testing.assert_array_equal(G[0], [1e-12, 1e-12, 1e-12])
+    np.testing.assert_allclose(G[1], [0.6, 0.8, 0.0])
+    assert np.all(G[2] > 0)            # zero strength lifted off zero
+    np.testing.assert_array_equal(G[3], [1e-3, 0, 0])

@@ -309,3 +309,26 @@ def test_energy_definition():
     G = np.array([[0.0, 1.0, 0]])
     U = np.array([[0.0, 0.0, 2.0]])
     assert O.energy(X, G, U) == pytest.approx(2.0)   # U . (x x Gamma) = 2 * 1
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans", "singular"])
+def test_ffv_direct_through_the_uj_oracle(kernel):
+    """_Ffv_direct (src/FLOWUnsteady_processing_force.jl:879-929) by two other
+    routes through the pinned UJ oracle: Ffv_b = sum_f U_b(x_f) x Gamma_f with
+    U_b the velocity of bound particle b alone; Fbxf_b = the velocity at x_b of
+    the free particles carrying strengths Gamma_b x Gamma_f and core sigma_b."""
+    rng = np.random.default_rng(21)
+    nb, nf = 7, 150
+    bX = rng.uniform(-1, 1, (nb, 3)); bG = rng.normal(0, 1, (nb, 3))
+    bs = rng.uniform(0.05, 0.3, nb)
+    fX = rng.uniform(-1, 1, (nf, 3)); fG = rng.normal(0, 1, (nf, 3))
+    fX[3] = bX[2]                     # a coincident pair is skipped
+    out = O.ffv_direct(kernel, bX, bG, bs, fX, fG)
+    for b in range(nb):
+        Ub, _ = O.uj_direct(kernel, fX, bX[b:b + 1], bG[b:b + 1], bs[b:b + 1])
+        np.testing.assert_allclose(out[b, :3], np.cross(Ub, fG).sum(axis=0), rtol=1e-12,
+                                   atol=1e-12 * np.abs(out[b, :3]).max())
+        Gp = np.cross(bG[b], fG)
+        Uat, _ = O.uj_direct(kernel, bX[b:b + 1], fX, Gp, np.full(nf, bs[b]))
+        np.testing.assert_allclose(out[b, 3:], Uat[0], rtol=1e-12,
+                                   atol=1e-12 * np.abs(out[b, 3:]).max())
