@@ -307,6 +307,83 @@ def cpu_baseline(X, G, s, budget_s=12.0):
                       f"(C restatement of UJ_direct), {t:.1f} s"}
 
 
+def full_step_run(n, steps, log, precisions=("mixed", "fp64"), every=10):
+    """Full rVPM step on the Gaussian blob (N = 2,097,152; X ~ N(0, 0.5^2),
+    Gamma = 0.01 (-y, x, 0) + N(0, 1e-3), sigma = exp(N(ln 0.03, 0.2^2)),
+    vol = sigma^3): FMM p = 8 (ncrit 50, theta 0.4) UJ, stretching, dynamic
+    SFS (alpha 0.999, maxC 1), Pedrizzetti relaxation every step, FP64 RK3,
+    `steps` steps, once per precision of the pair math.  Energy and enstrophy
+    from a fresh UJ every `every` steps; per-stage device times."""
+    from flowunsteady_amd import _native as N
+    from flowunsteady_amd import vpm, workloads
+    L = N.lib()
+    X, G, s, v = workloads.gaussian_blob(n=n, seed=4)
+    res = {"workload": f"Gaussian blob N={n} (std 0.5, swirl 0.01, Gamma noise 1e-3, "
+                       "sigma lognormal(ln 0.03, 0.2)), FMM p=8 ncrit=50 theta=0.4, "
+                       "dynamic SFS, Pedrizzetti relaxation each step, RK3",
+           "N": n, "steps": steps}
+    dt = None
+    for prec in precisions:
+        ctx = N.Context(0)
+        pf = vpm.ParticleField(n, precision=prec, SFS=vpm.DynamicSFS(),
+                               relaxation=vpm.pedrizzetti, fmm=vpm.FMM(p=8, ncrit=50, theta=0.4))
+        vpm.add_particles(pf, X, G, s, vol=v)
+        U = pf._uj_params(sfs=True, fmm=True)
+        Uj = pf._uj_params(fmm=True)
+        S = pf._step_params()
+        ctx.check(L.vpmx_upload(ctx.ptr, pf._ptr(), n, ctypes.byref(pf.layout)))
+
+        def diag():
+            ctx.check(L.vpmx_uj_resident(ctx.ptr, ctypes.byref(Uj)))
+            ens, en = ctypes.c_double(), ctypes.c_double()
+            ctx.check(L.vpmx_diagnostics(ctx.ptr, ctypes.byref(ens), ctypes.byref(en)))
+            return ens.value, en.value
+
+        if dt is None:
+            ctx.check(L.vpmx_uj_resident(ctx.ptr, ctypes.byref(Uj)))
+            ctx.check(L.vpmx_download(ctx.ptr, pf._ptr(), n, ctypes.byref(pf.layout)))
+            umax = float(np.max(np.linalg.norm(vpm.get_U(pf), axis=1)))
+            dt = 0.05 * float(np.median(s)) / umax
+            res["dt"] = dt
+            res["dt_rule"] = "0.05 median(sigma) / max|U| of the initial field"
+        series = [(0,) + diag()]
+        ctx.set_timing(True)
+        wall = 0.0
+        for k in range(1, steps + 1):
+            t0 = time.perf_counter()
+            ctx.check(L.vpmx_rk3_resident(ctx.ptr, ctypes.byref(U), ctypes.byref(S), dt, 1))
+            wall += time.perf_counter() - t0
+            if k % every == 0 or k == steps:
+                ctx.set_timing(False)
+                series.append((k,) + diag())
+                ctx.set_timing(True)
+            log(f"full step {prec} {k}/{steps}: {1e3 * wall / k:.0f} ms/step")
+        split = {name: ctx.timing(w)[0] / steps
+                 for name, w in (("fmm_device_ms", N.T_FMM), ("estr_ms", N.T_ESTR),
+                                 ("update_ms", N.T_UPDATE))}
+        ctx.set_timing(False)
+        st = ctx.fmm_stats()
+        ens0, en0 = series[0][1], series[0][2]
+        res[prec] = {"ms_per_step": 1e3 * wall / steps, "per_step_split": split,
+                     "host_tree_ms_per_uj": st["ms_tree"], "traverse_ms_per_uj": st["ms_traverse"],
+                     "uj_per_step": 7, "estr_per_step": 6,
+                     "enstrophy": [[k, e] for k, e, _ in series],
+                     "energy": [[k, e] for k, _, e in series],
+                     "enstrophy_drift": (series[-1][1] - ens0) / abs(ens0),
+                     "energy_drift": (series[-1][2] - en0) / abs(en0)}
+        log(f"full step {prec}: {res[prec]['ms_per_step']:.0f} ms/step, drift enstrophy "
+            f"{res[prec]['enstrophy_drift']:.3e} energy {res[prec]['energy_drift']:.3e}")
+        ctx.close()
+    if "mixed" in res and "fp64" in res:
+        e_m = np.array([e for _, e in res["mixed"]["enstrophy"]])
+        e_d = np.array([e for _, e in res["fp64"]["enstrophy"]])
+        w_m = np.array([e for _, e in res["mixed"]["energy"]])
+        w_d = np.array([e for _, e in res["fp64"]["energy"]])
+        res["mixed_vs_fp64"] = {"enstrophy_max_rel_diff": float(np.max(np.abs(e_m - e_d) / np.abs(e_d))),
+                                "energy_max_rel_diff": float(np.max(np.abs(w_m - w_d) / np.abs(w_d)))}
+    return res
+
+
 def traffic_probe(n, copy_bytes=1 << 30):
     """One launch of the pair kernel on the bench workload and a plain device
     copy of `copy_bytes`, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs: the
@@ -350,6 +427,10 @@ def main():
     ap.add_argument("--fmm-n", type=int, default=1 << 22)
     ap.add_argument("--fmm-orders", type=str, default="4,8,10")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-fields section")
+    ap.add_argument("--full-step", type=int, default=0, metavar="STEPS",
+                    help="only run the full rVPM step on the 2M Gaussian blob for STEPS steps "
+                         "(mixed and FP64) and print its JSON")
+    ap.add_argument("--full-step-n", type=int, default=1 << 21)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run one pair-kernel launch and a reference copy (for PMC runs)")
     args = ap.parse_args()
@@ -360,6 +441,11 @@ def main():
 
     if args.traffic_probe:
         traffic_probe(args.n)
+        return
+    if args.full_step:
+        log = lambda *a: print(*a, file=sys.stderr, flush=True)
+        print(json.dumps({"full_step": full_step_run(args.full_step_n, args.full_step, log)}),
+              flush=True)
         return
 
     from flowunsteady_amd import _native as N

@@ -27,7 +27,7 @@ class _Field(ctypes.Structure):
     _fields_ = [("np", ctypes.c_int64),
                 ("X", _dp), ("G", _dp), ("sigma", _dp), ("U", _dp), ("J", _dp),
                 ("E", _dp), ("M", _dp), ("C", _dp),
-                ("is_static", ctypes.POINTER(ctypes.c_uint8))]
+                ("is_static", ctypes.POINTER(ctypes.c_uint8)), ("vol", _dp)]
 
 
 class _Params(ctypes.Structure):
@@ -38,7 +38,9 @@ class _Params(ctypes.Structure):
                 ("clip_backscatter", ctypes.c_int),
                 ("relax_kind", ctypes.c_int), ("rlxf", ctypes.c_double),
                 ("alpha", ctypes.c_double), ("sfs_rlxf", ctypes.c_double),
-                ("minC", ctypes.c_double), ("maxC", ctypes.c_double)]
+                ("minC", ctypes.c_double), ("maxC", ctypes.c_double),
+                ("viscous", ctypes.c_int), ("nu", ctypes.c_double), ("sgm0", ctypes.c_double),
+                ("beta", ctypes.c_double), ("itmax", ctypes.c_int), ("tol", ctypes.c_double)]
 
 
 def build(force: bool = False) -> str:
@@ -120,6 +122,73 @@ def uj_direct(kernel, tX, sX, sG, ssig):
     return U, J
 
 
+def _viscous_ready(L):
+    if not hasattr(L, "_viscous_ready"):
+        L.orc_eta.restype = ctypes.c_double
+        L.orc_eta.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.orc_pse.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int64, _dp, _dp, _dp, _dp,
+                              ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _dp]
+        L.orc_zeta_sum.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_int64, _dp, _dp,
+                                   _dp, ctypes.c_double, _dp]
+        L.orc_rbf_cg.restype = ctypes.c_int
+        L.orc_rbf_cg.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_double, _dp, _dp,
+                                 ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_core_spreading_reset.restype = ctypes.c_int
+        L.orc_core_spreading_reset.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(_Field),
+                                               ctypes.POINTER(ctypes.c_int)]
+        L._viscous_ready = True
+    return L
+
+
+def eta(kernel, rho):
+    """eta(rho) = -zeta'(rho)/rho, the PSE kernel."""
+    return _viscous_ready(lib()).orc_eta(_kid(kernel), float(rho))
+
+
+def pse(kernel, nu, X, G, sigma, vol, targets=None):
+    """PSE dGamma/dt (n or len(targets), 3) of the viscous term nu lap(omega)."""
+    X = _c(X, (-1, 3)); G = _c(G, (-1, 3)); sigma = _c(sigma, (-1,)); vol = _c(vol, (-1,))
+    n = X.shape[0]
+    if targets is None:
+        nt, tp = n, None
+    else:
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        nt, tp = len(t), t.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    out = np.zeros((nt, 3))
+    _viscous_ready(lib()).orc_pse(_kid(kernel), float(nu), n, _p(X), _p(G), _p(sigma), _p(vol),
+                                  nt, tp, _p(out))
+    return out
+
+
+def zeta_sum(kernel, tX, sX, sG, ssig=None, s0=0.0):
+    """Vorticity sum_j Gamma_j zeta_s(x - x_j) at tX (s = ssig_j, or s0 when s0 > 0)."""
+    tX = _c(tX, (-1, 3)); sX = _c(sX, (-1, 3)); sG = _c(sG, (-1, 3))
+    ssig = _c(np.zeros(len(sX)) if ssig is None else ssig, (-1,))
+    w = np.zeros((tX.shape[0], 3))
+    _viscous_ready(lib()).orc_zeta_sum(_kid(kernel), tX.shape[0], _p(tX), sX.shape[0], _p(sX),
+                                       _p(sG), _p(ssig), float(s0), _p(w))
+    return w
+
+
+def rbf_cg(kernel, X, s0, w, G0, itmax=20, tol=1e-3):
+    """CG on sum_j G_j zeta_s0(x_i - x_j) = w_i from G0: (G, iterations, residuals)."""
+    X = _c(X, (-1, 3)); w = _c(w, (-1, 3))
+    G = _c(G0, (-1, 3)).copy()
+    res = np.zeros(3)
+    it = _viscous_ready(lib()).orc_rbf_cg(_kid(kernel), X.shape[0], _p(X), float(s0), _p(w), _p(G),
+                                          int(itmax), float(tol), _p(res))
+    return G, it, res
+
+
+def core_spreading_reset(params, field):
+    """Spatial adaptation of core spreading on an oracle Field: (reset?, iterations)."""
+    F = field._struct()
+    it = ctypes.c_int()
+    r = _viscous_ready(lib()).orc_core_spreading_reset(ctypes.byref(params), ctypes.byref(F),
+                                                       ctypes.byref(it))
+    return bool(r), it.value
+
+
 def ffv_direct(kernel, bX, bG, bsig, fX, fG):
     """(nb, 6): Ffv/rho and Fbxf/rho of each bound particle from the free
     particles (src/FLOWUnsteady_processing_force.jl:879-929)."""
@@ -182,10 +251,11 @@ def energy(X, G, U):
 class Field:
     """A particle field for the oracle integrators (AoS float64 arrays)."""
 
-    def __init__(self, X, G, sigma, is_static=None):
+    def __init__(self, X, G, sigma, is_static=None, vol=None):
         self.X = _c(X, (-1, 3)).copy()
         self.G = _c(G, (-1, 3)).copy()
         self.sigma = _c(sigma, (-1,)).copy()
+        self.vol = (self.sigma ** 3 if vol is None else _c(vol, (-1,))).copy()
         n = self.X.shape[0]
         self.U = np.zeros((n, 3)); self.J = np.zeros((n, 9)); self.E = np.zeros((n, 3))
         self.C = np.zeros((n, 3)); self.M = np.zeros((n, 7))
@@ -198,13 +268,16 @@ class Field:
         F.U, F.J, F.E, F.C, F.M = _p(self.U), _p(self.J), _p(self.E), _p(self.C), _p(self.M)
         F.is_static = (self.is_static.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
                        if self.is_static is not None else None)
+        F.vol = _p(self.vol)
         return F
 
 
 def make_params(kernel="gaussianerf", transposed=True, formulation="rVPM", Uinf=(0, 0, 0),
                 sfs=False, Cs=0.0, clip_backscatter=True, relax_kind=0, rlxf=0.3,
-                alpha=0.999, sfs_rlxf=0.005, minC=0.0, maxC=1.0):
-    """sfs: False/0 none, True/1 constant Cs, "dynamic"/2 dynamic procedure."""
+                alpha=0.999, sfs_rlxf=0.005, minC=0.0, maxC=1.0, viscous=0, nu=0.0,
+                sgm0=0.0, beta=1.5, itmax=20, tol=1e-3):
+    """sfs: False/0 none, True/1 constant Cs, "dynamic"/2 dynamic procedure.
+    viscous: 0 inviscid, 1 / "corespreading", 2 / "pse"."""
     P = _Params()
     P.kernel = _kid(kernel)
     P.transposed = int(transposed)
@@ -220,10 +293,14 @@ def make_params(kernel="gaussianerf", transposed=True, formulation="rVPM", Uinf=
     P.Cs = float(Cs); P.clip_backscatter = int(clip_backscatter)
     P.relax_kind = int(relax_kind); P.rlxf = float(rlxf)
     P.alpha, P.sfs_rlxf, P.minC, P.maxC = float(alpha), float(sfs_rlxf), float(minC), float(maxC)
+    P.viscous = {"corespreading": 1, "pse": 2}.get(viscous, viscous)
+    P.nu, P.sgm0, P.beta = float(nu), float(sgm0), float(beta)
+    P.itmax, P.tol = int(itmax), float(tol)
     return P
 
 
 def rk3_step(params, field: Field, dt: float, relax: bool = False):
+    _viscous_ready(lib())
     F = field._struct()
     lib().orc_rk3_step(ctypes.byref(params), ctypes.byref(F), dt, int(relax))
 

@@ -323,6 +323,7 @@ typedef struct {
     double *X, *G, *sigma, *U, *J, *E, *M;    /* M: 7 registers per particle */
     double *C;                                /* 3 per particle: C, <num>, <den> */
     const uint8_t *is_static;                  /* may be NULL: no static particles */
+    const double *vol;                         /* may be NULL: PSE needs it */
 } orc_field;
 
 typedef struct {
@@ -335,7 +336,62 @@ typedef struct {
     int relax_kind;            /* 0 none, 1 pedrizzetti, 2 corrected */
     double rlxf;
     double alpha, sfs_rlxf, minC, maxC;   /* dynamic procedure */
+    int viscous;               /* 0 inviscid, 1 core spreading, 2 PSE */
+    double nu, sgm0, beta;     /* viscosity; core spreading reset size and trigger */
+    int itmax;                 /* core spreading: CG iterations */
+    double tol;                /* core spreading: CG relative residual */
 } orc_params;
+
+/* viscous_oracle.c */
+void orc_pse(int kernel, double nu, int64_t n, const double *X, const double *G,
+             const double *sig, const double *vol, int64_t nt, const int64_t *tidx, double *dG);
+void orc_zeta_sum(int kernel, int64_t nt, const double *tX, int64_t ns, const double *sX,
+                  const double *sG, const double *ssig, double s0, double *w);
+int orc_rbf_cg(int kernel, int64_t n, const double *X, double s0, const double *w, double *G,
+               int itmax, double tol, double *res);
+
+/* Core spreading's spatial adaptation (viscous_oracle.c): when a free particle's
+ * core exceeds beta sgm0, every free particle's core is reset to sgm0 and its
+ * strength re-fitted so that the free particles' vorticity at the free
+ * particles is kept (static particles are left as they are and drop out of
+ * both sides).  Returns 1 when a reset happened. */
+int orc_core_spreading_reset(const orc_params *P, orc_field *F, int *iters)
+{
+    const int64_t n = F->np;
+    int64_t nf = 0;
+    double smax = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        if (F->is_static && F->is_static[i]) continue;
+        nf++;
+        smax = fmax(smax, F->sigma[i]);
+    }
+    if (iters) *iters = 0;
+    if (nf == 0 || !(smax > P->beta * P->sgm0)) return 0;
+    double *X = malloc(sizeof(double) * 3 * nf), *G = malloc(sizeof(double) * 3 * nf),
+           *s = malloc(sizeof(double) * nf), *w = malloc(sizeof(double) * 3 * nf);
+    int64_t *idx = malloc(sizeof(int64_t) * nf);
+    int64_t q = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (F->is_static && F->is_static[i]) continue;
+        idx[q] = i;
+        for (int k = 0; k < 3; k++) {
+            X[3 * q + k] = F->X[3 * i + k];
+            G[3 * q + k] = F->G[3 * i + k];
+        }
+        s[q] = F->sigma[i];
+        q++;
+    }
+    orc_zeta_sum(P->kernel, nf, X, nf, X, G, s, 0.0, w);
+    const int it = orc_rbf_cg(P->kernel, nf, X, P->sgm0, w, G, P->itmax, P->tol, NULL);
+    for (int64_t j = 0; j < nf; j++) {
+        const int64_t i = idx[j];
+        for (int k = 0; k < 3; k++) F->G[3 * i + k] = G[3 * j + k];
+        F->sigma[i] = P->sgm0;
+    }
+    if (iters) *iters = it;
+    free(X); free(G); free(s); free(w); free(idx);
+    return 1;
+}
 
 static void field_uj(const orc_params *P, orc_field *F)
 {
@@ -438,6 +494,26 @@ static void field_relax(const orc_params *P, orc_field *F)
  * particles), then optional relaxation after a fresh UJ.  Stage structure per
  * vpm.nextstep (src/FLOWUnsteady_simulation.jl:358); relax cadence per
  * src/FLOWUnsteady_simulation.jl:346-348. */
+/* Viscous part of one stage's right-hand side: PSE adds to dGamma/dt
+ * (computed on the stage's state), core spreading adds nu / sigma to
+ * dsigma/dt.  V (n x 3) holds the PSE term. */
+static double *viscous_stage(const orc_params *P, const orc_field *F)
+{
+    if (P->viscous != 2) return NULL;
+    double *V = malloc(sizeof(double) * 3 * (F->np > 0 ? F->np : 1));
+    orc_pse(P->kernel, P->nu, F->np, F->X, F->G, F->sigma, F->vol, F->np, NULL, V);
+    return V;
+}
+
+static void add_viscous(const orc_params *P, const double *V, int64_t i, double sigma,
+                        double *dG, double *ds)
+{
+    if (P->viscous == 2)
+        for (int k = 0; k < 3; k++) dG[k] += V[3 * i + k];
+    else if (P->viscous == 1)
+        *ds += P->nu / sigma;
+}
+
 void orc_rk3_step(const orc_params *P, orc_field *F, double dt, int relax)
 {
     double zeta0 = orc_zeta0(P->kernel);
@@ -445,12 +521,14 @@ void orc_rk3_step(const orc_params *P, orc_field *F, double dt, int relax)
     for (int s = 0; s < 3; s++) {
         double a = LSRK3_A[s], b = LSRK3_B[s];
         field_stage(P, F);
+        double *V = viscous_stage(P, F);
         for (int64_t i = 0; i < F->np; i++) {
             if (F->is_static && F->is_static[i]) continue;
             double dX[3], dG[3], ds;
             orc_rhs(P->transposed, P->f, P->g, zeta0, P->Uinf, F->U + 3 * i, F->J + 9 * i,
                     F->G + 3 * i, F->sigma[i], P->sfs ? F->C[3 * i] : 0.0,
                     P->sfs ? F->E + 3 * i : NULL, dX, dG, &ds);
+            add_viscous(P, V, i, F->sigma[i], dG, &ds);
             double *M = F->M + 7 * i;
             for (int k = 0; k < 3; k++) {
                 M[k] = a * M[k] + dt * dX[k];
@@ -463,8 +541,10 @@ void orc_rk3_step(const orc_params *P, orc_field *F, double dt, int relax)
             M[6] = a * M[6] + dt * ds;
             F->sigma[i] += b * M[6];
         }
+        free(V);
     }
     if (relax && P->relax_kind) field_relax(P, F);
+    if (P->viscous == 1) orc_core_spreading_reset(P, F, NULL);
 }
 
 /* euler: one UJ (+SFS), forward-Euler update, then relaxation with that same J
@@ -473,12 +553,14 @@ void orc_euler_step(const orc_params *P, orc_field *F, double dt, int relax)
 {
     double zeta0 = orc_zeta0(P->kernel);
     field_stage(P, F);
+    double *V = viscous_stage(P, F);
     for (int64_t i = 0; i < F->np; i++) {
         if (F->is_static && F->is_static[i]) continue;
         double dX[3], dG[3], ds;
         orc_rhs(P->transposed, P->f, P->g, zeta0, P->Uinf, F->U + 3 * i, F->J + 9 * i,
                 F->G + 3 * i, F->sigma[i], P->sfs ? F->C[3 * i] : 0.0, P->sfs ? F->E + 3 * i : NULL,
                 dX, dG, &ds);
+        add_viscous(P, V, i, F->sigma[i], dG, &ds);
         for (int k = 0; k < 3; k++) {
             F->X[3 * i + k] += dt * dX[k];
             F->G[3 * i + k] += dt * dG[k];
@@ -487,6 +569,8 @@ void orc_euler_step(const orc_params *P, orc_field *F, double dt, int relax)
         if (relax && P->relax_kind)
             orc_relax(P->relax_kind, P->rlxf, F->J + 9 * i, F->G + 3 * i);
     }
+    free(V);
+    if (P->viscous == 1) orc_core_spreading_reset(P, F, NULL);
 }
 
 /* Enstrophy 0.5 sum_p Gamma_p . omega(x_p), omega from J

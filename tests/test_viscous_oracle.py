@@ -1,0 +1,157 @@
+"""Pins of the oracle's viscous schemes (PSE and core spreading with RBF
+adaptation): the defining properties of the published methods, since
+FLOWVPM's implementation is not in the reference tree (parity with it is
+unpinned).  CPU only."""
+import numpy as np
+import pytest
+from scipy import integrate
+
+from oracle import oracle as O
+
+KERNELS = ["gaussian", "gaussianerf", "winckelmans"]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_eta_second_moment_is_one(kernel):
+    """int x_1^2 eta(|x|) d^3x = (4 pi / 3) int rho^4 eta drho = 1: the PSE
+    normalisation that makes (2/s^2)(eta_s * f - f) -> lap f."""
+    val, _ = integrate.quad(lambda r: 4 * np.pi / 3 * r ** 4 * O.eta(kernel, r), 0, np.inf,
+                            limit=400)
+    assert val == pytest.approx(1.0, rel=1e-9)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_pse_conserves_total_circulation(kernel):
+    rng = np.random.default_rng(3)
+    n = 300
+    X = rng.uniform(-0.5, 0.5, (n, 3))
+    G = rng.normal(0, 1, (n, 3))
+    s = rng.uniform(0.05, 0.15, n)
+    vol = rng.uniform(0.5, 2.0, n) * s ** 3
+    dG = O.pse(kernel, 0.01, X, G, s, vol)
+    assert np.abs(dG.sum(axis=0)).max() < 1e-12 * np.abs(dG).max()
+    assert np.abs(dG).max() > 0
+
+
+def lattice(h, half):
+    a = np.arange(-half, half + 1) * h
+    Z, Y, X = np.meshgrid(a, a, a, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+
+
+def gaussian_field(P, L, amp):
+    return np.exp(-np.sum(P ** 2, axis=1) / (2 * L * L))[:, None] * amp[None, :]
+
+
+def smoothed(P, L, s, amp):
+    """(zeta_s * omega)(P) for omega a Gaussian of std L and zeta_s the
+    normalised Gaussian of std s: a Gaussian of std sqrt(L^2 + s^2)."""
+    v = L * L + s * s
+    return (L * L / v) ** 1.5 * np.exp(-np.sum(P ** 2, axis=1) / (2 * v))[:, None] * amp[None, :]
+
+
+def test_pse_and_zeta_sum_on_a_lattice_match_the_continuous_operators():
+    """Particles on a lattice carrying Gamma = omega h^3 of a Gaussian vorticity:
+    zeta_sum is the smoothed field zeta_s * omega and the PSE rate per volume is
+    (2 nu / s^2)(zeta_s * omega - omega), both in closed form (the lattice sum
+    is spectrally accurate for h < s / 1.5)."""
+    L, s, h, nu = 0.2, 0.06, 0.03, 0.01
+    amp = np.array([1.0, 0.5, -0.3])
+    P = lattice(h, 34)                                   # +-1.02
+    G = gaussian_field(P, L, amp) * h ** 3
+    sig = np.full(len(P), s)
+    vol = np.full(len(P), h ** 3)
+    center = np.argsort(np.sum(P ** 2, axis=1))[:40]     # interior targets
+    dG = O.pse("gaussianerf", nu, P, G, sig, vol, targets=center)
+    exact = 2 * nu / s ** 2 * (smoothed(P[center], L, s, amp) - gaussian_field(P[center], L, amp))
+    np.testing.assert_allclose(dG / h ** 3, exact, rtol=1e-9, atol=1e-9 * np.abs(exact).max())
+    w = O.zeta_sum("gaussianerf", P[center], P, G, sig)
+    np.testing.assert_allclose(w, smoothed(P[center], L, s, amp), rtol=1e-9)
+    # and the uniform-core override gives the same
+    w0 = O.zeta_sum("gaussianerf", P[center], P, G, s0=s)
+    np.testing.assert_array_equal(w0, w)
+
+
+def rbf_matrix(kernel, X, s0):
+    n = len(X)
+    A = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros((1, 3))
+        e[0, 0] = 1.0
+        A[:, j] = O.zeta_sum(kernel, X, X[j:j + 1], e, s0=s0)[:, 0]
+    return A
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_rbf_cg_solves_the_interpolation_system(kernel):
+    rng = np.random.default_rng(5)
+    P = lattice(0.1, 2) + rng.uniform(-0.02, 0.02, (125, 3))
+    s0 = 0.08
+    A = rbf_matrix(kernel, P, s0)
+    assert np.allclose(A, A.T) and np.linalg.eigvalsh(A).min() > 0
+    w = rng.normal(0, 1, (125, 3))
+    G, it, res = O.rbf_cg(kernel, P, s0, w, np.zeros((125, 3)), itmax=1000, tol=1e-12)
+    assert res.max() <= 1e-12 and it < 1000
+    np.testing.assert_allclose(G, np.linalg.solve(A, w), rtol=1e-8, atol=1e-8 * np.abs(G).max())
+    # a loose tolerance stops early with the residual it promises
+    G1, it1, res1 = O.rbf_cg(kernel, P, s0, w, np.zeros((125, 3)), itmax=1000, tol=1e-2)
+    assert it1 < it and res1.max() <= 1e-2
+    np.testing.assert_allclose(np.linalg.norm(w - A @ G1, axis=0) / np.linalg.norm(w, axis=0),
+                               res1, rtol=1e-6)
+
+
+def test_core_spreading_growth_is_the_heat_kernel():
+    """A lone particle (no velocity, no stretching): sigma^2 grows by 2 nu t,
+    the Gaussian solution of the heat equation; RK3 of d sigma/dt = nu/sigma
+    reaches it with third-order convergence."""
+    P = O.make_params("gaussianerf", viscous="corespreading", nu=0.01, sgm0=0.1, beta=10.0)
+    errs = []
+    for nst in (5, 10, 20):
+        F = O.Field(np.zeros((1, 3)), np.array([[0.0, 0.0, 1.0]]), np.array([0.1]))
+        for _ in range(nst):
+            O.rk3_step(P, F, 0.25 / nst)
+        errs.append(abs(F.sigma[0] ** 2 - (0.1 ** 2 + 2 * 0.01 * 0.25)))
+        np.testing.assert_array_equal(F.G, [[0.0, 0.0, 1.0]])
+    assert errs[0] < 1e-4 * 0.015
+    assert errs[0] / errs[1] > 7 and errs[1] / errs[2] > 7
+
+
+def test_core_spreading_reset_keeps_the_vorticity_at_the_particles():
+    rng = np.random.default_rng(7)
+    X = lattice(0.05, 3) + rng.uniform(-0.005, 0.005, (343, 3))
+    G = gaussian_field(X, 0.1, np.array([0.0, 0.0, 1.0])) * 0.05 ** 3
+    s = np.full(len(X), 0.09)
+    st = np.zeros(len(X), np.uint8)
+    st[:5] = 1                                         # static particles are left alone
+    F = O.Field(X, G, s, is_static=st)
+    free = st == 0
+    w_before = O.zeta_sum("gaussianerf", X[free], X[free], G[free], s[free])
+    P = O.make_params("gaussianerf", viscous="corespreading", nu=0.01, sgm0=0.04, beta=1.4,
+                      itmax=200, tol=1e-6)
+    did, it = O.core_spreading_reset(P, F)
+    assert did and 0 < it <= 200
+    assert np.all(F.sigma[free] == 0.04) and np.all(F.sigma[~free] == 0.09)
+    np.testing.assert_array_equal(F.G[~free], G[~free])
+    w_after = O.zeta_sum("gaussianerf", X[free], X[free], F.G[free], s0=0.04)
+    assert np.linalg.norm(w_after - w_before) / np.linalg.norm(w_before) <= 1e-6
+    # below the trigger nothing happens
+    P2 = O.make_params("gaussianerf", viscous="corespreading", nu=0.01, sgm0=0.06, beta=1.6)
+    F2 = O.Field(X, G, s)
+    assert O.core_spreading_reset(P2, F2) == (False, 0)
+
+
+def test_pse_enters_the_step_as_an_additive_rate():
+    """Euler step with PSE minus the inviscid step = dt * PSE rate."""
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-0.3, 0.3, (150, 3))
+    G = rng.normal(0, 1e-3, (150, 3))
+    s = rng.uniform(0.05, 0.1, 150)
+    vol = s ** 3
+    Pv = O.make_params("gaussianerf", viscous="pse", nu=0.05)
+    Pi = O.make_params("gaussianerf")
+    Fv, Fi = O.Field(X, G, s, vol=vol), O.Field(X, G, s, vol=vol)
+    O.euler_step(Pv, Fv, 1e-3)
+    O.euler_step(Pi, Fi, 1e-3)
+    np.testing.assert_allclose(Fv.G - Fi.G, 1e-3 * O.pse("gaussianerf", 0.05, X, G, s, vol),
+                               rtol=1e-9, atol=1e-18)
+    np.testing.assert_array_equal(Fv.X, Fi.X)
