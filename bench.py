@@ -233,7 +233,7 @@ def fmm_section(ctx, n, orders, log, cpu=True, nref=1024):
            "N": n, "error_reference": f"direct mixed UJ at {nref} sampled targets", "orders": {}}
     P = N.UJParams()
     P.kernel, P.precision, P.transposed, P.reset = N.GAUSSIANERF, N.MIXED, 1, 1
-    P.fmm, P.ncrit, P.theta, P.sigma_cut = 1, 50, 0.4, 0.0
+    P.fmm, P.ncrit, P.theta, P.sigma_cut = 1, 50, 0.4, -1.0   # core-aware criterion
     for p in orders:
         P.p = p
         ctx.check(N.lib().vpmx_uj_resident(ctx.ptr, ctypes.byref(P)))   # warm (tables)
@@ -319,7 +319,8 @@ def full_step_run(n, steps, log, precisions=("mixed", "fp64"), every=10):
     L = N.lib()
     X, G, s, v = workloads.gaussian_blob(n=n, seed=4)
     res = {"workload": f"Gaussian blob N={n} (std 0.5, swirl 0.01, Gamma noise 1e-3, "
-                       "sigma lognormal(ln 0.03, 0.2)), FMM p=8 ncrit=50 theta=0.4, "
+                       "sigma lognormal(ln 0.03, 0.2)), FMM p=8 ncrit=50 theta=0.4 with "
+                       "FLOWUnsteady's default geometric acceptance (nonzero_sigma=false), "
                        "dynamic SFS, Pedrizzetti relaxation each step, RK3",
            "N": n, "steps": steps}
     dt = None
@@ -348,19 +349,19 @@ def full_step_run(n, steps, log, precisions=("mixed", "fp64"), every=10):
             res["dt_rule"] = "0.05 median(sigma) / max|U| of the initial field"
         series = [(0,) + diag()]
         ctx.set_timing(True)
+        kinds = (("fmm_device_ms", N.T_FMM), ("estr_ms", N.T_ESTR), ("update_ms", N.T_UPDATE))
+        split = {name: 0.0 for name, _ in kinds}
         wall = 0.0
         for k in range(1, steps + 1):
+            before = {name: ctx.timing(w)[0] for name, w in kinds}
             t0 = time.perf_counter()
             ctx.check(L.vpmx_rk3_resident(ctx.ptr, ctypes.byref(U), ctypes.byref(S), dt, 1))
             wall += time.perf_counter() - t0
+            for name, w in kinds:
+                split[name] += (ctx.timing(w)[0] - before[name]) / steps
             if k % every == 0 or k == steps:
-                ctx.set_timing(False)
                 series.append((k,) + diag())
-                ctx.set_timing(True)
             log(f"full step {prec} {k}/{steps}: {1e3 * wall / k:.0f} ms/step")
-        split = {name: ctx.timing(w)[0] / steps
-                 for name, w in (("fmm_device_ms", N.T_FMM), ("estr_ms", N.T_ESTR),
-                                 ("update_ms", N.T_UPDATE))}
         ctx.set_timing(False)
         st = ctx.fmm_stats()
         ens0, en0 = series[0][1], series[0][2]
@@ -431,6 +432,7 @@ def main():
                     help="only run the full rVPM step on the 2M Gaussian blob for STEPS steps "
                          "(mixed and FP64) and print its JSON")
     ap.add_argument("--full-step-n", type=int, default=1 << 21)
+    ap.add_argument("--full-step-precisions", type=str, default="mixed,fp64")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run one pair-kernel launch and a reference copy (for PMC runs)")
     args = ap.parse_args()
@@ -444,8 +446,9 @@ def main():
         return
     if args.full_step:
         log = lambda *a: print(*a, file=sys.stderr, flush=True)
-        print(json.dumps({"full_step": full_step_run(args.full_step_n, args.full_step, log)}),
-              flush=True)
+        prec = tuple(p for p in args.full_step_precisions.split(",") if p)
+        print(json.dumps({"full_step": full_step_run(args.full_step_n, args.full_step, log,
+                                                     precisions=prec)}), flush=True)
         return
 
     from flowunsteady_amd import _native as N

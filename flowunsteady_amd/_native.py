@@ -26,6 +26,8 @@ SFS_NONE, SFS_CONSTANT, SFS_DYNAMIC = 0, 1, 2
 T_UJ, T_ESTR, T_UPDATE, T_FMM, T_GRAPH = 0, 1, 2, 3, 4
 # schedules of the batched pairwise pass
 SCHED_FLAT, SCHED_QUEUE = 0, 1
+# viscous schemes
+VISCOUS_NONE, VISCOUS_CORESPREADING, VISCOUS_PSE = 0, 1, 2
 FMM_STATS = ("ms_tree", "ms_traverse", "ms_upload", "cells", "leaves", "m2l_pairs",
              "p2p_leaf_pairs", "p2p_pairs", "ms_eval")
 
@@ -51,7 +53,9 @@ class StepParams(ctypes.Structure):
                 ("clip_backscatter", ctypes.c_int32), ("relax_kind", ctypes.c_int32),
                 ("rlxf", ctypes.c_double), ("alpha", ctypes.c_double),
                 ("sfs_rlxf", ctypes.c_double), ("minC", ctypes.c_double),
-                ("maxC", ctypes.c_double)]
+                ("maxC", ctypes.c_double), ("viscous", ctypes.c_int32), ("nu", ctypes.c_double),
+                ("sgm0", ctypes.c_double), ("beta", ctypes.c_double), ("itmax", ctypes.c_int32),
+                ("tol", ctypes.c_double)]
 
 
 class VPMXError(RuntimeError):
@@ -122,6 +126,9 @@ SIGNATURES = {
     "vpmx_batch_autotune": (ctypes.c_int, [_ctxp, ctypes.POINTER(UJParams), ctypes.c_int32, _dp,
                                            ctypes.c_int32, _i32p]),
     "vpmx_set_graph": (ctypes.c_int, [_ctxp, ctypes.c_int32]),
+    "vpmx_core_spreading_reset": (ctypes.c_int, [_ctxp, ctypes.POINTER(UJParams),
+                                                 ctypes.POINTER(StepParams), ctypes.c_int32,
+                                                 _i32p, _i32p, _dp]),
     "vpmx_ffv": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64, ctypes.POINTER(Layout), ctypes.c_int32,
                                 _dp]),
     "vpmx_radial_f32": (ctypes.c_int, [ctypes.c_int32, ctypes.c_float, ctypes.c_float,

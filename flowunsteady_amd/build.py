@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvpmx.so")
-SOURCES = ["kernels.hip", "fmm.hip", "batch.hip", "force.hip", "vpmx.hip"]
+SOURCES = ["kernels.hip", "fmm.hip", "batch.hip", "force.hip", "viscous.hip", "vpmx.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -259,6 +259,11 @@ struct MixedAcc {
     float J0 = 0.f, J1 = 0.f, J2 = 0.f, J3 = 0.f, J4 = 0.f, J5 = 0.f, J6 = 0.f, J7 = 0.f;
 };
 
+// Accumulates one pair with radial coefficients A = g/r^3 and
+// B = g'/(sigma r^4) - 3 g / r^5 (U, W = sum A Gamma, radial part of J).
+__device__ __forceinline__ void mixed_accumulate(float dx, float dy, float dz, float A, float B,
+                                                 const float4 b, MixedAcc &acc);
+
 // One pair in FP32.  a = (source position relative to the block origin, p0),
 // b = (Gamma, p1).  When every lane of the wave is in the far field
 // (rho^2 >= 49) the gaussian kernels skip exp/erf: g = 1, g' = 0 to FP32
@@ -298,6 +303,28 @@ __device__ __forceinline__ void mixed_pair_d(float dx, float dy, float dz, float
     }
     A = self ? 0.f : A;
     B = self ? 0.f : B;
+    mixed_accumulate(dx, dy, dz, A, B, b, acc);
+}
+
+// A pair known to lie in the far field: rho^2 >= 49 for the gaussian kernels
+// (g = 1, g' = 0 to FP32 rounding), r > 0 for the singular kernel.  The same
+// operations as the far branch of mixed_pair_d without the per-pair tests.
+template <int K>
+__device__ __forceinline__ void mixed_pair_far(const float4 a, const float4 b, float xi,
+                                               float yi, float zi, MixedAcc &acc)
+{
+    const float dx = xi - a.x, dy = yi - a.y, dz = zi - a.z;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float rinv = frsq(r2);
+    const float ri2 = rinv * rinv;
+    const float A = ri2 * rinv;
+    const float B = -3.0f * A * ri2;
+    mixed_accumulate(dx, dy, dz, A, B, b, acc);
+}
+
+__device__ __forceinline__ void mixed_accumulate(float dx, float dy, float dz, float A, float B,
+                                                 const float4 b, MixedAcc &acc)
+{
     // c = dX x Gamma
     const float c0 = fmaf(dy, b.z, -dz * b.y);
     const float c1 = fmaf(dz, b.x, -dx * b.z);

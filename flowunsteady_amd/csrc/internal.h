@@ -31,6 +31,8 @@ struct DeviceField {
     double *E = nullptr, *C = nullptr, *M = nullptr;   // C: 3 components
     double *D = nullptr;       // dynamic SFS: test-filter stretching and Estr (6)
     double *sig0 = nullptr;    // dynamic SFS: saved sigma
+    double *vol = nullptr;     // particle volume (sigma^3 when the layout has none)
+    double *V = nullptr;       // viscous: PSE rate of Gamma (3), allocated on first use
     uint8_t *is_static = nullptr;
     bool any_static = false;
 };
@@ -42,6 +44,11 @@ struct Scratch {
     double4 *srcE = nullptr;         // FP64 Estr source records, 5 per particle
     double4 *spos = nullptr;         // Morton-sorted (x, y, z, p0) FP64
     float4 *sgam = nullptr;          // Morton-sorted (Gx, Gy, Gz, p1) FP32
+    double4 *tbnd = nullptr;         // per 256-slot tile of the sorted field:
+                                     // (centre, radius), (largest sigma, -, -, -)
+    // split source ranges of small launches: partial sums (nseg x 14 x kcap)
+    int64_t kcap = 0;
+    double *kpart = nullptr;
     uint32_t *kin = nullptr, *kout = nullptr;
     int32_t *vin = nullptr, *perm = nullptr;
     uint8_t *sort_tmp = nullptr;
@@ -53,6 +60,12 @@ struct Scratch {
     double *tX = nullptr, *tU = nullptr, *tJ = nullptr;   // SoA, stride = number of targets
     double4 *tpos = nullptr;         // Morton-sorted targets (x, y, z, 0)
     int32_t *tperm = nullptr;        // sorted slot -> target index
+    // viscous schemes: source records, core-spreading CG vectors (3 x vcap
+    // each), reduction partials and device scalars
+    int64_t vcap = 0;
+    double4 *vrec = nullptr;
+    double *cw = nullptr, *cr = nullptr, *cp = nullptr, *cAp = nullptr;
+    double *cpart = nullptr, *cdev = nullptr;
     // free-vortex force: bound vortex records and per-segment partials
     int64_t bcap = 0, fpcap = 0;
     double *bnd = nullptr, *fpart = nullptr, *fout = nullptr;
@@ -169,13 +182,34 @@ hipError_t launch_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, S
 // every particle reads uinf[0..2]).  Replaces the Uinf argument of launch_update.
 hipError_t launch_update_dev(hipStream_t st, const DeviceField &f, int transposed, int sfs,
                              double fcoef, double gcoef, double zeta0, const double *uinf,
-                             const int32_t *fid, double a, double b, double dt);
+                             const int32_t *fid, double a, double b, double dt, int viscous,
+                             double nu);
 
 // Free-vortex force (force.hip): segments of the free field per bound vortex,
 // then out[6 b + k] for nb bound vortices (bnd: 8 doubles each).
 int ffv_segments(int64_t nb, int64_t nf);
 hipError_t launch_ffv(hipStream_t st, int kernel, const double *bnd, int nb, const DeviceField &f,
                       int nseg, double *part, double *out);
+
+// Viscous schemes (viscous.hip).  PSE rate into f.V; vorticity sums
+// w = sum_j G_j zeta_s(x_i - x_j) with G of stride gld (s = sigma_j, or s0 > 0),
+// static sources masked when mask_static; CG vector steps over 3 components
+// (stride ld) with static entries held fixed.
+hipError_t launch_pse(hipStream_t st, int kernel, const DeviceField &f, double nu, double4 *rec);
+hipError_t launch_zeta_sum(hipStream_t st, int kernel, const DeviceField &f, const double *G,
+                           int64_t gld, double s0, bool mask_static, double4 *rec, double *w,
+                           int64_t wld);
+hipError_t launch_cg_dot(hipStream_t st, const double *a, const double *b, const uint8_t *mask,
+                         int64_t n, int64_t ld, double *part, double *out3);
+hipError_t launch_cg_init(hipStream_t st, const double *w, const double *Ap, const uint8_t *mask,
+                          int64_t n, int64_t ld, double *r, double *p);
+hipError_t launch_cg_step(hipStream_t st, const double *coef, const double *p, const double *Ap,
+                          const uint8_t *mask, int64_t n, int64_t ld, double *G, int64_t gld,
+                          double *r);
+hipError_t launch_cg_dir(hipStream_t st, const double *coef, const double *r, const uint8_t *mask,
+                         int64_t n, int64_t ld, double *p);
+hipError_t launch_max_sigma(hipStream_t st, const DeviceField &f, double *part, double *out);
+hipError_t launch_set_sigma(hipStream_t st, const DeviceField &f, double s0);
 
 // Batched fields (batch.hip).
 size_t batch_sort_temp_bytes(int64_t n);

@@ -60,6 +60,66 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_f64_kernel(const double4 *__restr
     uj_f64_block<K, UJ_BLOCK>(src, 0, ns, X, t0, nt, cap, U, J, reset, tile);
 }
 
+// Split source range: workgroup (block b, segment g) sums sources
+// [g seglen, (g+1) seglen) for target block b into part[(g 12 + k) pld + i].
+template <int K>
+__global__ __launch_bounds__(UJ_BLOCK) void uj_f64_split_kernel(const double4 *__restrict__ src,
+                                                                int64_t ns,
+                                                                const double *__restrict__ X,
+                                                                int64_t nt, int64_t cap,
+                                                                int64_t seglen, int nseg,
+                                                                double *__restrict__ part,
+                                                                int64_t pld)
+{
+    __shared__ double4 tile[2 * UJ_BLOCK];
+    const int g = (int)(blockIdx.x % nseg);
+    const int64_t t0 = (int64_t)(blockIdx.x / nseg) * UJ_BLOCK;
+    const int64_t s0 = (int64_t)g * seglen;
+    const int64_t s1 = s0 + seglen < ns ? s0 + seglen : ns;
+    double d[12];
+    uj_f64_sums<K, UJ_BLOCK>(src, s0, s1, X, t0, nt, cap, tile, d);
+    const int64_t i = t0 + threadIdx.x;
+    if (i < nt)
+        for (int k = 0; k < 12; ++k) part[((int64_t)g * 12 + k) * pld + i] = d[k];
+}
+
+// Segments of each target added in segment order (bitwise repeatable).
+__global__ __launch_bounds__(EW_BLOCK) void uj_f64_split_reduce_kernel(const double *__restrict__ part,
+                                                                       int64_t pld, int nseg,
+                                                                       int64_t nt, int64_t ldo,
+                                                                       double *__restrict__ U,
+                                                                       double *__restrict__ J,
+                                                                       int reset)
+{
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= nt) return;
+    double d[12];
+    for (int k = 0; k < 12; ++k) {
+        double v = 0.0;
+        for (int g = 0; g < nseg; ++g) v += part[((int64_t)g * 12 + k) * pld + i];
+        d[k] = v;
+    }
+    uj_f64_store(d, i, ldo, U, J, reset);
+}
+
+// How a launch over nb target blocks and n sources is split: when the target
+// blocks alone cannot fill the device, the source range is cut into segments
+// (multiples of UJ_BLOCK, at least four tiles each) so that about 1024
+// workgroups run; their partial sums go through `part` (SPLIT_PART doubles).
+struct Split {
+    int nseg;
+    int64_t seglen;
+};
+
+static Split split_for(int64_t nb, int64_t n)
+{
+    if (nb >= 512 || n <= 4 * UJ_BLOCK) return {1, n};
+    int64_t nseg = std::min<int64_t>((1024 + nb - 1) / nb, (n + 4 * UJ_BLOCK - 1) / (4 * UJ_BLOCK));
+    nseg = std::max<int64_t>(nseg, 1);
+    const int64_t seglen = ((n + nseg - 1) / nseg + UJ_BLOCK - 1) / UJ_BLOCK * UJ_BLOCK;
+    return {(int)((n + seglen - 1) / seglen), seglen};
+}
+
 hipError_t launch_pack_f64(hipStream_t st, const DeviceField &f, Scratch &s)
 {
     if (f.np == 0) return hipSuccess;
@@ -68,53 +128,53 @@ hipError_t launch_pack_f64(hipStream_t st, const DeviceField &f, Scratch &s)
     return hipGetLastError();
 }
 
+template <int K>
+static hipError_t uj_f64_go(hipStream_t st, const double4 *src, int64_t ns, const double *X,
+                            int64_t nt, int64_t cap, double *U, double *J, int reset,
+                            double *part)
+{
+    const int64_t nb = (nt + UJ_BLOCK - 1) / UJ_BLOCK;
+    const Split sp = split_for(nb, ns);
+    if (sp.nseg == 1 || !part) {
+        uj_f64_kernel<K><<<(unsigned)nb, UJ_BLOCK, 0, st>>>(src, ns, X, nt, cap, U, J, reset);
+        return hipGetLastError();
+    }
+    uj_f64_split_kernel<K><<<(unsigned)(nb * sp.nseg), UJ_BLOCK, 0, st>>>(src, ns, X, nt, cap,
+                                                                          sp.seglen, sp.nseg,
+                                                                          part, nt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    uj_f64_split_reduce_kernel<<<grid_for(nt, EW_BLOCK), EW_BLOCK, 0, st>>>(part, nt, sp.nseg, nt,
+                                                                           cap, U, J, reset);
+    return hipGetLastError();
+}
+
+static hipError_t uj_f64_any(hipStream_t st, int kernel, const double4 *src, int64_t ns,
+                             const double *X, int64_t nt, int64_t cap, double *U, double *J,
+                             int reset, double *part)
+{
+    switch (kernel) {
+    case SINGULAR: return uj_f64_go<SINGULAR>(st, src, ns, X, nt, cap, U, J, reset, part);
+    case GAUSSIAN: return uj_f64_go<GAUSSIAN>(st, src, ns, X, nt, cap, U, J, reset, part);
+    case GAUSSIANERF: return uj_f64_go<GAUSSIANERF>(st, src, ns, X, nt, cap, U, J, reset, part);
+    case WINCKELMANS: return uj_f64_go<WINCKELMANS>(st, src, ns, X, nt, cap, U, J, reset, part);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_uj_f64_targets(hipStream_t st, int kernel, const DeviceField &f,
                                  const Scratch &s, const double *tX, int64_t nt, double *U,
                                  double *J)
 {
     if (nt == 0 || f.np == 0) return hipSuccess;
-    const unsigned g = grid_for(nt, UJ_BLOCK);
-    switch (kernel) {
-    case SINGULAR:
-        uj_f64_kernel<SINGULAR><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, tX, nt, nt, U, J, 1);
-        break;
-    case GAUSSIAN:
-        uj_f64_kernel<GAUSSIAN><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, tX, nt, nt, U, J, 1);
-        break;
-    case GAUSSIANERF:
-        uj_f64_kernel<GAUSSIANERF><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, tX, nt, nt, U, J, 1);
-        break;
-    case WINCKELMANS:
-        uj_f64_kernel<WINCKELMANS><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, tX, nt, nt, U, J, 1);
-        break;
-    default:
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return uj_f64_any(st, kernel, s.src64, f.np, tX, nt, nt, U, J, 1, s.kpart);
 }
 
 hipError_t launch_uj_f64(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
                          int reset)
 {
     if (f.np == 0) return hipSuccess;
-    const unsigned g = grid_for(f.np, UJ_BLOCK);
-    switch (kernel) {
-    case SINGULAR:
-        uj_f64_kernel<SINGULAR><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset);
-        break;
-    case GAUSSIAN:
-        uj_f64_kernel<GAUSSIAN><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset);
-        break;
-    case GAUSSIANERF:
-        uj_f64_kernel<GAUSSIANERF><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset);
-        break;
-    case WINCKELMANS:
-        uj_f64_kernel<WINCKELMANS><<<g, UJ_BLOCK, 0, st>>>(s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset);
-        break;
-    default:
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return uj_f64_any(st, kernel, s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset, s.kpart);
 }
 
 // ===========================================================================
@@ -209,19 +269,108 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_kernel(const double4 *__res
                                                             const float4 *__restrict__ sgam,
                                                             int64_t n, int64_t cap,
                                                             double *__restrict__ U,
-                                                            double *__restrict__ J, int reset)
+                                                            double *__restrict__ J, int reset,
+                                                            const double4 *__restrict__ tb)
 {
     __shared__ float4 tpos[UJ_BLOCK];
     __shared__ float4 tgam[UJ_BLOCK];
     const int64_t t0 = (int64_t)blockIdx.x * UJ_BLOCK;
-    uj_mixed_block<K, UJ_BLOCK>(tgt, t0, nt, perm, spos, sgam, 0, n, cap, U, J, reset, tpos, tgam);
+    uj_mixed_block<K, UJ_BLOCK>(tgt, t0, nt, perm, spos, sgam, 0, n, cap, U, J, reset, tpos, tgam,
+                                tb);
+}
+
+// Split source range of the mixed kernel: workgroup (block b, segment g) writes
+// its 14 sums to part[(g 14 + k) pld + slot].
+template <int K>
+__global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_split_kernel(const double4 *__restrict__ tgt,
+                                                                  int64_t nt,
+                                                                  const double4 *__restrict__ spos,
+                                                                  const float4 *__restrict__ sgam,
+                                                                  int64_t n, int64_t seglen,
+                                                                  int nseg,
+                                                                  const double4 *__restrict__ tb,
+                                                                  double *__restrict__ part,
+                                                                  int64_t pld)
+{
+    __shared__ float4 tpos[UJ_BLOCK];
+    __shared__ float4 tgam[UJ_BLOCK];
+    const int g = (int)(blockIdx.x % nseg);
+    const int64_t t0 = (int64_t)(blockIdx.x / nseg) * UJ_BLOCK;
+    const int64_t s0 = (int64_t)g * seglen;
+    const int64_t s1 = s0 + seglen < n ? s0 + seglen : n;
+    double d[14];
+    uj_mixed_sums<K, UJ_BLOCK>(tgt, t0, nt, spos, sgam, s0, s1, tb, tpos, tgam, d);
+    const int64_t s = t0 + threadIdx.x;
+    if (s < nt)
+        for (int k = 0; k < 14; ++k) part[((int64_t)g * 14 + k) * pld + s] = d[k];
+}
+
+__global__ __launch_bounds__(EW_BLOCK) void uj_mixed_split_reduce_kernel(
+    const double *__restrict__ part, int64_t pld, int nseg, int64_t nt,
+    const int32_t *__restrict__ perm, int64_t ldo, double *__restrict__ U, double *__restrict__ J,
+    int reset)
+{
+    const int64_t s = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (s >= nt) return;
+    double d[14];
+    for (int k = 0; k < 14; ++k) {
+        double v = 0.0;
+        for (int g = 0; g < nseg; ++g) v += part[((int64_t)g * 14 + k) * pld + s];
+        d[k] = v;
+    }
+    uj_mixed_store(d, perm[s], ldo, U, J, reset);
 }
 
 // Target-subset variant (probes): targets need not be spatially compact, so
 // both target and source carry their offset from the block origin as an FP32
 // (hi, lo) pair and dX = (t_hi - s_hi) + (t_lo - s_lo); the hi difference is
 // exact for the nearby pairs that matter, leaving dX accurate to ~1e-15
-// absolute for any block extent.
+// absolute for any block extent.  Sums of target block t0 over [s0, s1).
+template <int K>
+__device__ __forceinline__ void uj_mixed2_sums(const double4 *__restrict__ tgt, int64_t t0,
+                                               int64_t nt, const double4 *__restrict__ spos,
+                                               const float4 *__restrict__ sgam, int64_t s0,
+                                               int64_t s1, float4 *shi, float4 *slo, float4 *sgm,
+                                               double d[14])
+{
+    const int t = threadIdx.x;
+    const int64_t s = t0 + t;
+    const bool active = s < nt;
+    const double4 o = tgt[t0];
+    float xh = 0.f, yh = 0.f, zh = 0.f, xl = 0.f, yl = 0.f, zl = 0.f;
+    if (active) {
+        const double4 p = tgt[s];
+        const double dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+        xh = (float)dx; yh = (float)dy; zh = (float)dz;
+        xl = (float)(dx - xh); yl = (float)(dy - yh); zl = (float)(dz - zh);
+    }
+    for (int k = 0; k < 14; ++k) d[k] = 0.0;
+    for (int64_t base = s0; base < s1; base += UJ_BLOCK) {
+        __syncthreads();
+        const int64_t js = base + t;
+        if (js < s1) {
+            const double4 p = spos[js];
+            const double dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
+            const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
+            shi[t] = make_float4(hx, hy, hz, (float)p.w);
+            slo[t] = make_float4((float)(dx - hx), (float)(dy - hy), (float)(dz - hz), 0.f);
+            sgm[t] = sgam[js];
+        }
+        __syncthreads();
+        const int cnt = (int)((s1 - base) < UJ_BLOCK ? (s1 - base) : UJ_BLOCK);
+        MixedAcc acc;
+        for (int jj = 0; jj < cnt; ++jj) {
+            const float4 h = shi[jj], l = slo[jj];
+            mixed_pair_d<K>((xh - h.x) + (xl - l.x), (yh - h.y) + (yl - l.y),
+                            (zh - h.z) + (zl - l.z), h.w, sgm[jj], acc);
+        }
+        d[0] += acc.U0; d[1] += acc.U1; d[2] += acc.U2;
+        d[3] += acc.W0; d[4] += acc.W1; d[5] += acc.W2;
+        d[6] += acc.J0; d[7] += acc.J1; d[8] += acc.J2; d[9] += acc.J3;
+        d[10] += acc.J4; d[11] += acc.J5; d[12] += acc.J6; d[13] += acc.J7;
+    }
+}
+
 template <int K>
 __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed2_kernel(const double4 *__restrict__ tgt,
                                                              int64_t nt,
@@ -235,57 +384,86 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed2_kernel(const double4 *__re
     __shared__ float4 shi[UJ_BLOCK];
     __shared__ float4 slo[UJ_BLOCK];
     __shared__ float4 sgm[UJ_BLOCK];
-    const int t = threadIdx.x;
-    const int64_t s0 = (int64_t)blockIdx.x * UJ_BLOCK;
-    const int64_t s = s0 + t;
-    const bool active = s < nt;
-    const double4 o = tgt[s0];
-    float xh = 0.f, yh = 0.f, zh = 0.f, xl = 0.f, yl = 0.f, zl = 0.f;
-    if (active) {
-        const double4 p = tgt[s];
-        const double dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
-        xh = (float)dx; yh = (float)dy; zh = (float)dz;
-        xl = (float)(dx - xh); yl = (float)(dy - yh); zl = (float)(dz - zh);
+    const int64_t t0 = (int64_t)blockIdx.x * UJ_BLOCK;
+    double d[14];
+    uj_mixed2_sums<K>(tgt, t0, nt, spos, sgam, 0, n, shi, slo, sgm, d);
+    const int64_t s = t0 + threadIdx.x;
+    if (s < nt) uj_mixed_store(d, perm[s], ldo, U, J, 1);
+}
+
+template <int K>
+__global__ __launch_bounds__(UJ_BLOCK) void uj_mixed2_split_kernel(const double4 *__restrict__ tgt,
+                                                                   int64_t nt,
+                                                                   const double4 *__restrict__ spos,
+                                                                   const float4 *__restrict__ sgam,
+                                                                   int64_t n, int64_t seglen,
+                                                                   int nseg,
+                                                                   double *__restrict__ part,
+                                                                   int64_t pld)
+{
+    __shared__ float4 shi[UJ_BLOCK];
+    __shared__ float4 slo[UJ_BLOCK];
+    __shared__ float4 sgm[UJ_BLOCK];
+    const int g = (int)(blockIdx.x % nseg);
+    const int64_t t0 = (int64_t)(blockIdx.x / nseg) * UJ_BLOCK;
+    const int64_t s0 = (int64_t)g * seglen;
+    const int64_t s1 = s0 + seglen < n ? s0 + seglen : n;
+    double d[14];
+    uj_mixed2_sums<K>(tgt, t0, nt, spos, sgam, s0, s1, shi, slo, sgm, d);
+    const int64_t s = t0 + threadIdx.x;
+    if (s < nt)
+        for (int k = 0; k < 14; ++k) part[((int64_t)g * 14 + k) * pld + s] = d[k];
+}
+
+// Mixed launches.  two = 1: the two-float target kernel (probes).  tb: tile
+// bounds of the sorted field when the targets are its own slots (self UJ).
+template <int K>
+static hipError_t uj_mixed_go(hipStream_t st, bool two, const double4 *tgt, int64_t nt,
+                              const int32_t *tperm, const double4 *spos, const float4 *sgam,
+                              int64_t n, int64_t ldo, double *U, double *J, int reset,
+                              const double4 *tb, double *part)
+{
+    const int64_t nb = (nt + UJ_BLOCK - 1) / UJ_BLOCK;
+    const Split sp = split_for(nb, n);
+    if (sp.nseg == 1 || !part) {
+        if (two)
+            uj_mixed2_kernel<K><<<(unsigned)nb, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, spos, sgam, n,
+                                                                  ldo, U, J);
+        else
+            uj_mixed_kernel<K><<<(unsigned)nb, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, spos, sgam, n,
+                                                                 ldo, U, J, reset, tb);
+        return hipGetLastError();
     }
-    double dU0 = 0, dU1 = 0, dU2 = 0, dW0 = 0, dW1 = 0, dW2 = 0;
-    double dJ0 = 0, dJ1 = 0, dJ2 = 0, dJ3 = 0, dJ4 = 0, dJ5 = 0, dJ6 = 0, dJ7 = 0;
-    for (int64_t base = 0; base < n; base += UJ_BLOCK) {
-        __syncthreads();
-        const int64_t js = base + t;
-        if (js < n) {
-            const double4 p = spos[js];
-            const double dx = p.x - o.x, dy = p.y - o.y, dz = p.z - o.z;
-            const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
-            shi[t] = make_float4(hx, hy, hz, (float)p.w);
-            slo[t] = make_float4((float)(dx - hx), (float)(dy - hy), (float)(dz - hz), 0.f);
-            sgm[t] = sgam[js];
-        }
-        __syncthreads();
-        const int cnt = (int)((n - base) < UJ_BLOCK ? (n - base) : UJ_BLOCK);
-        MixedAcc acc;
-        for (int jj = 0; jj < cnt; ++jj) {
-            const float4 h = shi[jj], l = slo[jj];
-            mixed_pair_d<K>((xh - h.x) + (xl - l.x), (yh - h.y) + (yl - l.y),
-                            (zh - h.z) + (zl - l.z), h.w, sgm[jj], acc);
-        }
-        dU0 += acc.U0; dU1 += acc.U1; dU2 += acc.U2;
-        dW0 += acc.W0; dW1 += acc.W1; dW2 += acc.W2;
-        dJ0 += acc.J0; dJ1 += acc.J1; dJ2 += acc.J2; dJ3 += acc.J3;
-        dJ4 += acc.J4; dJ5 += acc.J5; dJ6 += acc.J6; dJ7 += acc.J7;
+    const unsigned g = (unsigned)(nb * sp.nseg);
+    if (two)
+        uj_mixed2_split_kernel<K><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, spos, sgam, n, sp.seglen,
+                                                         sp.nseg, part, nt);
+    else
+        uj_mixed_split_kernel<K><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, spos, sgam, n, sp.seglen,
+                                                        sp.nseg, tb, part, nt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    uj_mixed_split_reduce_kernel<<<grid_for(nt, EW_BLOCK), EW_BLOCK, 0, st>>>(
+        part, nt, sp.nseg, nt, tperm, ldo, U, J, two ? 1 : reset);
+    return hipGetLastError();
+}
+
+static hipError_t uj_mixed_any(hipStream_t st, int kernel, bool two, const double4 *tgt,
+                               int64_t nt, const int32_t *tperm, const Scratch &s, int64_t n,
+                               int64_t ldo, double *U, double *J, int reset, const double4 *tb)
+{
+    switch (kernel) {
+    case SINGULAR:
+        return uj_mixed_go<SINGULAR>(st, two, tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset, tb, s.kpart);
+    case GAUSSIAN:
+        return uj_mixed_go<GAUSSIAN>(st, two, tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset, tb, s.kpart);
+    case GAUSSIANERF:
+        return uj_mixed_go<GAUSSIANERF>(st, two, tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset, tb, s.kpart);
+    case WINCKELMANS:
+        return uj_mixed_go<WINCKELMANS>(st, two, tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset, tb, s.kpart);
+    default:
+        return hipErrorInvalidValue;
     }
-    if (!active) return;
-    const int64_t i = perm[s];
-    const double k = -INV4PI;
-    U[i] = k * dU0; U[ldo + i] = k * dU1; U[2 * ldo + i] = k * dU2;
-    J[i] = k * dJ0;
-    J[ldo + i] = k * (dJ1 - dW2);
-    J[2 * ldo + i] = k * (dJ2 + dW1);
-    J[3 * ldo + i] = k * (dJ3 + dW2);
-    J[4 * ldo + i] = k * dJ4;
-    J[5 * ldo + i] = k * (dJ5 - dW0);
-    J[6 * ldo + i] = k * (dJ6 - dW1);
-    J[7 * ldo + i] = k * (dJ7 + dW0);
-    J[8 * ldo + i] = -k * (dJ0 + dJ4);
 }
 
 hipError_t launch_uj_mixed2_targets(hipStream_t st, int kernel, const DeviceField &f,
@@ -293,24 +471,7 @@ hipError_t launch_uj_mixed2_targets(hipStream_t st, int kernel, const DeviceFiel
                                     const int32_t *tperm, int64_t ldo, double *U, double *J)
 {
     if (nt == 0 || f.np == 0) return hipSuccess;
-    const unsigned g = grid_for(nt, UJ_BLOCK);
-    switch (kernel) {
-    case SINGULAR:
-        uj_mixed2_kernel<SINGULAR><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, f.np, ldo, U, J);
-        break;
-    case GAUSSIAN:
-        uj_mixed2_kernel<GAUSSIAN><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, f.np, ldo, U, J);
-        break;
-    case GAUSSIANERF:
-        uj_mixed2_kernel<GAUSSIANERF><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, f.np, ldo, U, J);
-        break;
-    case WINCKELMANS:
-        uj_mixed2_kernel<WINCKELMANS><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, f.np, ldo, U, J);
-        break;
-    default:
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return uj_mixed_any(st, kernel, true, tgt, nt, tperm, s, f.np, ldo, U, J, 1, nullptr);
 }
 
 hipError_t launch_bbox(hipStream_t st, const DeviceField &f, double *part, int *nparts)
@@ -347,6 +508,42 @@ hipError_t launch_sort_mixed(hipStream_t st, const DeviceField &f, Scratch &s)
                                      (unsigned int)n, 0, 30, st);
 }
 
+// Bounds of each 256-slot tile of the sorted field: bbox centre, half
+// diagonal, largest sigma.  One workgroup per tile.
+__global__ __launch_bounds__(UJ_BLOCK) void tile_bounds_kernel(const double4 *__restrict__ spos,
+                                                               const int32_t *__restrict__ perm,
+                                                               const double *__restrict__ sg,
+                                                               int64_t n, double4 *__restrict__ tb)
+{
+    __shared__ double red[7][UJ_BLOCK];
+    const int t = threadIdx.x;
+    const int64_t s = (int64_t)blockIdx.x * UJ_BLOCK + t;
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    if (s < n) {
+        const double4 p = spos[s];
+        v[0] = v[3] = p.x;
+        v[1] = v[4] = p.y;
+        v[2] = v[5] = p.z;
+        v[6] = sg[perm[s]];
+    }
+    for (int k = 0; k < 7; ++k) red[k][t] = v[k];
+    __syncthreads();
+    for (int w = UJ_BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int k = 0; k < 3; ++k) red[k][t] = fmin(red[k][t], red[k][t + w]);
+            for (int k = 3; k < 7; ++k) red[k][t] = fmax(red[k][t], red[k][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double hx = 0.5 * (red[3][0] - red[0][0]), hy = 0.5 * (red[4][0] - red[1][0]),
+                     hz = 0.5 * (red[5][0] - red[2][0]);
+        tb[2 * blockIdx.x] = make_double4(red[0][0] + hx, red[1][0] + hy, red[2][0] + hz,
+                                          sqrt(hx * hx + hy * hy + hz * hz));
+        tb[2 * blockIdx.x + 1] = make_double4(red[6][0], 0.0, 0.0, 0.0);
+    }
+}
+
 hipError_t launch_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, Scratch &s)
 {
     const int64_t n = f.np;
@@ -368,6 +565,10 @@ hipError_t launch_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, S
     default:
         return hipErrorInvalidValue;
     }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    tile_bounds_kernel<<<grid_for(n, UJ_BLOCK), UJ_BLOCK, 0, st>>>(s.spos, s.perm, f.sigma, n,
+                                                                 s.tbnd);
     return hipGetLastError();
 }
 
@@ -380,7 +581,9 @@ hipError_t launch_sort_pack_mixed(hipStream_t st, int kernel, const DeviceField 
 hipError_t launch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
                            int reset)
 {
-    return launch_uj_mixed_targets(st, kernel, f, s, s.spos, f.np, s.perm, f.cap, f.U, f.J, reset);
+    if (f.np == 0) return hipSuccess;
+    return uj_mixed_any(st, kernel, false, s.spos, f.np, s.perm, s, f.np, f.cap, f.U, f.J, reset,
+                        s.tbnd);
 }
 
 hipError_t launch_uj_mixed_targets(hipStream_t st, int kernel, const DeviceField &f,
@@ -388,26 +591,8 @@ hipError_t launch_uj_mixed_targets(hipStream_t st, int kernel, const DeviceField
                                    const int32_t *tperm, int64_t ldo, double *U, double *J,
                                    int reset)
 {
-    const int64_t n = f.np;
     if (nt == 0) return hipSuccess;
-    const unsigned g = grid_for(nt, UJ_BLOCK);
-    switch (kernel) {
-    case SINGULAR:
-        uj_mixed_kernel<SINGULAR><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset);
-        break;
-    case GAUSSIAN:
-        uj_mixed_kernel<GAUSSIAN><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset);
-        break;
-    case GAUSSIANERF:
-        uj_mixed_kernel<GAUSSIANERF><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset);
-        break;
-    case WINCKELMANS:
-        uj_mixed_kernel<WINCKELMANS><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, s.spos, s.sgam, n, ldo, U, J, reset);
-        break;
-    default:
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return uj_mixed_any(st, kernel, false, tgt, nt, tperm, s, f.np, ldo, U, J, reset, nullptr);
 }
 
 // ===========================================================================
@@ -685,6 +870,8 @@ hipError_t launch_fill(hipStream_t st, double *p, int64_t n, double v)
 struct UpdateArgs {
     double a, b, dt, f, g, zeta0, Ux, Uy, Uz;
     int transposed, sfs;
+    int viscous;      // 1 core spreading: dsigma/dt += nu / sigma; 2 PSE: dGamma/dt += V
+    double nu;
 };
 
 __global__ __launch_bounds__(EW_BLOCK) void update_kernel(int64_t n, int64_t cap, UpdateArgs q,
@@ -698,7 +885,8 @@ __global__ __launch_bounds__(EW_BLOCK) void update_kernel(int64_t n, int64_t cap
                                                           double *__restrict__ M,
                                                           const uint8_t *__restrict__ st,
                                                           const double *__restrict__ uinf,
-                                                          const int32_t *__restrict__ fid)
+                                                          const int32_t *__restrict__ fid,
+                                                          const double *__restrict__ V)
 {
     const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -735,9 +923,16 @@ __global__ __launch_bounds__(EW_BLOCK) void update_kernel(int64_t n, int64_t cap
     double Z = 0.0;
     if (Gm2 > 0.0)
         Z = ((q.g + q.f) / (1.0 + 3.0 * q.f) * SG - q.f / (1.0 + 3.0 * q.f) * csfs * EG) / Gm2;
-    const double F[7] = {U[i] + Ux, U[cap + i] + Uy, U[2 * cap + i] + Uz,
-                         S0 - 3.0 * Z * G0 - csfs * e0, S1 - 3.0 * Z * G1 - csfs * e1,
-                         S2 - 3.0 * Z * G2 - csfs * e2, -sig * Z};
+    double F[7] = {U[i] + Ux, U[cap + i] + Uy, U[2 * cap + i] + Uz,
+                   S0 - 3.0 * Z * G0 - csfs * e0, S1 - 3.0 * Z * G1 - csfs * e1,
+                   S2 - 3.0 * Z * G2 - csfs * e2, -sig * Z};
+    if (q.viscous == 2) {            // particle strength exchange
+        F[3] += V[i];
+        F[4] += V[cap + i];
+        F[5] += V[2 * cap + i];
+    } else if (q.viscous == 1) {     // core spreading
+        F[6] += q.nu / sig;
+    }
     double Mq[7];
     for (int k = 0; k < 7; ++k) Mq[k] = q.a * M[k * cap + i] + q.dt * F[k];
     for (int k = 0; k < 7; ++k) M[k * cap + i] = Mq[k];
@@ -755,22 +950,24 @@ hipError_t launch_update(hipStream_t st, const DeviceField &f, int transposed, i
                          double a, double b, double dt)
 {
     if (f.np == 0) return hipSuccess;
-    UpdateArgs q{a, b, dt, fcoef, gcoef, zeta0, Uinf[0], Uinf[1], Uinf[2], transposed, sfs};
+    UpdateArgs q{a, b, dt, fcoef, gcoef, zeta0, Uinf[0], Uinf[1], Uinf[2], transposed, sfs, 0, 0.0};
     update_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(
         f.np, f.cap, q, f.X, f.G, f.sigma, f.U, f.J, f.E, f.C, f.M,
-        f.any_static ? f.is_static : nullptr, nullptr, nullptr);
+        f.any_static ? f.is_static : nullptr, nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_update_dev(hipStream_t st, const DeviceField &f, int transposed, int sfs,
                              double fcoef, double gcoef, double zeta0, const double *uinf,
-                             const int32_t *fid, double a, double b, double dt)
+                             const int32_t *fid, double a, double b, double dt, int viscous,
+                             double nu)
 {
     if (f.np == 0) return hipSuccess;
-    UpdateArgs q{a, b, dt, fcoef, gcoef, zeta0, 0.0, 0.0, 0.0, transposed, sfs};
+    if (viscous == 2 && !f.V) return hipErrorInvalidValue;
+    UpdateArgs q{a, b, dt, fcoef, gcoef, zeta0, 0.0, 0.0, 0.0, transposed, sfs, viscous, nu};
     update_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(
         f.np, f.cap, q, f.X, f.G, f.sigma, f.U, f.J, f.E, f.C, f.M,
-        f.any_static ? f.is_static : nullptr, uinf, fid);
+        f.any_static ? f.is_static : nullptr, uinf, fid, f.V);
     return hipGetLastError();
 }
 

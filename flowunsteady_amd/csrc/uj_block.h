@@ -14,13 +14,11 @@ namespace vpmx {
 // FP64 source record pair (x, y, z, Gx) (Gy, Gz, sigma, 1/sigma), as written by
 // pack_f64_kernel.  Targets [t0, t1) read X (SoA, stride cap); sources
 // [s0, s1).  Reference pair algebra: src/FLOWUnsteady_processing_force.jl:888-905
-// extended to J = grad U.  `tile` holds 2*BLK records.
+// extended to J = grad U.  `tile` holds 2*BLK records.  d = (U0..2, J0..8).
 template <int K, int BLK>
-__device__ __forceinline__ void uj_f64_block(const double4 *__restrict__ src, int64_t s0,
-                                             int64_t s1, const double *__restrict__ X,
-                                             int64_t t0, int64_t t1, int64_t cap,
-                                             double *__restrict__ U, double *__restrict__ J,
-                                             int reset, double4 *tile)
+__device__ __forceinline__ void uj_f64_sums(const double4 *__restrict__ src, int64_t s0,
+                                            int64_t s1, const double *__restrict__ X, int64_t t0,
+                                            int64_t t1, int64_t cap, double4 *tile, double d[12])
 {
     const int t = threadIdx.x;
     const int64_t i = t0 + t;
@@ -74,33 +72,51 @@ __device__ __forceinline__ void uj_f64_block(const double4 *__restrict__ src, in
             }
         }
     }
-    if (active) {
-        if (!reset) {
-            u0 += U[i]; u1 += U[cap + i]; u2 += U[2 * cap + i];
-            j0 += J[i]; j1 += J[cap + i]; j2 += J[2 * cap + i];
-            j3 += J[3 * cap + i]; j4 += J[4 * cap + i]; j5 += J[5 * cap + i];
-            j6 += J[6 * cap + i]; j7 += J[7 * cap + i]; j8 += J[8 * cap + i];
-        }
-        U[i] = u0; U[cap + i] = u1; U[2 * cap + i] = u2;
-        J[i] = j0; J[cap + i] = j1; J[2 * cap + i] = j2;
-        J[3 * cap + i] = j3; J[4 * cap + i] = j4; J[5 * cap + i] = j5;
-        J[6 * cap + i] = j6; J[7 * cap + i] = j7; J[8 * cap + i] = j8;
-    }
+    d[0] = u0; d[1] = u1; d[2] = u2;
+    d[3] = j0; d[4] = j1; d[5] = j2; d[6] = j3; d[7] = j4; d[8] = j5;
+    d[9] = j6; d[10] = j7; d[11] = j8;
 }
 
-// Mixed precision.  Targets are the sorted slots [t0, t1) of `tgt`; sources
-// the sorted slots [s0, s1) of (spos, sgam).  Coordinates are taken relative
-// to the block's first target in FP64 and rounded to FP32; each tile's FP32
-// partials are folded into FP64.  Results go to the target's own index
-// perm[slot] (stride ldo).  `tpos`, `tgam` hold BLK records each.
+// U and J of target i (stride ldo) from the 12 FP64 sums.
+__device__ __forceinline__ void uj_f64_store(const double d[12], int64_t i, int64_t ldo,
+                                             double *__restrict__ U, double *__restrict__ J,
+                                             int reset)
+{
+    for (int k = 0; k < 3; ++k) U[k * ldo + i] = reset ? d[k] : U[k * ldo + i] + d[k];
+    for (int k = 0; k < 9; ++k) J[k * ldo + i] = reset ? d[3 + k] : J[k * ldo + i] + d[3 + k];
+}
+
 template <int K, int BLK>
-__device__ __forceinline__ void uj_mixed_block(const double4 *__restrict__ tgt, int64_t t0,
-                                               int64_t t1, const int32_t *__restrict__ perm,
-                                               const double4 *__restrict__ spos,
-                                               const float4 *__restrict__ sgam, int64_t s0,
-                                               int64_t s1, int64_t ldo,
-                                               double *__restrict__ U, double *__restrict__ J,
-                                               int reset, float4 *tpos, float4 *tgam)
+__device__ __forceinline__ void uj_f64_block(const double4 *__restrict__ src, int64_t s0,
+                                             int64_t s1, const double *__restrict__ X,
+                                             int64_t t0, int64_t t1, int64_t cap,
+                                             double *__restrict__ U, double *__restrict__ J,
+                                             int reset, double4 *tile)
+{
+    double d[12];
+    uj_f64_sums<K, BLK>(src, s0, s1, X, t0, t1, cap, tile, d);
+    const int64_t i = t0 + threadIdx.x;
+    if (i < t1) uj_f64_store(d, i, cap, U, J, reset);
+}
+
+// Mixed precision: the 14 FP64 sums of one target block against sources
+// [s0, s1).  Targets are the sorted slots [t0, t1) of `tgt`; sources the
+// sorted slots of (spos, sgam).  Coordinates are taken relative to the block's
+// first target in FP64 and rounded to FP32; each tile's FP32 partials are
+// folded into FP64.  d = (U0..2, W0..2, J0..7) before the -1/(4 pi) factor.
+//
+// With tile bounds `tb` (per BLK-source tile of the sorted field: (centre,
+// radius), (largest sigma, -, -, -); the block's own targets are tile t0/BLK),
+// a tile whose every pair is in the far field -- gap >= 7.001 sigma_max for the
+// gaussian kernels, gap > 0 for the singular one -- runs mixed_pair_far without
+// per-pair tests, the same arithmetic the per-pair path takes for such pairs.
+// Tiles must then start at multiples of BLK (s0 a multiple of BLK).
+template <int K, int BLK>
+__device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, int64_t t0,
+                                              int64_t t1, const double4 *__restrict__ spos,
+                                              const float4 *__restrict__ sgam, int64_t s0,
+                                              int64_t s1, const double4 *__restrict__ tb,
+                                              float4 *tpos, float4 *tgam, double d[14])
 {
     const int t = threadIdx.x;
     const int64_t s = t0 + t;
@@ -113,8 +129,9 @@ __device__ __forceinline__ void uj_mixed_block(const double4 *__restrict__ tgt, 
         yi = (float)(p.y - o.y);
         zi = (float)(p.z - o.z);
     }
-    double dU0 = 0, dU1 = 0, dU2 = 0, dW0 = 0, dW1 = 0, dW2 = 0;
-    double dJ0 = 0, dJ1 = 0, dJ2 = 0, dJ3 = 0, dJ4 = 0, dJ5 = 0, dJ6 = 0, dJ7 = 0;
+    double4 T = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (tb) T = tb[2 * (t0 / BLK)];
+    for (int k = 0; k < 14; ++k) d[k] = 0.0;
 
     for (int64_t base = s0; base < s1; base += BLK) {
         __syncthreads();
@@ -127,8 +144,24 @@ __device__ __forceinline__ void uj_mixed_block(const double4 *__restrict__ tgt, 
         }
         __syncthreads();
         const int cnt = (int)((s1 - base) < BLK ? (s1 - base) : BLK);
+        bool far = false;
+        if constexpr (K == GAUSSIANERF || K == GAUSSIAN || K == SINGULAR) {
+            if (tb && cnt == BLK) {
+                const double4 S = tb[2 * (base / BLK)];
+                const double ex = S.x - T.x, ey = S.y - T.y, ez = S.z - T.z;
+                const double gap = sqrt(ex * ex + ey * ey + ez * ez) - S.w - T.w;
+                if constexpr (K == SINGULAR)
+                    far = gap > 0.0;
+                else
+                    far = gap >= 7.001 * tb[2 * (base / BLK) + 1].x;
+            }
+        }
         MixedAcc acc;
-        if (cnt == BLK) {
+        if (far) {
+#pragma unroll 4
+            for (int jj = 0; jj < BLK; ++jj)
+                mixed_pair_far<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
+        } else if (cnt == BLK) {
 #pragma unroll 4
             for (int jj = 0; jj < BLK; ++jj)
                 mixed_pair<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
@@ -136,30 +169,51 @@ __device__ __forceinline__ void uj_mixed_block(const double4 *__restrict__ tgt, 
             for (int jj = 0; jj < cnt; ++jj)
                 mixed_pair<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
         }
-        dU0 += acc.U0; dU1 += acc.U1; dU2 += acc.U2;
-        dW0 += acc.W0; dW1 += acc.W1; dW2 += acc.W2;
-        dJ0 += acc.J0; dJ1 += acc.J1; dJ2 += acc.J2; dJ3 += acc.J3;
-        dJ4 += acc.J4; dJ5 += acc.J5; dJ6 += acc.J6; dJ7 += acc.J7;
+        d[0] += acc.U0; d[1] += acc.U1; d[2] += acc.U2;
+        d[3] += acc.W0; d[4] += acc.W1; d[5] += acc.W2;
+        d[6] += acc.J0; d[7] += acc.J1; d[8] += acc.J2; d[9] += acc.J3;
+        d[10] += acc.J4; d[11] += acc.J5; d[12] += acc.J6; d[13] += acc.J7;
     }
-    if (active) {
-        const int64_t i = perm[s];
-        const double k = -INV4PI;
-        double u0 = k * dU0, u1 = k * dU1, u2 = k * dU2;
-        // J = -1/(4 pi) (radial + eps_klm W_m); J22 from the zero trace
-        double j0 = k * dJ0, j1 = k * (dJ1 - dW2), j2 = k * (dJ2 + dW1);
-        double j3 = k * (dJ3 + dW2), j4 = k * dJ4, j5 = k * (dJ5 - dW0);
-        double j6 = k * (dJ6 - dW1), j7 = k * (dJ7 + dW0), j8 = -k * (dJ0 + dJ4);
-        if (!reset) {
-            u0 += U[i]; u1 += U[ldo + i]; u2 += U[2 * ldo + i];
-            j0 += J[i]; j1 += J[ldo + i]; j2 += J[2 * ldo + i];
-            j3 += J[3 * ldo + i]; j4 += J[4 * ldo + i]; j5 += J[5 * ldo + i];
-            j6 += J[6 * ldo + i]; j7 += J[7 * ldo + i]; j8 += J[8 * ldo + i];
-        }
-        U[i] = u0; U[ldo + i] = u1; U[2 * ldo + i] = u2;
-        J[i] = j0; J[ldo + i] = j1; J[2 * ldo + i] = j2;
-        J[3 * ldo + i] = j3; J[4 * ldo + i] = j4; J[5 * ldo + i] = j5;
-        J[6 * ldo + i] = j6; J[7 * ldo + i] = j7; J[8 * ldo + i] = j8;
+}
+
+// U and J of target i from the 14 sums: J = -1/(4 pi) (radial + eps_klm W_m),
+// J22 from the zero trace; reset = 0 adds to what U, J hold.
+__device__ __forceinline__ void uj_mixed_store(const double d[14], int64_t i, int64_t ldo,
+                                               double *__restrict__ U, double *__restrict__ J,
+                                               int reset)
+{
+    const double k = -INV4PI;
+    double u0 = k * d[0], u1 = k * d[1], u2 = k * d[2];
+    double j0 = k * d[6], j1 = k * (d[7] - d[5]), j2 = k * (d[8] + d[4]);
+    double j3 = k * (d[9] + d[5]), j4 = k * d[10], j5 = k * (d[11] - d[3]);
+    double j6 = k * (d[12] - d[4]), j7 = k * (d[13] + d[3]), j8 = -k * (d[6] + d[10]);
+    if (!reset) {
+        u0 += U[i]; u1 += U[ldo + i]; u2 += U[2 * ldo + i];
+        j0 += J[i]; j1 += J[ldo + i]; j2 += J[2 * ldo + i];
+        j3 += J[3 * ldo + i]; j4 += J[4 * ldo + i]; j5 += J[5 * ldo + i];
+        j6 += J[6 * ldo + i]; j7 += J[7 * ldo + i]; j8 += J[8 * ldo + i];
     }
+    U[i] = u0; U[ldo + i] = u1; U[2 * ldo + i] = u2;
+    J[i] = j0; J[ldo + i] = j1; J[2 * ldo + i] = j2;
+    J[3 * ldo + i] = j3; J[4 * ldo + i] = j4; J[5 * ldo + i] = j5;
+    J[6 * ldo + i] = j6; J[7 * ldo + i] = j7; J[8 * ldo + i] = j8;
+}
+
+// One block: sums over [s0, s1) and the store through perm (stride ldo).
+template <int K, int BLK>
+__device__ __forceinline__ void uj_mixed_block(const double4 *__restrict__ tgt, int64_t t0,
+                                               int64_t t1, const int32_t *__restrict__ perm,
+                                               const double4 *__restrict__ spos,
+                                               const float4 *__restrict__ sgam, int64_t s0,
+                                               int64_t s1, int64_t ldo,
+                                               double *__restrict__ U, double *__restrict__ J,
+                                               int reset, float4 *tpos, float4 *tgam,
+                                               const double4 *__restrict__ tb = nullptr)
+{
+    double d[14];
+    uj_mixed_sums<K, BLK>(tgt, t0, t1, spos, sgam, s0, s1, tb, tpos, tgam, d);
+    const int64_t s = t0 + threadIdx.x;
+    if (s < t1) uj_mixed_store(d, perm[s], ldo, U, J, reset);
 }
 
 }  // namespace vpmx

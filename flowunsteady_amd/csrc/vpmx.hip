@@ -134,7 +134,8 @@ int dalloc(vpmx_ctx *c, T **p, size_t count)
 
 void free_field(DeviceField &f)
 {
-    for (double **p : {&f.X, &f.G, &f.sigma, &f.U, &f.J, &f.E, &f.C, &f.M, &f.D, &f.sig0}) {
+    for (double **p : {&f.X, &f.G, &f.sigma, &f.U, &f.J, &f.E, &f.C, &f.M, &f.D, &f.sig0, &f.vol,
+                       &f.V}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
@@ -149,6 +150,8 @@ void free_scratch(Scratch &s)
     if (s.src64) (void)hipFree(s.src64);
     if (s.srcE) (void)hipFree(s.srcE);
     if (s.spos) (void)hipFree(s.spos);
+    if (s.tbnd) (void)hipFree(s.tbnd);
+    if (s.kpart) (void)hipFree(s.kpart);
     if (s.sgam) (void)hipFree(s.sgam);
     if (s.kin) (void)hipFree(s.kin);
     if (s.kout) (void)hipFree(s.kout);
@@ -157,7 +160,8 @@ void free_scratch(Scratch &s)
     if (s.sort_tmp) (void)hipFree(s.sort_tmp);
     if (s.part) (void)hipFree(s.part);
     for (void *p : {(void *)s.tX, (void *)s.tU, (void *)s.tJ, (void *)s.tpos, (void *)s.tperm,
-                    (void *)s.bnd, (void *)s.fpart, (void *)s.fout})
+                    (void *)s.bnd, (void *)s.fpart, (void *)s.fout, (void *)s.vrec, (void *)s.cw,
+                    (void *)s.cr, (void *)s.cp, (void *)s.cAp, (void *)s.cpart, (void *)s.cdev})
         if (p) (void)hipFree(p);
     s = Scratch();
 }
@@ -172,6 +176,7 @@ int ensure_capacity(vpmx_ctx *c, int64_t np)
     if (np <= f.cap) return VPMX_OK;
     int64_t cap = std::max<int64_t>({np, f.cap + f.cap / 2, 1024});
     free_field(f);
+    s.vcap = 0;
     VPMX_TRY(dalloc(c, &f.X, 3 * cap));
     VPMX_TRY(dalloc(c, &f.G, 3 * cap));
     VPMX_TRY(dalloc(c, &f.sigma, cap));
@@ -182,12 +187,18 @@ int ensure_capacity(vpmx_ctx *c, int64_t np)
     VPMX_TRY(dalloc(c, &f.D, 6 * cap));
     VPMX_TRY(dalloc(c, &f.sig0, cap));
     VPMX_TRY(dalloc(c, &f.M, 7 * cap));
+    VPMX_TRY(dalloc(c, &f.vol, cap));
     VPMX_TRY(dalloc(c, &f.is_static, cap));
     f.cap = cap;
     VPMX_TRY(dalloc(c, &s.src64, 2 * cap));
     VPMX_TRY(dalloc(c, &s.srcE, 5 * cap));
     VPMX_TRY(dalloc(c, &s.spos, cap));
     VPMX_TRY(dalloc(c, &s.sgam, cap));
+    VPMX_TRY(dalloc(c, &s.tbnd, 2 * ((cap + 255) / 256)));
+    if (!s.kpart) {
+        s.kcap = 14 * 256 * 1536;   // the most a split launch needs (see split_for)
+        VPMX_TRY(dalloc(c, &s.kpart, s.kcap));
+    }
     VPMX_TRY(dalloc(c, &s.kin, cap));
     VPMX_TRY(dalloc(c, &s.kout, cap));
     VPMX_TRY(dalloc(c, &s.vin, cap));
@@ -195,6 +206,26 @@ int ensure_capacity(vpmx_ctx *c, int64_t np)
     s.sort_tmp_bytes = sort_temp_bytes(cap);
     VPMX_TRY(dalloc(c, &s.sort_tmp, s.sort_tmp_bytes));
     s.cap = cap;
+    c->gen++;
+    return VPMX_OK;
+}
+
+// Buffers of the viscous schemes, sized with the field.
+int ensure_viscous(vpmx_ctx *c)
+{
+    DeviceField &f = c->f;
+    Scratch &s = c->s;
+    if (f.V && s.vcap == f.cap) return VPMX_OK;
+    const int64_t cap = f.cap;
+    VPMX_TRY(dalloc(c, &f.V, 3 * cap));
+    VPMX_TRY(dalloc(c, &s.vrec, 2 * cap));
+    VPMX_TRY(dalloc(c, &s.cw, 3 * cap));
+    VPMX_TRY(dalloc(c, &s.cr, 3 * cap));
+    VPMX_TRY(dalloc(c, &s.cp, 3 * cap));
+    VPMX_TRY(dalloc(c, &s.cAp, 3 * cap));
+    if (!s.cpart) VPMX_TRY(dalloc(c, &s.cpart, 3 * 512));
+    if (!s.cdev) VPMX_TRY(dalloc(c, &s.cdev, 16));
+    s.vcap = cap;
     c->gen++;
     return VPMX_OK;
 }
@@ -390,11 +421,23 @@ int check_step_params(vpmx_ctx *c, const vpmx_step_params *S, double dt)
     }
     if (!isfinite(S->f) || !isfinite(S->g) || 1.0 + 3.0 * S->f == 0.0)
         return fail(c, VPMX_EINVAL, "invalid formulation coefficients f, g");
+    if (S->viscous < VPMX_VISCOUS_NONE || S->viscous > VPMX_VISCOUS_PSE)
+        return fail(c, VPMX_EINVAL, "unknown viscous scheme " + std::to_string(S->viscous));
+    if (S->viscous != VPMX_VISCOUS_NONE && !(S->nu >= 0.0 && isfinite(S->nu)))
+        return fail(c, VPMX_EINVAL, "viscosity nu must be finite and >= 0");
+    if (S->viscous == VPMX_VISCOUS_CORESPREADING) {
+        if (!(S->sgm0 > 0.0) || !isfinite(S->sgm0))
+            return fail(c, VPMX_EINVAL, "core spreading needs a reset core size sgm0 > 0");
+        if (!(S->beta >= 1.0) || !isfinite(S->beta))
+            return fail(c, VPMX_EINVAL, "core spreading needs a reset trigger beta >= 1");
+        if (S->itmax < 0 || !(S->tol > 0.0))
+            return fail(c, VPMX_EINVAL, "core spreading needs itmax >= 0 and tol > 0");
+    }
     return VPMX_OK;
 }
 
 // ---- host <-> device ------------------------------------------------------------------
-// Upload staging (doubles, stride = total np): X[3] G[3] sigma C[3] | U[3] J[9],
+// Upload staging (doubles, stride = total np): X[3] G[3] sigma C[3] vol | U[3] J[9],
 // followed by total np bytes of static flags.  Field k of a batch fills slots
 // [off[k], off[k+1]).
 std::string where(int nf, int k, int64_t p)
@@ -418,7 +461,7 @@ int upload_fields(vpmx_ctx *c, int nf, const double *const *P, const int64_t *np
     B.nfields = 0;
     B.off.assign(2, 0);
     VPMX_TRY(ensure_capacity(c, np));
-    const size_t nd = (size_t)(with_uj ? 22 : 10) * np;
+    const size_t nd = (size_t)(with_uj ? 23 : 11) * np;
     VPMX_TRY(ensure_stage(c, nd * sizeof(double) + np + 8));
     double *h = c->h_stage;
     uint8_t *hs = (uint8_t *)(h + nd);
@@ -442,12 +485,13 @@ int upload_fields(vpmx_ctx *c, int nf, const double *const *P, const int64_t *np
                                                 " (must be positive and finite)");
             h[6 * np + q] = sg;
             for (int k = 0; k < 3; ++k) h[(7 + k) * np + q] = L->C >= 0 ? col[L->C + k] : 0.0;
+            h[10 * np + q] = L->vol >= 0 ? col[L->vol] : sg * sg * sg;
             const uint8_t st = (L->static_ >= 0 && col[L->static_] != 0.0) ? 1 : 0;
             hs[q] = st;
             any_static |= st != 0;
             if (with_uj) {
-                for (int k = 0; k < 3; ++k) h[(10 + k) * np + q] = col[L->U + k];
-                for (int k = 0; k < 9; ++k) h[(13 + k) * np + q] = col[L->J + k];
+                for (int k = 0; k < 3; ++k) h[(11 + k) * np + q] = col[L->U + k];
+                for (int k = 0; k < 9; ++k) h[(14 + k) * np + q] = col[L->J + k];
             }
         }
     }
@@ -465,12 +509,13 @@ int upload_fields(vpmx_ctx *c, int nf, const double *const *P, const int64_t *np
     VPMX_HIP(c, hipMemcpyAsync(f.sigma, h + 6 * np, b, hipMemcpyHostToDevice, st));
     for (int k = 0; k < 3; ++k)
         VPMX_HIP(c, hipMemcpyAsync(f.C + k * f.cap, h + (7 + k) * np, b, hipMemcpyHostToDevice, st));
+    VPMX_HIP(c, hipMemcpyAsync(f.vol, h + 10 * np, b, hipMemcpyHostToDevice, st));
     VPMX_HIP(c, hipMemcpyAsync(f.is_static, hs, (size_t)np, hipMemcpyHostToDevice, st));
     if (with_uj) {
         for (int k = 0; k < 3; ++k)
-            VPMX_HIP(c, hipMemcpyAsync(f.U + k * f.cap, h + (10 + k) * np, b, hipMemcpyHostToDevice, st));
+            VPMX_HIP(c, hipMemcpyAsync(f.U + k * f.cap, h + (11 + k) * np, b, hipMemcpyHostToDevice, st));
         for (int k = 0; k < 9; ++k)
-            VPMX_HIP(c, hipMemcpyAsync(f.J + k * f.cap, h + (13 + k) * np, b, hipMemcpyHostToDevice, st));
+            VPMX_HIP(c, hipMemcpyAsync(f.J + k * f.cap, h + (14 + k) * np, b, hipMemcpyHostToDevice, st));
     }
     if (B.nfields) {
         VPMX_TRY(ensure_batch(c, nf, np));
@@ -658,7 +703,7 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
     const int64_t n = f.np;
     for (double &v : S.stats) v = 0.0;
     if (n == 0) return VPMX_OK;
-    const double sigcut = P->sigma_cut > 0.0 ? P->sigma_cut : default_sigma_cut(P->kernel);
+    const double sigcut = P->sigma_cut < 0.0 ? default_sigma_cut(P->kernel) : P->sigma_cut;
     hipStream_t st = c->stream;
 
     // bounding cube (exact min/max, same arithmetic as the oracle)
@@ -915,15 +960,90 @@ int enqueue_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S
             else
                 VPMX_HIP(c, launch_sfs_coeff(c->stream, f, S->Cs, S->clip_backscatter));
         }
+        if (S->viscous == VPMX_VISCOUS_PSE)
+            VPMX_HIP(c, launch_pse(c->stream, U->kernel, f, S->nu, c->s.vrec));
         TimedScope ts(c, T_UPDATE);
         VPMX_HIP(c, launch_update_dev(c->stream, f, U->transposed, sfs, S->f, S->g, zeta0,
                                       c->b.uinf, fid, rk3 ? RK3_A[s] : 0.0,
-                                      rk3 ? RK3_B[s] : 1.0, dt));
+                                      rk3 ? RK3_B[s] : 1.0, dt, S->viscous, S->nu));
     }
     if (relax && S->relax_kind != VPMX_RELAX_NONE) {
         if (rk3) VPMX_TRY(enqueue_uj(c, U, 1, false));
         VPMX_HIP(c, launch_relax(c->stream, f, S->relax_kind, S->rlxf));
     }
+    return VPMX_OK;
+}
+
+// Core spreading's spatial adaptation on the resident field: when forced or a
+// free core exceeds beta * sgm0, free cores are set to sgm0 and the free
+// strengths re-fitted by conjugate gradients to the free particles' vorticity
+// at the free particles (three independent systems, one per component).
+// Host-driven: every iteration reads its three dot products back.
+int core_spreading_reset(vpmx_ctx *c, int kernel, const vpmx_step_params *S, bool force,
+                         int *did, int *iters, double res[3])
+{
+    DeviceField &f = c->f;
+    Scratch &s = c->s;
+    hipStream_t st = c->stream;
+    if (did) *did = 0;
+    if (iters) *iters = 0;
+    if (res) res[0] = res[1] = res[2] = 0.0;
+    if (f.np == 0) return VPMX_OK;
+    VPMX_TRY(ensure_viscous(c));
+    if (!force) {
+        double smax = 0.0;
+        VPMX_HIP(c, launch_max_sigma(st, f, s.cpart, s.cdev));
+        VPMX_HIP(c, hipMemcpyAsync(&smax, s.cdev, sizeof(double), hipMemcpyDeviceToHost, st));
+        VPMX_TRY(sync(c));
+        if (!(smax > S->beta * S->sgm0)) return VPMX_OK;
+    }
+    const int64_t n = f.np, ld = f.cap;
+    const uint8_t *mask = f.any_static ? f.is_static : nullptr;
+    // right-hand side: the free particles' vorticity with their current cores
+    VPMX_HIP(c, launch_zeta_sum(st, kernel, f, f.G, f.cap, 0.0, true, s.vrec, s.cw, ld));
+    VPMX_HIP(c, launch_set_sigma(st, f, S->sgm0));
+    VPMX_HIP(c, launch_zeta_sum(st, kernel, f, f.G, f.cap, S->sgm0, true, s.vrec, s.cAp, ld));
+    VPMX_HIP(c, launch_cg_init(st, s.cw, s.cAp, mask, n, ld, s.cr, s.cp));
+    double h[6];
+    VPMX_HIP(c, launch_cg_dot(st, s.cr, s.cr, mask, n, ld, s.cpart, s.cdev));
+    VPMX_HIP(c, launch_cg_dot(st, s.cw, s.cw, mask, n, ld, s.cpart, s.cdev + 3));
+    VPMX_HIP(c, hipMemcpyAsync(h, s.cdev, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+    VPMX_TRY(sync(c));
+    double rr[3] = {h[0], h[1], h[2]}, ww[3] = {h[3], h[4], h[5]};
+    bool done[3];
+    for (int k = 0; k < 3; ++k) done[k] = !(sqrt(rr[k]) > S->tol * sqrt(ww[k]));
+    int it = 0;
+    while (it < S->itmax && !(done[0] && done[1] && done[2])) {
+        VPMX_HIP(c, launch_zeta_sum(st, kernel, f, s.cp, ld, S->sgm0, true, s.vrec, s.cAp, ld));
+        double pAp[3];
+        VPMX_HIP(c, launch_cg_dot(st, s.cp, s.cAp, mask, n, ld, s.cpart, s.cdev));
+        VPMX_HIP(c, hipMemcpyAsync(pAp, s.cdev, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+        VPMX_TRY(sync(c));
+        double coef[6];
+        for (int k = 0; k < 3; ++k) coef[k] = done[k] || pAp[k] == 0.0 ? 0.0 : rr[k] / pAp[k];
+        VPMX_HIP(c, hipMemcpyAsync(s.cdev + 8, coef, 3 * sizeof(double), hipMemcpyHostToDevice, st));
+        VPMX_HIP(c, launch_cg_step(st, s.cdev + 8, s.cp, s.cAp, mask, n, ld, f.G, f.cap, s.cr));
+        double rr1[3];
+        VPMX_HIP(c, launch_cg_dot(st, s.cr, s.cr, mask, n, ld, s.cpart, s.cdev));
+        VPMX_HIP(c, hipMemcpyAsync(rr1, s.cdev, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+        VPMX_TRY(sync(c));
+        for (int k = 0; k < 3; ++k) {
+            coef[k] = done[k] || rr[k] == 0.0 ? 0.0 : rr1[k] / rr[k];
+            coef[3 + k] = done[k] ? 0.0 : 1.0;
+        }
+        VPMX_HIP(c, hipMemcpyAsync(s.cdev + 8, coef, 6 * sizeof(double), hipMemcpyHostToDevice, st));
+        VPMX_HIP(c, launch_cg_dir(st, s.cdev + 8, s.cr, mask, n, ld, s.cp));
+        for (int k = 0; k < 3; ++k) {
+            rr[k] = rr1[k];
+            done[k] = done[k] || !(sqrt(rr[k]) > S->tol * sqrt(ww[k]));
+        }
+        ++it;
+    }
+    VPMX_TRY(sync(c));
+    if (did) *did = 1;
+    if (iters) *iters = it;
+    if (res)
+        for (int k = 0; k < 3; ++k) res[k] = ww[k] > 0.0 ? sqrt(rr[k] / ww[k]) : 0.0;
     return VPMX_OK;
 }
 
@@ -945,6 +1065,7 @@ std::vector<unsigned char> step_key(vpmx_ctx *c, const vpmx_uj_params *U,
     k.put(S->f); k.put(S->g); k.put(S->sfs); k.put(S->Cs); k.put(S->clip_backscatter);
     k.put(S->relax_kind); k.put(S->rlxf); k.put(S->alpha); k.put(S->sfs_rlxf);
     k.put(S->minC); k.put(S->maxC);
+    k.put(S->viscous); k.put(S->nu);
     k.put(dt); k.put(relax); k.put(rk3);
     k.put(c->f.np); k.put(c->f.cap); k.put(c->f.any_static); k.put(c->gen);
     const BatchState &B = c->b;
@@ -966,7 +1087,17 @@ int run_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S,
         return fail(c, VPMX_EUNSUPPORTED, "the SFS model does not run on batched fields");
     if (B.nfields && U->fmm)
         return fail(c, VPMX_EUNSUPPORTED, "the FMM does not run on batched fields");
+    if (S->viscous != VPMX_VISCOUS_NONE) {
+        if (B.nfields)
+            return fail(c, VPMX_EUNSUPPORTED, "viscous schemes do not run on batched fields");
+        if (U->kernel == VPMX_KERNEL_SINGULAR)
+            return fail(c, VPMX_EINVAL, "viscous schemes need a smooth kernel");
+        if (S->viscous == VPMX_VISCOUS_CORESPREADING && U->kernel != VPMX_KERNEL_GAUSSIANERF)
+            return fail(c, VPMX_EINVAL,
+                        "core spreading needs the gaussian-erf kernel (its cores solve the heat equation)");
+    }
     if (c->f.np == 0) return VPMX_OK;
+    if (S->viscous != VPMX_VISCOUS_NONE) VPMX_TRY(ensure_viscous(c));
     VPMX_TRY(prepare_batch(c, U->kernel));
     const int nf = std::max(1, B.nfields);
     VPMX_TRY(ensure_batch_fields(c, nf));
@@ -988,6 +1119,10 @@ int run_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S,
     VPMX_HIP(c, hipMemcpyAsync(B.uinf, c->h_small, (size_t)3 * nf * sizeof(double),
                                hipMemcpyHostToDevice, c->stream));
     if (!U->fmm && U->precision == VPMX_MIXED) VPMX_TRY(enqueue_sort(c));
+    if (S->viscous == VPMX_VISCOUS_CORESPREADING) {   // host-driven adaptation after the step
+        VPMX_TRY(enqueue_step(c, U, S, dt, relax, rk3));
+        return core_spreading_reset(c, U->kernel, S, false, nullptr, nullptr, nullptr);
+    }
     if (!c->g.on || U->fmm) return enqueue_step(c, U, S, dt, relax, rk3);
 
     GraphState &G = c->g;
@@ -1652,6 +1787,29 @@ int vpmx_ffv(vpmx_ctx *c, const double *bound, int64_t nb, const vpmx_layout *L,
     VPMX_HIP(c, hipMemcpyAsync(out, s.fout, 6 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost,
                                c->stream));
     return sync(c);
+}
+
+// ---- viscous: core spreading adaptation ------------------------------------------
+int vpmx_core_spreading_reset(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S,
+                              int32_t force, int32_t *did, int32_t *iters, double *res)
+{
+    VPMX_TRY(entry(c));
+    VPMX_TRY(check_uj_params(c, U));
+    VPMX_TRY(check_step_params(c, S, 0.0));
+    if (S->viscous != VPMX_VISCOUS_CORESPREADING)
+        return fail(c, VPMX_EINVAL, "step params do not select core spreading");
+    if (U->kernel != VPMX_KERNEL_GAUSSIANERF)
+        return fail(c, VPMX_EINVAL, "core spreading needs the gaussian-erf kernel");
+    if (c->b.nfields)
+        return fail(c, VPMX_EUNSUPPORTED, "viscous schemes do not run on batched fields");
+    int d = 0, it = 0;
+    double r3[3];
+    VPMX_TRY(core_spreading_reset(c, U->kernel, S, force != 0, &d, &it, r3));
+    if (did) *did = d;
+    if (iters) *iters = it;
+    if (res)
+        for (int k = 0; k < 3; ++k) res[k] = r3[k];
+    return VPMX_OK;
 }
 
 }  // extern "C"

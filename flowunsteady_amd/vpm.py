@@ -164,12 +164,67 @@ def SFS_Cs_nobackscatter(Cs: float = 0.15) -> SubFilterScale:
 # ---- FMM settings (src/FLOWUnsteady_simulation.jl:43) -------------------------------
 @dataclass(frozen=True)
 class FMM:
-    """Expansion order p, leaf size ncrit, acceptance theta; sigma_cut <= 0
-    selects the kernel's default far-field separation in core radii."""
+    """vpm.FMM(; p, ncrit, theta, nonzero_sigma) (src/FLOWUnsteady_simulation.jl:43):
+    expansion order p, leaf size ncrit, acceptance (R_A + R_B) < theta d.
+    nonzero_sigma = False (FLOWUnsteady's default) keeps the criterion purely
+    geometric; True also requires the gap to exceed the kernel's default
+    number of core radii of the source cell's largest sigma (gaussian-erf 7),
+    so no source is expanded while its core still matters.  sigma_cut, when
+    given, sets that number of core radii directly."""
     p: int = 4
     ncrit: int = 50
     theta: float = 0.4
-    sigma_cut: float = 0.0
+    nonzero_sigma: bool = False
+    sigma_cut: Optional[float] = None
+
+    def mac_sigma_cut(self) -> float:
+        if self.sigma_cut is not None:
+            return float(self.sigma_cut)
+        return -1.0 if self.nonzero_sigma else 0.0
+
+
+# ---- viscous schemes (src/FLOWUnsteady_simulation.jl:42,133,229-234) ----------------------
+@dataclass
+class ViscousScheme:
+    """vpm_viscous.  Mutable like FLOWVPM's: run_simulation sets `nu` (and a
+    core spreading's `sgm0`) before building the field (:229-234)."""
+    name: str
+    kind: int
+    nu: float = 0.0
+    sgm0: float = 0.0
+    beta: float = 1.5
+    itmax: int = 20
+    tol: float = 1e-3
+
+
+def Inviscid() -> ViscousScheme:
+    return ViscousScheme("Inviscid", N.VISCOUS_NONE)
+
+
+def ParticleStrengthExchange(nu: float) -> ViscousScheme:
+    """Particle strength exchange: dGamma_i/dt += (2 nu / s^2) sum_j
+    (vol_i Gamma_j - vol_j Gamma_i) eta_s(x_i - x_j), eta = -zeta'/rho."""
+    return ViscousScheme("ParticleStrengthExchange", N.VISCOUS_PSE, float(nu))
+
+
+def CoreSpreading(nu: float, sgm0: float, zeta=None, *, beta: float = 1.5, itmax: int = 20,
+                  tol: float = 1e-3) -> ViscousScheme:
+    """Core spreading (cores grow as d sigma/dt = nu/sigma) with radial-basis
+    function re-fitting when a core exceeds beta * sgm0 (conjugate gradients,
+    itmax iterations, relative residual tol).  `zeta` (vpm.zeta_fmm in the
+    reference) selects the vorticity evaluation there; here the sums are
+    direct.  The reference's examples pass beta=100.0, whose meaning in FLOWVPM
+    is not available here; this trigger is beta * sgm0."""
+    return ViscousScheme("CoreSpreading", N.VISCOUS_CORESPREADING, float(nu), float(sgm0),
+                         float(beta), int(itmax), float(tol))
+
+
+def isinviscid(v: ViscousScheme) -> bool:
+    return v.kind == N.VISCOUS_NONE
+
+
+def iscorespreading(v: ViscousScheme) -> bool:
+    return v.kind == N.VISCOUS_CORESPREADING
 
 
 # ---- contexts: one per device, created on first use -----------------------------------
@@ -197,7 +252,8 @@ class ParticleField:
                  SFS: SubFilterScale = noSFS, integration: Optional[Callable] = None,
                  transposed: bool = True, relaxation: Relaxation = pedrizzetti,
                  fmm: FMM = FMM(), Uinf: Callable = lambda t: np.zeros(3),
-                 precision: str = "fp64", device: int = 0):
+                 precision: str = "fp64", device: int = 0,
+                 viscous: Optional[ViscousScheme] = None):
         if maxparticles < 0:
             raise ValueError("maxparticles must be non-negative")
         self.particles = np.zeros((NFIELDS, maxparticles), dtype=np.float64, order="F")
@@ -213,6 +269,7 @@ class ParticleField:
         self.transposed = transposed
         self.relaxation = relaxation
         self.fmm = fmm
+        self.viscous = viscous if viscous is not None else Inviscid()
         self.Uinf = Uinf
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -233,7 +290,7 @@ class ParticleField:
         P.sfs = int(bool(sfs))
         P.fmm = int(self.UJ is UJ_fmm_gpu) if fmm is None else int(fmm)
         P.p, P.ncrit, P.theta = self.fmm.p, self.fmm.ncrit, self.fmm.theta
-        P.sigma_cut = self.fmm.sigma_cut
+        P.sigma_cut = self.fmm.mac_sigma_cut()
         return P
 
     def _step_params(self) -> N.StepParams:
@@ -249,6 +306,9 @@ class ParticleField:
         S.rlxf = self.relaxation.rlxf
         S.alpha, S.sfs_rlxf = self.SFS.alpha, self.SFS.rlxf
         S.minC, S.maxC = self.SFS.minC, self.SFS.maxC
+        v = self.viscous
+        S.viscous, S.nu, S.sgm0, S.beta = v.kind, v.nu, v.sgm0, v.beta
+        S.itmax, S.tol = v.itmax, v.tol
         return S
 
 
@@ -492,6 +552,24 @@ def set_graphs(on: bool, device: int = 0):
     """Run time steps on the direct scheme as replayed hipGraphs (bitwise the
     same results as eager launches)."""
     _context(device).set_graph(on)
+
+
+def rbf_conjugategradient(pfield: ParticleField, *, force: bool = True):
+    """Core spreading's spatial adaptation on the field (vpm.rbf_conjugategradient):
+    free cores reset to the scheme's sgm0 and strengths re-fitted to the
+    vorticity at the particles.  Returns (reset?, CG iterations, residuals)."""
+    if not iscorespreading(pfield.viscous):
+        raise ValueError("the field's viscous scheme is not CoreSpreading")
+    ctx = _context(pfield.device)
+    L = N.lib()
+    ctx.check(L.vpmx_upload(ctx.ptr, pfield._ptr(), pfield.np, ctypes.byref(pfield.layout)))
+    did, it = ctypes.c_int32(), ctypes.c_int32()
+    res = (ctypes.c_double * 3)()
+    ctx.check(L.vpmx_core_spreading_reset(ctx.ptr, ctypes.byref(pfield._uj_params(fmm=False)),
+                                          ctypes.byref(pfield._step_params()), int(force),
+                                          ctypes.byref(did), ctypes.byref(it), res))
+    ctx.check(L.vpmx_download(ctx.ptr, pfield._ptr(), pfield.np, ctypes.byref(pfield.layout)))
+    return bool(did.value), it.value, list(res)
 
 
 def relax_due(pfield: ParticleField, i: int) -> bool:

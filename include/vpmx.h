@@ -72,6 +72,12 @@ extern "C" {
 #define VPMX_SFS_DYNAMIC 2    /* C from the dynamic procedure (test filter
                                  alpha*sigma, Lagrangian average), clipped   */
 
+/* Viscous schemes: vpm.Inviscid, vpm.CoreSpreading, vpm.ParticleStrengthExchange
+ * (src/FLOWUnsteady_simulation.jl:42). */
+#define VPMX_VISCOUS_NONE 0
+#define VPMX_VISCOUS_CORESPREADING 1
+#define VPMX_VISCOUS_PSE 2
+
 /* Schedules of the batched pairwise pass (vpmx_batch_set_schedule). */
 #define VPMX_SCHED_FLAT 0     /* one workgroup per work item, largest field first  */
 #define VPMX_SCHED_QUEUE 1    /* persistent grid draining a device work queue      */
@@ -108,7 +114,10 @@ typedef struct {
     int32_t p, ncrit;     /* FMM expansion order (0..12) and leaf size      */
     double theta;         /* FMM multipole acceptance: (R_A+R_B) < theta d  */
     double sigma_cut;     /* FMM: M2L also needs a gap of sigma_cut * max
-                             sigma of the source cell; <= 0 selects the
+                             sigma of the source cell.  0: the purely
+                             geometric criterion (FLOWVPM's FMM with
+                             nonzero_sigma = false, FLOWUnsteady's default,
+                             src/FLOWUnsteady_simulation.jl:43); < 0: the
                              kernel's default (gaussian-erf 7, gaussian 3,
                              Winckelmans 20, singular 0)                   */
 } vpmx_uj_params;
@@ -125,6 +134,17 @@ typedef struct {
     /* dynamic SFS (sfs = VPMX_SFS_DYNAMIC): test filter width alpha*sigma,
      * Lagrangian-average weight of the newest sample, bounds of C */
     double alpha, sfs_rlxf, minC, maxC;
+    /* viscous diffusion nu lap(omega) (vpm_viscous, src/FLOWUnsteady_simulation.jl:42,229-234):
+     * VPMX_VISCOUS_PSE adds the particle-strength-exchange rate to dGamma/dt;
+     * VPMX_VISCOUS_CORESPREADING adds nu/sigma to dsigma/dt and, after the
+     * step, when a free particle's core exceeds beta*sgm0, resets every free
+     * core to sgm0 and re-fits the strengths to the vorticity at the particles
+     * by conjugate gradients (itmax iterations, relative residual tol). */
+    int32_t viscous;      /* VPMX_VISCOUS_*                                 */
+    double nu;            /* kinematic viscosity                            */
+    double sgm0, beta;    /* core spreading: reset core size and trigger    */
+    int32_t itmax;        /* core spreading: CG iterations                  */
+    double tol;           /* core spreading: CG relative residual           */
 } vpmx_step_params;
 
 /* ---- context ----------------------------------------------------------------- */
@@ -198,6 +218,20 @@ int64_t vpmx_resident_np(const vpmx_ctx *ctx);
  * FP64, fixed summation order.  kernel: VPMX_KERNEL_*. */
 int vpmx_ffv(vpmx_ctx *ctx, const double *bound, int64_t nb, const vpmx_layout *layout,
              int32_t kernel, double *out);
+
+/* Core spreading's spatial adaptation on the resident field (what
+ * vpm.rbf_conjugategradient provides to vpm.CoreSpreading): when force != 0
+ * or a free particle's core exceeds step->beta * step->sgm0, every free
+ * (non-static) core is set to sgm0 and the free strengths are re-fitted so
+ * that sum_j Gamma'_j zeta_sgm0(x_i - x_j) equals the free particles' vorticity
+ * sum_j Gamma_j zeta_sigma_j(x_i - x_j) at every free particle, by conjugate
+ * gradients from the current strengths (step->itmax, step->tol).  Direct
+ * FP64 sums; the kernel must be gaussian-erf.  *did (optional) = 1 when the
+ * reset ran, *iters the CG iterations, res[3] (optional) the final relative
+ * residual per component. */
+int vpmx_core_spreading_reset(vpmx_ctx *ctx, const vpmx_uj_params *uj,
+                              const vpmx_step_params *step, int32_t force, int32_t *did,
+                              int32_t *iters, double *res);
 
 /* ---- batched fields ------------------------------------------------------------- */
 /* Many independent particle fields of uneven size (a sweep of rotor or wing

@@ -2,9 +2,12 @@
 the schedules and autotuner of the batched pairwise pass, and graph replay of
 time steps.
 
-  FP64 batch vs each field alone (vpmx_uj per field): bitwise, any schedule.
-  Mixed batch at block 256 vs each field alone: bitwise (same Morton order,
-  same blocks, same tiles).  Other blocks: within the mixed tolerance of FP64.
+  FP64 batch vs each field alone (vpmx_uj per field): <= 1e-14 at any schedule
+  (a field alone small enough to leave the device idle splits its source range
+  across workgroups, which only regroups the FP64 sums).
+  Mixed batch at block 256 vs each field alone: <= 1e-13 (same Morton order,
+  blocks and FP32 tiles; the FP64 folds regroup as above).  Other blocks:
+  within the mixed tolerance of FP64.
   Batched RK3 with a freestream per field vs the oracle's rungekutta3 per field:
   <= 1e-12.  Graph replay vs eager launches: bitwise, every replay.
 """
@@ -51,7 +54,7 @@ def _restore_schedule():
 
 @pytest.mark.parametrize("sched,block", SCHEDULES)
 @pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
-def test_batch_fp64_equals_each_field_alone(sched, block, kernel):
+def test_batch_fp64_matches_each_field_alone(sched, block, kernel):
     cases = sweep()
     alone = fields(cases, kernel=kernel)
     for pf in alone:
@@ -60,8 +63,11 @@ def test_batch_fp64_equals_each_field_alone(sched, block, kernel):
     ctx().set_schedule(sched, block)
     vpm.UJ_gpu_batch(batch)
     for a, b in zip(alone, batch):
-        np.testing.assert_array_equal(vpm.get_U(b), vpm.get_U(a))
-        np.testing.assert_array_equal(vpm.get_J(b), vpm.get_J(a))
+        if a.np < 2:
+            np.testing.assert_array_equal(vpm.get_U(b), vpm.get_U(a))
+            continue
+        assert normwise(vpm.get_U(b), vpm.get_U(a))[0] < 1e-14
+        assert normwise(vpm.get_J(b), vpm.get_J(a))[0] < 1e-14
     # and the oracle on one small field
     X, G, s, _ = cases[1]
     Ur, Jr = O.uj_direct(kernel, X, X, G, s)
@@ -70,7 +76,7 @@ def test_batch_fp64_equals_each_field_alone(sched, block, kernel):
 
 
 @pytest.mark.parametrize("sched", [N.SCHED_FLAT, N.SCHED_QUEUE])
-def test_batch_mixed_block256_equals_each_field_alone(sched):
+def test_batch_mixed_block256_matches_each_field_alone(sched):
     cases = sweep()
     alone = fields(cases, precision="mixed")
     for pf in alone:
@@ -79,8 +85,11 @@ def test_batch_mixed_block256_equals_each_field_alone(sched):
     ctx().set_schedule(sched, 256)
     vpm.UJ_gpu_batch(batch)
     for a, b in zip(alone, batch):
-        np.testing.assert_array_equal(vpm.get_U(b), vpm.get_U(a))
-        np.testing.assert_array_equal(vpm.get_J(b), vpm.get_J(a))
+        if a.np < 2:
+            np.testing.assert_array_equal(vpm.get_U(b), vpm.get_U(a))
+            continue
+        assert normwise(vpm.get_U(b), vpm.get_U(a))[0] < 1e-13
+        assert normwise(vpm.get_J(b), vpm.get_J(a))[0] < 1e-13
 
 
 @pytest.mark.parametrize("sched,block", SCHEDULES)

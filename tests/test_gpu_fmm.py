@@ -23,6 +23,7 @@ GEOMS = {
 
 
 def fmm_field(X, G, s, precision="fp64", kernel="gaussianerf", **fmm):
+    fmm.setdefault("nonzero_sigma", True)
     pf = make_field(X, G, s, kernel=kernel, precision=precision, fmm=vpm.FMM(**fmm))
     pf.UJ = vpm.UJ_fmm_gpu
     return pf
@@ -108,7 +109,8 @@ def test_fmm_coincident_cluster():
 def test_fmm_rk3_step_close_to_direct():
     X, G, s, _ = workloads.vortex_ring()
     a = make_field(X, G, s, relaxation=vpm.pedrizzetti)
-    b = make_field(X, G, s, relaxation=vpm.pedrizzetti, fmm=vpm.FMM(p=10, ncrit=50, theta=0.4))
+    b = make_field(X, G, s, relaxation=vpm.pedrizzetti,
+                   fmm=vpm.FMM(p=10, ncrit=50, theta=0.4, nonzero_sigma=True))
     b.UJ = vpm.UJ_fmm_gpu
     ctx = vpm._context(0)
     for pf in (a, b):
@@ -119,3 +121,20 @@ def test_fmm_rk3_step_close_to_direct():
                                         ctypes.byref(P), ctypes.byref(S), 0.01, 1))
     assert normwise(vpm.get_X(b), vpm.get_X(a))[0] < 1e-6
     assert normwise(vpm.get_Gamma(b), vpm.get_Gamma(a))[0] < 1e-5
+
+
+def test_geometric_criterion_matches_the_oracle():
+    """nonzero_sigma = False (FLOWUnsteady's default FMM): the purely geometric
+    acceptance, same tree and lists as the oracle's FMM with sigcut = 0."""
+    X, G, s, _ = GEOMS["blob"]()
+    pf = fmm_field(X, G, s, p=6, ncrit=50, theta=0.4, nonzero_sigma=False)
+    U, J = run(pf)
+    st = vpm._context(0).fmm_stats()
+    Uo, Jo, so = O.uj_fmm("gaussianerf", X, G, s, p=6, ncrit=50, theta=0.4, sigcut=0.0)
+    assert int(st["m2l_pairs"]) == so["nm2l"]
+    assert normwise(U, Uo)[0] < 1e-12 and normwise(J, Jo)[0] < 1e-12
+    # the core-aware criterion only refuses acceptances: never fewer near pairs
+    pf7 = fmm_field(X, G, s, p=6, ncrit=50, theta=0.4, nonzero_sigma=True)
+    run(pf7)
+    assert vpm._context(0).fmm_stats()["p2p_pairs"] >= st["p2p_pairs"]
+

@@ -270,3 +270,25 @@ def test_targets_match_appended_probes():
     vpm.UJ_gpu(pf)
     Up = vpm.get_U(pf)[len(X):]
     assert normwise(Ut, Up)[0] < 1e-13
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "gaussian"])
+@pytest.mark.parametrize("n", [1 << 18, 60000])
+def test_mixed_far_tiles_and_split_sources(kernel, n):
+    """Mixed vs FP64 on fields large enough that most source tiles are skipped
+    past the per-pair tests (all pairs in the far field), with (60k: the
+    source range split across workgroups) and without (256k) split sources."""
+    X, G, s, v = workloads.uniform_cube(n=n, seed=12)
+    a = make_field(X, G, s, kernel=kernel, precision="fp64")
+    b = make_field(X, G, s, kernel=kernel, precision="mixed")
+    U64, J64 = run_uj(a)
+    Umx, Jmx = run_uj(b)
+    eu, ej = normwise(Umx, U64), normwise(Jmx, J64)
+    print(f"cube{n} {kernel}: U {eu}, J {ej}")
+    # fields sparser than the 1M cube put more pairs in the near-field branch,
+    # whose FP32 radial functions carry up to ~2.5e-6 relative error
+    tu, tj = {"gaussianerf": (2.5e-6, 3e-6), "gaussian": (4e-6, 5e-6)}[kernel]
+    assert eu[0] < tu and ej[0] < tj
+    idx = sample_targets(n, 256, seed=4)
+    Ur, Jr = oracle_uj(kernel, X, G, s, idx)
+    assert normwise(U64[idx], Ur)[0] < 1e-12 and normwise(J64[idx], Jr)[0] < 1e-12

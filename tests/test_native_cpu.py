@@ -46,6 +46,8 @@ int main(void) {
   printf("%zu %zu %zu\n", sizeof(vpmx_layout), sizeof(vpmx_uj_params), sizeof(vpmx_step_params));
   printf("%zu %zu %zu %zu\n", offsetof(vpmx_uj_params, theta), offsetof(vpmx_step_params, Uinf),
          offsetof(vpmx_step_params, Cs), offsetof(vpmx_step_params, rlxf));
+  printf("%zu %zu %zu\n", offsetof(vpmx_step_params, maxC), offsetof(vpmx_step_params, nu),
+         offsetof(vpmx_step_params, tol));
   return 0;
 }''')
     exe = tmp_path / "probe"
@@ -55,7 +57,8 @@ int main(void) {
     got = [int(v) for v in out]
     exp = [ctypes.sizeof(N.Layout), ctypes.sizeof(N.UJParams), ctypes.sizeof(N.StepParams),
            N.UJParams.theta.offset, N.StepParams.Uinf.offset, N.StepParams.Cs.offset,
-           N.StepParams.rlxf.offset]
+           N.StepParams.rlxf.offset, N.StepParams.maxC.offset, N.StepParams.nu.offset,
+           N.StepParams.tol.offset]
     assert got == exp
 
 
