@@ -13,6 +13,41 @@ namespace vpmx {
 
 constexpr int FMM_MAX_P = 12;   // K = 455 <= 2 coefficients per thread of 256
 
+// M2L on the matrix cores (orders 1..M2L_MFMA_MAX_P): the operator of one
+// source, T[b][a] = E_{a+b} on the region |a| + |b| <= P, cut into 4 x 4 tiles
+// (rows b = 4 tb + 0..3, columns a = 4 ta + 0..3).  Coefficients are in graded
+// order, so the row length of b -- the number of a with |a| <= P - |b| -- only
+// falls with b: b-tile tb needs a-tiles 0 .. m2l_natile(P, tb) - 1, and a-tile
+// ta meets b-tiles 0 .. m2l_nrows(P, ta) - 1.  Tiles are numbered a-tile by
+// a-tile: tile (tb, ta) is m2l_base(P, ta) + tb.
+constexpr int M2L_MFMA_MAX_P = 10;
+constexpr int m2l_ncoef(int p) { return p < 0 ? 0 : (p + 1) * (p + 2) * (p + 3) / 6; }
+constexpr int m2l_degree(int k)
+{
+    int n = 0;
+    while (m2l_ncoef(n) <= k) ++n;
+    return n;
+}
+constexpr int m2l_ntile1(int P) { return (m2l_ncoef(P) + 3) / 4; }
+constexpr int m2l_natile(int P, int tb)
+{
+    return 4 * tb < m2l_ncoef(P) ? (m2l_ncoef(P - m2l_degree(4 * tb)) + 3) / 4 : 0;
+}
+constexpr int m2l_nrows(int P, int ta)
+{
+    int n = 0;
+    for (int tb = 0; tb < m2l_ntile1(P); ++tb)
+        if (m2l_natile(P, tb) > ta) ++n;
+    return n;
+}
+constexpr int m2l_base(int P, int ta)
+{
+    int s = 0;
+    for (int t = 0; t < ta; ++t) s += m2l_nrows(P, t);
+    return s;
+}
+constexpr int m2l_ntiles(int P) { return m2l_base(P, m2l_ntile1(P)); }
+
 struct FmmCellDev {
     int32_t begin, end, child0, nchild;
     double cx, cy, cz, R, smax;
@@ -55,6 +90,12 @@ struct FmmTables {
     // of b; m2l_rounds = chunks per thread of a 256-thread block (1, 2, 4 or 8).
     std::vector<int> m2l_chunk, m2l_cfirst;
     int m2l_nchunk = 0, m2l_rounds = 1, m2l_chunk_len = 0;
+    // Matrix-core M2L (P <= M2L_MFMA_MAX_P): per tile, for (row, k) at
+    // row + 4 k, the index of a + b in E, or K (a zero slot) outside the region;
+    // bscale[a] = (-1)^|a| / a!, lscale[b] = 1 / b!.
+    std::vector<uint16_t> m2l_tile_idx;
+    std::vector<double> m2l_bscale, m2l_lscale;
+    std::vector<int> m2l_erec;   // 4 per k, packed as m2l_mfma_kernel reads them
 };
 
 // Tree + lists of one evaluation (host).
@@ -87,6 +128,9 @@ struct FmmDev {
     const int4 *m2l_chunks = nullptr;
     const int *m2l_cfirst = nullptr;
     int m2l_nchunk = 0, m2l_rounds = 1;
+    const uint16_t *m2l_tile_idx = nullptr;
+    const double *m2l_bscale = nullptr, *m2l_lscale = nullptr;
+    const int4 *m2l_erec = nullptr;
 };
 
 // Per-context FMM state: host plan, operator tables and device buffers.
@@ -101,6 +145,9 @@ struct FmmState {
     double *m2m_coef = nullptr, *l2l_coef = nullptr, *m2l_tcoef = nullptr;
     int4 *m2l_chunks = nullptr;
     int *m2l_cfirst = nullptr;
+    uint16_t *m2l_tile_idx = nullptr;
+    double *m2l_bscale = nullptr, *m2l_lscale = nullptr;
+    int4 *m2l_erec = nullptr;
     // device tree and lists
     int64_t cap_cells = 0, cap_m2l = 0, cap_p2p = 0, cap_coef = 0, cap_n = 0;
     FmmCellDev *cells = nullptr;

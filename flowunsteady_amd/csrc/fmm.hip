@@ -365,6 +365,184 @@ __global__ __launch_bounds__(FMM_BLOCK) void m2l_kernel(const FmmCellDev *__rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// M2L on the matrix cores
+// ---------------------------------------------------------------------------
+// With E_k = d^k (1/|r|) (the raw derivatives, k! times the Taylor
+// coefficients), m'_a = (-1)^|a| m_a / a! and l'_b = b! l_b the translation is
+//     l'_b = sum_a E_{a+b} m'_a,   |a| + |b| <= P,
+// a K x K matrix (a Hankel matrix in the multi-index) times the 3 columns of
+// the source's multipole.  One wave owns one target cell and takes its sources
+// four at a time, one per block of v_mfma_f64_4x4x4f64 (four independent
+// 4 x 4 x 4 products): the A operand of block j is a tile of source j's matrix,
+// gathered from its E in LDS through the tile index table; B is the source's
+// m' (columns q = 0, 1, 2 and a zero fourth column); C accumulates l' of the
+// target over every source of the block, one 4 x 4 tile per b-tile, in
+// registers.  At the end the four blocks are added (fixed order) and scaled
+// by 1/b!.
+//
+// Operand layouts of v_mfma_f64_4x4x4f64 (lane of element (row, k) of A,
+// (k, n) of B and (row, n) of C in block j): A at row + 4 j + 16 k, B at
+// n + 4 j + 16 k, C at n + 4 j + 16 row.
+//
+// E of a source by the recurrence of the derivatives of 1/r,
+//     n R^2 E_k = -(2n - 1) sum_i k_i r_i E_{k - e_i}
+//                 - (n - 1) sum_i k_i (k_i - 1) E_{k - 2 e_i},  n = |k|,
+// degree by degree, 16 lanes per source.
+template <int P, int TA, int TB>
+__device__ __forceinline__ void m2l_mfma_tiles(double (&C)[m2l_ntile1(P)], double B,
+                                               const uint16_t *tab, const double *Eb, int rk)
+{
+    if constexpr (TB < m2l_nrows(P, TA)) {
+        constexpr int t = m2l_base(P, TA) + TB;
+        const double a = Eb[tab[16 * t + rk]];
+        C[TB] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, B, C[TB], 0, 0, 0);
+        m2l_mfma_tiles<P, TA, TB + 1>(C, B, tab, Eb, rk);
+    }
+}
+
+template <int P, int TA>
+__device__ __forceinline__ void m2l_mfma_group(double (&C)[m2l_ntile1(P)],
+                                               const double (&B)[m2l_ntile1(P)],
+                                               const uint16_t *tab, const double *Eb, int rk)
+{
+    if constexpr (TA < m2l_ntile1(P)) {
+        m2l_mfma_tiles<P, TA, 0>(C, B[TA], tab, Eb, rk);
+        m2l_mfma_group<P, TA + 1>(C, B, tab, Eb, rk);
+    }
+}
+
+// E row stride in LDS: K + 1 (the zero slot at K), rounded so that the four
+// sources' rows start 16 banks apart.
+constexpr int m2l_erow(int P) { return (m2l_ncoef(P) + 1 + 31) / 32 * 32 + 8; }
+
+// Ordering of LDS accesses between the lanes of one wave: a wave's LDS
+// operations execute in issue order, so a compiler-level barrier suffices
+// (no wait for outstanding global loads, unlike a workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// M <- m' = (-1)^|a| / a! m_a in place, once the upward pass is done (the
+// multipoles have no reader after the M2L).
+__global__ __launch_bounds__(FMM_BLOCK) void m2l_prescale_kernel(double *__restrict__ M, int64_t n,
+                                                                 int K,
+                                                                 const double *__restrict__ bscale)
+{
+    const int64_t i = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (i < n) M[i] *= bscale[i % K];
+}
+
+// Persistent: block b serves cells k G + (b mod 8) W + b / 8 (k = 0, 1, ...;
+// G = gridDim.x = 8 W), so each XCD (workgroups go round-robin over the 8
+// XCDs) sweeps contiguous runs of cells whose sources overlap in its L2, and
+// the tables are staged in LDS once per block.  `erec` packs, per k, the E
+// indices of k - e_i and k - 2 e_i (K where absent: the zero slot) and k:
+// (j1x | j1y << 16, j1z | j2x << 16, j2y | j2z << 16, kx | ky << 8 | kz << 16).
+template <int P>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P <= 8 ? 2 : 1)))
+void m2l_mfma_kernel(
+    const FmmCellDev *__restrict__ cells, int ncell, const int64_t *__restrict__ off,
+    const int32_t *__restrict__ src, const int4 *__restrict__ erec,
+    const uint16_t *__restrict__ tile_idx, const double *__restrict__ lscale,
+    const double *__restrict__ M, double *__restrict__ L)
+{
+    constexpr int K = m2l_ncoef(P), NT = m2l_ntile1(P), NTILE = m2l_ntiles(P), ER = m2l_erow(P);
+    __shared__ double E[4 * ER];
+    __shared__ uint16_t tab[16 * NTILE];
+    __shared__ int4 rec[K];
+    const int lane = threadIdx.x;
+    for (int i = lane; i < 16 * NTILE; i += 64) tab[i] = tile_idx[i];
+    for (int i = lane; i < K; i += 64) rec[i] = erec[i];
+    if (lane < 4) E[lane * ER + K] = 0.0;
+    __syncthreads();
+    const int nq = lane & 3, blk = (lane >> 2) & 3, hi = lane >> 4, sub = lane & 15;
+    const int rk = nq + 4 * hi;                 // A element (row = lane & 3, k = lane >> 4)
+    const double *Eb = E + blk * ER;
+    double *Ej = E + hi * ER;                   // E of source hi, built by lanes 16 hi ..
+    const int G = gridDim.x, W = G >> 3;
+    const int first = (int)(blockIdx.x & 7) * W + (int)(blockIdx.x >> 3);
+    for (int c = first; c < ncell; c += G) {
+        const FmmCellDev A = cells[c];
+        double Cacc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) Cacc[t] = 0.0;
+        const int64_t e0 = off[c], e1 = off[c + 1];
+        // Source indices run two groups ahead and centres one group ahead, so
+        // no wait for them lands behind the burst of B loads (loads complete
+        // in issue order).  Indices past the list are clamped to its last entry.
+        const int64_t elast = e1 - 1;
+        int ib = 0, ib1 = 0, ie1 = 0;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        if (e0 < e1) {
+            ib = src[min(e0 + blk, elast)];
+            ib1 = src[min(e0 + 4 + blk, elast)];
+            ie1 = src[min(e0 + 4 + hi, elast)];
+            const FmmCellDev S = cells[src[min(e0 + hi, elast)]];
+            sx = S.cx; sy = S.cy; sz = S.cz;
+        }
+        for (int64_t eb = e0; eb < e1; eb += 4) {
+            const int nb = (int)min<int64_t>(4, e1 - eb);
+            const FmmCellDev Sn = cells[ie1];   // next group's centre of source hi
+            const int ie2 = src[min(eb + 8 + hi, elast)], ib2 = src[min(eb + 8 + blk, elast)];
+            // B operand: m'_blk[4 ta + k][n], k = lane >> 4, n = lane & 3 (M holds
+            // m' here).  Every lane loads from a valid address and multiplies by
+            // a 0/1 mask, so the loads issue back to back instead of one guarded
+            // load at a time (the multipoles are finite).
+            double Bv[NT];
+            const double bmask = (blk < nb && nq < 3) ? 1.0 : 0.0;
+            const double *Mb = M + ((size_t)ib * 3 + (nq < 3 ? nq : 0)) * K;
+#pragma unroll
+            for (int ta = 0; ta < NT; ++ta)
+                Bv[ta] = Mb[min(4 * ta + hi, K - 1)] * (4 * ta + hi < K ? bmask : 0.0);
+            // E of source hi (absent sources: r = (1, 0, 0), finite E, zero B)
+            double r0 = 1.0, r1 = 0.0, r2 = 0.0;
+            if (hi < nb) {
+                r0 = A.cx - sx;
+                r1 = A.cy - sy;
+                r2 = A.cz - sz;
+            }
+            sx = Sn.cx; sy = Sn.cy; sz = Sn.cz;
+            ie1 = ie2;
+            ib = ib1;
+            ib1 = ib2;
+            const double R2 = r0 * r0 + r1 * r1 + r2 * r2, iR2 = 1.0 / R2;
+            wave_lds_sync();   // the previous group's reads of E are done
+            if (sub == 0) Ej[0] = 1.0 / sqrt(R2);
+#pragma unroll
+            for (int n = 1; n <= P; ++n) {
+                wave_lds_sync();
+                const double f1 = -(2.0 * n - 1.0) / n * iR2, f2 = -(n - 1.0) / n * iR2;
+                for (int k = m2l_ncoef(n - 1) + sub; k < m2l_ncoef(n); k += 16) {
+                    const int4 q = rec[k];
+                    const double kx = q.w & 255, ky = (q.w >> 8) & 255, kz = (q.w >> 16) & 255;
+                    const double s1 = kx * r0 * Ej[q.x & 0xffff] + ky * r1 * Ej[q.x >> 16] +
+                                      kz * r2 * Ej[q.y & 0xffff];
+                    const double s2 = kx * (kx - 1.0) * Ej[q.y >> 16] +
+                                      ky * (ky - 1.0) * Ej[q.z & 0xffff] +
+                                      kz * (kz - 1.0) * Ej[q.z >> 16];
+                    Ej[k] = f1 * s1 + f2 * s2;
+                }
+            }
+            wave_lds_sync();
+            m2l_mfma_group<P, 0>(Cacc, Bv, tab, Eb, rk);
+        }
+        // sum the four blocks (lanes 4 and 8 apart), scale by 1/b!, write l
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            double v = Cacc[t];
+            v += __shfl_xor(v, 4);
+            v += __shfl_xor(v, 8);
+            const int b = 4 * t + hi;
+            const double ls = lscale[min(b, K - 1)];
+            if (blk == 0 && nq < 3 && b < K) L[((size_t)c * 3 + nq) * K + b] = v * ls;
+        }
+    }
+}
+
 // L2L for the children listed in `kids` (one level): l_g(child) +=
 // sum_{b >= g} C(b,g) l_b(parent) e^{b-g}, e = c_child - c_parent.
 __global__ __launch_bounds__(FMM_BLOCK) void l2l_kernel(const int32_t *__restrict__ kids,
@@ -936,6 +1114,50 @@ void fmm_build_tables(FmmTables &T, int P)
         T.m2l_rounds = best_rounds;
         T.m2l_chunk_len = best_len;
     }
+    // matrix-core M2L: scales and the E index of every tile element
+    {
+        auto fact = [](int n) {
+            double f = 1.0;
+            for (int i = 2; i <= n; i++) f *= i;
+            return f;
+        };
+        T.m2l_bscale.assign(K, 0.0);
+        T.m2l_lscale.assign(K, 0.0);
+        for (int k = 0; k < K; k++) {
+            const int *a = &M.mi[3 * k];
+            const double f = fact(a[0]) * fact(a[1]) * fact(a[2]);
+            T.m2l_bscale[k] = ((a[0] + a[1] + a[2]) % 2 ? -1.0 : 1.0) / f;
+            T.m2l_lscale[k] = 1.0 / f;
+        }
+        T.m2l_erec.assign(4 * K, 0);
+        for (int k = 0; k < K; k++) {
+            int j[6];
+            for (int i = 0; i < 3; i++) {
+                j[i] = T.rec1[4 * k + i] >= 0 ? T.rec1[4 * k + i] : K;
+                j[3 + i] = T.rec2[4 * k + i] >= 0 ? T.rec2[4 * k + i] : K;
+            }
+            const int *m = &M.mi[3 * k];
+            T.m2l_erec[4 * k] = j[0] | (j[1] << 16);
+            T.m2l_erec[4 * k + 1] = j[2] | (j[3] << 16);
+            T.m2l_erec[4 * k + 2] = j[4] | (j[5] << 16);
+            T.m2l_erec[4 * k + 3] = m[0] | (m[1] << 8) | (m[2] << 16);
+        }
+        T.m2l_tile_idx.clear();
+        if (P >= 1 && P <= M2L_MFMA_MAX_P) {
+            for (int ta = 0; ta < m2l_ntile1(P); ta++)
+                for (int tb = 0; tb < m2l_nrows(P, ta); tb++)
+                    for (int e = 0; e < 16; e++) {
+                        const int row = e & 3, k = e >> 2, a = 4 * ta + k, b = 4 * tb + row;
+                        int v = K;
+                        if (a < K && b < K) {
+                            const int *ma = &M.mi[3 * a], *mb = &M.mi[3 * b];
+                            const int s = M.idx(ma[0] + mb[0], ma[1] + mb[1], ma[2] + mb[2]);
+                            if (s >= 0) v = s;
+                        }
+                        T.m2l_tile_idx.push_back((uint16_t)v);
+                    }
+        }
+    }
     // L2P: idx(b + e_j) and idx(b + e_j + e_l)
     T.d1.assign(4 * K, -1);
     T.d2a.assign(4 * K, -1);
@@ -1444,6 +1666,24 @@ hipError_t launch_fmm_gather(hipStream_t st, const DeviceField &f, const int32_t
     return hipGetLastError();
 }
 
+// Grid of the persistent matrix-core M2L: resident blocks per CU (from the
+// kernel's register and LDS use) times the CUs, a multiple of 8 (one share
+// per XCD), at most one block per 8 cells.
+static int m2l_mfma_grid(const void *fn, int P, int ncell)
+{
+    static int cached[M2L_MFMA_MAX_P + 1] = {0};
+    if (!cached[P]) {
+        int dev = 0, ncu = 0, nb = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess)
+            ncu = 256, nb = 4;
+        cached[P] = std::max(1, ncu) * std::max(1, nb);
+    }
+    const int want = std::min(cached[P], std::max(1, ncell / 8));
+    return (want + 7) / 8 * 8;
+}
+
 hipError_t launch_fmm_eval(hipStream_t st, const FmmDev &d, const FmmPlan &pl, int kernel,
                            int precision, const DeviceField &f, int reset)
 {
@@ -1463,7 +1703,26 @@ hipError_t launch_fmm_eval(hipStream_t st, const FmmDev &d, const FmmPlan &pl, i
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (d.ncell > 0) {
+    if (d.ncell > 0 && P >= 1 && P <= M2L_MFMA_MAX_P) {
+        const int64_t nm = (int64_t)d.ncell * 3 * K;
+        m2l_prescale_kernel<<<gridn(nm, FMM_BLOCK), FMM_BLOCK, 0, st>>>(d.M, nm, K, d.m2l_bscale);
+#define VPMX_M2LM(PP)                                                                           \
+    case PP: {                                                                                  \
+        const int grid = m2l_mfma_grid((const void *)m2l_mfma_kernel<PP>, PP, d.ncell);          \
+        m2l_mfma_kernel<PP><<<grid, 64, 0, st>>>(d.cells, d.ncell, d.m2l_off, d.m2l_src,         \
+                                                 d.m2l_erec, d.m2l_tile_idx, d.m2l_lscale, d.M, \
+                                                 d.L);                                          \
+        break;                                                                                  \
+    }
+        switch (P) {
+            VPMX_M2LM(1) VPMX_M2LM(2) VPMX_M2LM(3) VPMX_M2LM(4) VPMX_M2LM(5)
+            VPMX_M2LM(6) VPMX_M2LM(7) VPMX_M2LM(8) VPMX_M2LM(9) VPMX_M2LM(10)
+        default: return hipErrorInvalidValue;
+        }
+#undef VPMX_M2LM
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    } else if (d.ncell > 0) {
         // derivatives + multipoles of a batch, or the chunk sums at the end
         const size_t shm = sizeof(double) * std::max<size_t>((size_t)M2L_BATCH * 4 * K,
                                                              (size_t)3 * d.m2l_nchunk);

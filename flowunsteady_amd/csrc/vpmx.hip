@@ -650,6 +650,10 @@ int fmm_tables_to_device(vpmx_ctx *c, FmmState &S)
     VPMX_TRY(upload_as(c, &S.m2l_tcoef, T.m2l_coef));
     VPMX_TRY(upload_as(c, &S.m2l_chunks, T.m2l_chunk));
     VPMX_TRY(upload_as(c, &S.m2l_cfirst, T.m2l_cfirst));
+    VPMX_TRY(upload_as(c, &S.m2l_tile_idx, T.m2l_tile_idx));
+    VPMX_TRY(upload_as(c, &S.m2l_bscale, T.m2l_bscale));
+    VPMX_TRY(upload_as(c, &S.m2l_lscale, T.m2l_lscale));
+    VPMX_TRY(upload_as(c, &S.m2l_erec, T.m2l_erec));
     S.dev_tab_p = T.P;
     return VPMX_OK;
 }
@@ -659,7 +663,8 @@ void free_fmm(FmmState *S)
     if (!S) return;
     void *ptrs[] = {S->mi, S->m2m_off, S->l2l_off, S->m2l_toff, S->rec1, S->rec2, S->d1, S->d2a,
                     S->d2b, S->m2m_idx, S->l2l_idx, S->m2l_tidx, S->m2m_coef, S->l2l_coef,
-                    S->m2l_tcoef, S->m2l_chunks, S->m2l_cfirst, S->cells, S->leaves, S->parents, S->kids, S->parent,
+                    S->m2l_tcoef, S->m2l_chunks, S->m2l_cfirst, S->m2l_tile_idx, S->m2l_bscale,
+                    S->m2l_lscale, S->m2l_erec, S->cells, S->leaves, S->parents, S->kids, S->parent,
                     S->m2l_off, S->p2p_off, S->m2l_src, S->p2p_src, S->M, S->L, S->kin, S->kout,
                     S->ssig, S->sort_tmp};
     for (void *p : ptrs)
@@ -844,6 +849,10 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
     d.m2l_cfirst = S.m2l_cfirst;
     d.m2l_nchunk = S.tab.m2l_nchunk;
     d.m2l_rounds = S.tab.m2l_rounds;
+    d.m2l_tile_idx = S.m2l_tile_idx;
+    d.m2l_bscale = S.m2l_bscale;
+    d.m2l_lscale = S.m2l_lscale;
+    d.m2l_erec = S.m2l_erec;
     const double ms_before = c->timer.total_ms[T_FMM];
     {
         TimedScope ts(c, T_FMM);
