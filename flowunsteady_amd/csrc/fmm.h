@@ -47,6 +47,8 @@ constexpr int m2l_base(int P, int ta)
     return s;
 }
 constexpr int m2l_ntiles(int P) { return m2l_base(P, m2l_ntile1(P)); }
+// Tile count padded to whole batches of 8 (the index table's row length).
+constexpr int m2l_ntiles8(int P) { return (m2l_ntiles(P) + 7) / 8 * 8; }
 
 struct FmmCellDev {
     int32_t begin, end, child0, nchild;
@@ -90,9 +92,10 @@ struct FmmTables {
     // of b; m2l_rounds = chunks per thread of a 256-thread block (1, 2, 4 or 8).
     std::vector<int> m2l_chunk, m2l_cfirst;
     int m2l_nchunk = 0, m2l_rounds = 1, m2l_chunk_len = 0;
-    // Matrix-core M2L (P <= M2L_MFMA_MAX_P): per tile, for (row, k) at
-    // row + 4 k, the index of a + b in E, or K (a zero slot) outside the region;
-    // bscale[a] = (-1)^|a| / a!, lscale[b] = 1 / b!.
+    // Matrix-core M2L (P <= M2L_MFMA_MAX_P): for element (row, k) of tile t, at
+    // (row + 4 k) m2l_ntiles8(P) + t, the index of a + b in E, or K (a zero
+    // slot) outside the region and in the padding; bscale[a] = (-1)^|a| / a!,
+    // lscale[b] = 1 / b!.
     std::vector<uint16_t> m2l_tile_idx;
     std::vector<double> m2l_bscale, m2l_lscale;
     std::vector<int> m2l_erec;   // 4 per k, packed as m2l_mfma_kernel reads them
@@ -169,9 +172,10 @@ struct FmmState {
     DttBufs dtt;
 };
 
-// Estr over the near-field lists of the last FMM evaluation (J must be current).
+// Estr over the near-field lists of the last FMM evaluation (J must be current);
+// precision 1 runs the FP32 pair math with FP64 accumulation.
 hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int transposed,
-                           const DeviceField &f, Scratch &s);
+                           int precision, const DeviceField &f, Scratch &s);
 
 // Dual tree traversal on the device: fills m2l_off/m2l_src and p2p_off/p2p_src
 // (CSR by target cell, sources ascending), growing the buffers in S.

@@ -26,6 +26,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <utility>
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -389,32 +390,63 @@ __global__ __launch_bounds__(FMM_BLOCK) void m2l_kernel(const FmmCellDev *__rest
 //     n R^2 E_k = -(2n - 1) sum_i k_i r_i E_{k - e_i}
 //                 - (n - 1) sum_i k_i (k_i - 1) E_{k - 2 e_i},  n = |k|,
 // degree by degree, 16 lanes per source.
-template <int P, int TA, int TB>
-__device__ __forceinline__ void m2l_mfma_tiles(double (&C)[m2l_ntile1(P)], double B,
-                                               const uint16_t *tab, const double *Eb, int rk)
+// Tile t -> (ta, tb), in the order the tile index table holds them.
+template <int P>
+struct M2LTiles {
+    struct Map {
+        int ta[m2l_ntiles8(P)], tb[m2l_ntiles8(P)];
+    };
+    static constexpr Map make()
+    {
+        Map m{};
+        int nat[m2l_ntile1(P)] = {};
+        for (int tb = 0; tb < m2l_ntile1(P); ++tb) nat[tb] = m2l_natile(P, tb);
+        int t = 0;
+        for (int ta = 0; ta < m2l_ntile1(P); ++ta)
+            for (int tb = 0; tb < m2l_ntile1(P) && nat[tb] > ta; ++tb, ++t) {
+                m.ta[t] = ta;
+                m.tb[t] = tb;
+            }
+        return m;
+    }
+    static constexpr Map map = make();
+};
+
+template <int P, int T>
+__device__ __forceinline__ void m2l_mfma_one(double (&C)[m2l_ntile1(P)],
+                                             const double (&B)[m2l_ntile1(P)], double a)
 {
-    if constexpr (TB < m2l_nrows(P, TA)) {
-        constexpr int t = m2l_base(P, TA) + TB;
-        const double a = Eb[tab[16 * t + rk]];
-        C[TB] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, B, C[TB], 0, 0, 0);
-        m2l_mfma_tiles<P, TA, TB + 1>(C, B, tab, Eb, rk);
+    if constexpr (T < m2l_ntiles(P)) {
+        constexpr int ta = M2LTiles<P>::map.ta[T], tb = M2LTiles<P>::map.tb[T];
+        C[tb] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, B[ta], C[tb], 0, 0, 0);
     }
 }
 
-template <int P, int TA>
+// Eight tiles: one 16-byte LDS read brings this lane's eight E indices, then
+// eight independent gathers of E feed eight MFMAs.
+template <int P, int T0, int... J>
+__device__ __forceinline__ void m2l_mfma_batch(double (&C)[m2l_ntile1(P)],
+                                               const double (&B)[m2l_ntile1(P)],
+                                               const uint16_t *tl, const double *Eb,
+                                               std::integer_sequence<int, J...>)
+{
+    const uint4 q = *reinterpret_cast<const uint4 *>(tl + T0);
+    const unsigned w[4] = {q.x, q.y, q.z, q.w};
+    double a[8];
+    ((a[J] = Eb[(w[J >> 1] >> (16 * (J & 1))) & 0xffffu]), ...);
+    (m2l_mfma_one<P, T0 + J>(C, B, a[J]), ...);
+}
+
+template <int P, int T0>
 __device__ __forceinline__ void m2l_mfma_group(double (&C)[m2l_ntile1(P)],
                                                const double (&B)[m2l_ntile1(P)],
-                                               const uint16_t *tab, const double *Eb, int rk)
+                                               const uint16_t *tl, const double *Eb)
 {
-    if constexpr (TA < m2l_ntile1(P)) {
-        m2l_mfma_tiles<P, TA, 0>(C, B[TA], tab, Eb, rk);
-        m2l_mfma_group<P, TA + 1>(C, B, tab, Eb, rk);
+    if constexpr (T0 < m2l_ntiles(P)) {
+        m2l_mfma_batch<P, T0>(C, B, tl, Eb, std::make_integer_sequence<int, 8>{});
+        m2l_mfma_group<P, T0 + 8>(C, B, tl, Eb);
     }
 }
-
-// E row stride in LDS: K + 1 (the zero slot at K), rounded so that the four
-// sources' rows start 16 banks apart.
-constexpr int m2l_erow(int P) { return (m2l_ncoef(P) + 1 + 31) / 32 * 32 + 8; }
 
 // Ordering of LDS accesses between the lanes of one wave: a wave's LDS
 // operations execute in issue order, so a compiler-level barrier suffices
@@ -425,6 +457,8 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+constexpr int m2l_erow(int P) { return (m2l_ncoef(P) + 1 + 31) / 32 * 32 + 8; }
 
 // M <- m' = (-1)^|a| / a! m_a in place, once the upward pass is done (the
 // multipoles have no reader after the M2L).
@@ -442,6 +476,8 @@ __global__ __launch_bounds__(FMM_BLOCK) void m2l_prescale_kernel(double *__restr
 // the tables are staged in LDS once per block.  `erec` packs, per k, the E
 // indices of k - e_i and k - 2 e_i (K where absent: the zero slot) and k:
 // (j1x | j1y << 16, j1z | j2x << 16, j2y | j2z << 16, kx | ky << 8 | kz << 16).
+// The four sources' E are rows of m2l_erow(P) doubles: K + 1 (the zero slot at
+// K), rounded so that the rows start 16 banks apart.
 template <int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P <= 8 ? 2 : 1)))
 void m2l_mfma_kernel(
@@ -450,17 +486,17 @@ void m2l_mfma_kernel(
     const uint16_t *__restrict__ tile_idx, const double *__restrict__ lscale,
     const double *__restrict__ M, double *__restrict__ L)
 {
-    constexpr int K = m2l_ncoef(P), NT = m2l_ntile1(P), NTILE = m2l_ntiles(P), ER = m2l_erow(P);
+    constexpr int K = m2l_ncoef(P), NT = m2l_ntile1(P), N8 = m2l_ntiles8(P), ER = m2l_erow(P);
     __shared__ double E[4 * ER];
-    __shared__ uint16_t tab[16 * NTILE];
+    __shared__ __align__(16) uint16_t tab[16 * N8];
     __shared__ int4 rec[K];
     const int lane = threadIdx.x;
-    for (int i = lane; i < 16 * NTILE; i += 64) tab[i] = tile_idx[i];
+    for (int i = lane; i < 16 * N8; i += 64) tab[i] = tile_idx[i];
     for (int i = lane; i < K; i += 64) rec[i] = erec[i];
     if (lane < 4) E[lane * ER + K] = 0.0;
     __syncthreads();
     const int nq = lane & 3, blk = (lane >> 2) & 3, hi = lane >> 4, sub = lane & 15;
-    const int rk = nq + 4 * hi;                 // A element (row = lane & 3, k = lane >> 4)
+    const uint16_t *tl = tab + (nq + 4 * hi) * N8;   // A element (row = lane & 3, k = lane >> 4)
     const double *Eb = E + blk * ER;
     double *Ej = E + hi * ER;                   // E of source hi, built by lanes 16 hi ..
     const int G = gridDim.x, W = G >> 3;
@@ -528,7 +564,7 @@ void m2l_mfma_kernel(
                 }
             }
             wave_lds_sync();
-            m2l_mfma_group<P, 0>(Cacc, Bv, tab, Eb, rk);
+            m2l_mfma_group<P, 0>(Cacc, Bv, tl, Eb);
         }
         // sum the four blocks (lanes 4 and 8 apart), scale by 1/b!, write l
 #pragma unroll
@@ -753,6 +789,41 @@ __global__ __launch_bounds__(P2P_BLOCK) void p2p_f64_kernel(const int32_t *__res
 // sums of a target are added in slice order at the end (deterministic).
 constexpr int P2P_SLOTS = 64;
 
+// Source-leaf stream of one slot (up to P2P_SLOTS leaves from the list at eb):
+// lbeg = each leaf's first particle, lpre = inclusive prefix of the leaf sizes
+// (a wave scan; the block is one wave).  Returns the slot's particle count.
+__device__ __forceinline__ int stream_setup(const FmmCellDev *__restrict__ cells,
+                                            const int32_t *__restrict__ src, int64_t eb, int nl,
+                                            int lane, int32_t *lbeg, int32_t *lpre)
+{
+    int v = 0;
+    if (lane < nl) {
+        const FmmCellDev B = cells[src[eb + lane]];
+        lbeg[lane] = B.begin;
+        v = B.end - B.begin;
+    }
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    lpre[lane + 1] = v;
+    if (lane == 0) lpre[0] = 0;
+    __syncthreads();
+    return lpre[nl];
+}
+
+// Particle at stream position q < total of the slot.
+__device__ __forceinline__ int stream_particle(const int32_t *lbeg, const int32_t *lpre, int nl,
+                                               int q)
+{
+    int lo = 0, hi = nl;   // lpre[lo] <= q < lpre[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (lpre[mid] <= q) lo = mid; else hi = mid;
+    }
+    return lbeg[lo] + (q - lpre[lo]);
+}
+
 template <int KK, int R>
 __device__ __forceinline__ void p2p_mixed_tile(const float4 *tpos, const float4 *tgam, int cnt,
                                                int s, float xi, float yi, float zi, MixedAcc &acc)
@@ -769,7 +840,8 @@ __device__ __forceinline__ void p2p_mixed_tile(const float4 *tpos, const float4 
 }
 
 template <int KK>
-__global__ __launch_bounds__(P2P_BLOCK) void p2p_mixed_kernel(const int32_t *__restrict__ leaves,
+__global__ __launch_bounds__(P2P_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+void p2p_mixed_kernel(const int32_t *__restrict__ leaves,
                                                               const FmmCellDev *__restrict__ cells,
                                                               const int64_t *__restrict__ off,
                                                               const int32_t *__restrict__ src,
@@ -803,41 +875,35 @@ __global__ __launch_bounds__(P2P_BLOCK) void p2p_mixed_kernel(const int32_t *__r
         for (int64_t eb = e0; eb < e1; eb += P2P_SLOTS) {
             const int nl = (int)min<int64_t>(P2P_SLOTS, e1 - eb);
             __syncthreads();
-            if (lane < nl) {
-                const FmmCellDev B = cells[src[eb + lane]];
-                lbeg[lane] = B.begin;
-                lpre[lane + 1] = B.end - B.begin;
+            const int total = stream_setup(cells, src, eb, nl, lane, lbeg, lpre);
+            // the records of the next tile load while this one computes
+            double4 ra = make_double4(0.0, 0.0, 0.0, 0.0), rb = ra;
+            if (lane < total) {
+                const int js = stream_particle(lbeg, lpre, nl, lane);
+                ra = rec[2 * js];
+                rb = rec[2 * js + 1];
             }
-            __syncthreads();
-            if (lane == 0) {
-                lpre[0] = 0;
-                for (int k = 1; k <= nl; ++k) lpre[k] += lpre[k - 1];
-            }
-            __syncthreads();
-            const int total = lpre[nl];
             for (int base = 0; base < total; base += P2P_BLOCK) {
-                const int q = base + lane;
                 __syncthreads();
-                if (q < total) {
-                    int lo = 0, hi = nl;          // lpre[lo] <= q < lpre[hi]
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (lpre[mid] <= q) lo = mid; else hi = mid;
-                    }
-                    const int js = lbeg[lo] + (q - lpre[lo]);
-                    const double4 a = rec[2 * js], b = rec[2 * js + 1];
+                if (base + lane < total) {
                     float p0 = 0.f, p1 = 0.f;
                     if constexpr (KK == GAUSSIANERF || KK == GAUSSIAN) {
-                        p0 = (float)b.w;
-                        p1 = (float)(b.w * b.w);
+                        p0 = (float)rb.w;
+                        p1 = (float)(rb.w * rb.w);
                     } else if constexpr (KK == WINCKELMANS) {
-                        p0 = (float)(b.z * b.z);
+                        p0 = (float)(rb.z * rb.z);
                     }
-                    tpos[lane] = make_float4((float)(a.x - A.cx), (float)(a.y - A.cy),
-                                             (float)(a.z - A.cz), p0);
-                    tgam[lane] = make_float4((float)a.w, (float)b.x, (float)b.y, p1);
+                    tpos[lane] = make_float4((float)(ra.x - A.cx), (float)(ra.y - A.cy),
+                                             (float)(ra.z - A.cz), p0);
+                    tgam[lane] = make_float4((float)ra.w, (float)rb.x, (float)rb.y, p1);
                 }
                 __syncthreads();
+                const int qn = base + P2P_BLOCK + lane;
+                if (qn < total) {
+                    const int js = stream_particle(lbeg, lpre, nl, qn);
+                    ra = rec[2 * js];
+                    rb = rec[2 * js + 1];
+                }
                 const int cnt = min(P2P_BLOCK, total - base);
                 MixedAcc acc;
                 switch (R) {
@@ -1144,17 +1210,17 @@ void fmm_build_tables(FmmTables &T, int P)
         }
         T.m2l_tile_idx.clear();
         if (P >= 1 && P <= M2L_MFMA_MAX_P) {
+            const int N8 = m2l_ntiles8(P);
+            T.m2l_tile_idx.assign((size_t)16 * N8, (uint16_t)K);
+            int t = 0;
             for (int ta = 0; ta < m2l_ntile1(P); ta++)
-                for (int tb = 0; tb < m2l_nrows(P, ta); tb++)
+                for (int tb = 0; tb < m2l_nrows(P, ta); tb++, t++)
                     for (int e = 0; e < 16; e++) {
                         const int row = e & 3, k = e >> 2, a = 4 * ta + k, b = 4 * tb + row;
-                        int v = K;
-                        if (a < K && b < K) {
-                            const int *ma = &M.mi[3 * a], *mb = &M.mi[3 * b];
-                            const int s = M.idx(ma[0] + mb[0], ma[1] + mb[1], ma[2] + mb[2]);
-                            if (s >= 0) v = s;
-                        }
-                        T.m2l_tile_idx.push_back((uint16_t)v);
+                        if (a >= K || b >= K) continue;
+                        const int *ma = &M.mi[3 * a], *mb = &M.mi[3 * b];
+                        const int s = M.idx(ma[0] + mb[0], ma[1] + mb[1], ma[2] + mb[2]);
+                        if (s >= 0) T.m2l_tile_idx[(size_t)e * N8 + t] = (uint16_t)s;
                     }
         }
     }
@@ -1274,8 +1340,181 @@ __global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_kernel(const int32_t *__re
     }
 }
 
+// Estr over the near-field lists, mixed: FP32 pair math on coordinates
+// relative to the target leaf's centre and on J relative to the J of the
+// leaf's first particle (J_i - J_j then keeps its relative precision), FP32
+// partials per source tile folded into FP64.  Same streaming of the source
+// leaves and replication of small target leaves as p2p_mixed_kernel.
+
+// zeta(r / sigma) / sigma^3 in FP32; p0 = 1/sigma, p1 = 1/sigma^2.
+template <int KK>
+__device__ __forceinline__ float zeta_s3_f32(float r2, float p0, float p1)
+{
+    const float is3 = p0 * p1;
+    if constexpr (KK == GAUSSIANERF) {
+        return 0.06349363593424097f * fexp2(-0.5f * LOG2E * r2 * p1) * is3;   // (2 pi)^-3/2
+    } else if constexpr (KK == GAUSSIAN) {
+        const float rho2 = r2 * p1;
+        return 0.238732414637843f * fexp2(-LOG2E * rho2 * sqrtf(rho2)) * is3;   // 3/(4 pi)
+    } else {   // WINCKELMANS: 15/(8 pi) (rho^2 + 1)^-7/2
+        const float si = frsq(fmaf(r2, p1, 1.0f)), si2 = si * si;
+        return 0.5968310365946075f * si2 * si2 * si2 * si * is3;
+    }
+}
+
+template <int KK, bool TRANSPOSED>
+__device__ __forceinline__ void estr_mixed_pair(const float4 a, const float4 g, const float4 q0,
+                                                const float4 q1, float q8, float xi, float yi,
+                                                float zi, const float (&Ji)[9], float &e0,
+                                                float &e1, float &e2)
+{
+    const float dx = xi - a.x, dy = yi - a.y, dz = zi - a.z;
+    const float zs = zeta_s3_f32<KK>(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), a.w, g.w);
+    const float d0 = Ji[0] - q0.x, d1 = Ji[1] - q0.y, d2 = Ji[2] - q0.z;
+    const float d3 = Ji[3] - q0.w, d4 = Ji[4] - q1.x, d5 = Ji[5] - q1.y;
+    const float d6 = Ji[6] - q1.z, d7 = Ji[7] - q1.w, d8 = Ji[8] - q8;
+    float S0, S1, S2;
+    if constexpr (TRANSPOSED) {
+        S0 = fmaf(d0, g.x, fmaf(d1, g.y, d2 * g.z));
+        S1 = fmaf(d3, g.x, fmaf(d4, g.y, d5 * g.z));
+        S2 = fmaf(d6, g.x, fmaf(d7, g.y, d8 * g.z));
+    } else {
+        S0 = fmaf(d0, g.x, fmaf(d3, g.y, d6 * g.z));
+        S1 = fmaf(d1, g.x, fmaf(d4, g.y, d7 * g.z));
+        S2 = fmaf(d2, g.x, fmaf(d5, g.y, d8 * g.z));
+    }
+    e0 = fmaf(zs, S0, e0);
+    e1 = fmaf(zs, S1, e1);
+    e2 = fmaf(zs, S2, e2);
+}
+
+template <int KK, bool TRANSPOSED, int R>
+__device__ __forceinline__ void estr_mixed_tile(const float4 *sp, const float4 *sg,
+                                                const float4 *sj0, const float4 *sj1,
+                                                const float *sj8, int cnt, int s, float xi,
+                                                float yi, float zi, const float (&Ji)[9],
+                                                float &e0, float &e1, float &e2)
+{
+    if (cnt == P2P_BLOCK) {
+#pragma unroll 4
+        for (int i = 0; i < P2P_BLOCK / R; ++i) {
+            const int j = i * R + s;
+            estr_mixed_pair<KK, TRANSPOSED>(sp[j], sg[j], sj0[j], sj1[j], sj8[j], xi, yi, zi, Ji,
+                                            e0, e1, e2);
+        }
+    } else {
+        for (int j = s; j < cnt; j += R)
+            estr_mixed_pair<KK, TRANSPOSED>(sp[j], sg[j], sj0[j], sj1[j], sj8[j], xi, yi, zi, Ji,
+                                            e0, e1, e2);
+    }
+}
+
+template <int KK, bool TRANSPOSED>
+__global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_mixed_kernel(
+    const int32_t *__restrict__ leaves, const FmmCellDev *__restrict__ cells,
+    const int64_t *__restrict__ off, const int32_t *__restrict__ src,
+    const double4 *__restrict__ rec, const int32_t *__restrict__ perm, int64_t cap,
+    double *__restrict__ E)
+{
+    __shared__ float4 sp[P2P_BLOCK], sg[P2P_BLOCK], sj0[P2P_BLOCK], sj1[P2P_BLOCK];
+    __shared__ float sj8[P2P_BLOCK];
+    __shared__ int32_t lbeg[P2P_SLOTS], lpre[P2P_SLOTS + 1];
+    __shared__ double red[3][P2P_BLOCK];
+    const int lane = threadIdx.x;
+    const int c = leaves[blockIdx.x];
+    const FmmCellDev A = cells[c];
+    const int64_t e0 = off[c], e1 = off[c + 1];
+    for (int tb = A.begin; tb < A.end; tb += P2P_BLOCK) {
+        const int nt = min(P2P_BLOCK, A.end - tb);
+        const int R = nt <= 8 ? 8 : nt <= 16 ? 4 : nt <= 32 ? 2 : 1;
+        const int sl = lane / nt;                  // slice of this lane
+        const int t = lane - sl * nt;              // target of this lane
+        const bool active = sl < R;
+        // reference J: the block's first target
+        double Jr[9];
+        {
+            const double4 u = rec[5 * tb + 2], v = rec[5 * tb + 3], w = rec[5 * tb + 4];
+            Jr[0] = u.x; Jr[1] = u.y; Jr[2] = u.z; Jr[3] = u.w;
+            Jr[4] = v.x; Jr[5] = v.y; Jr[6] = v.z; Jr[7] = v.w; Jr[8] = w.x;
+        }
+        float xi, yi, zi, Ji[9];
+        {
+            const double4 a = rec[5 * (tb + t)], u = rec[5 * (tb + t) + 2],
+                          v = rec[5 * (tb + t) + 3], w = rec[5 * (tb + t) + 4];
+            xi = (float)(a.x - A.cx); yi = (float)(a.y - A.cy); zi = (float)(a.z - A.cz);
+            Ji[0] = (float)(u.x - Jr[0]); Ji[1] = (float)(u.y - Jr[1]);
+            Ji[2] = (float)(u.z - Jr[2]); Ji[3] = (float)(u.w - Jr[3]);
+            Ji[4] = (float)(v.x - Jr[4]); Ji[5] = (float)(v.y - Jr[5]);
+            Ji[6] = (float)(v.z - Jr[6]); Ji[7] = (float)(v.w - Jr[7]);
+            Ji[8] = (float)(w.x - Jr[8]);
+        }
+        const int s = active ? sl : 0;
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+        for (int64_t eb = e0; eb < e1; eb += P2P_SLOTS) {
+            const int nl = (int)min<int64_t>(P2P_SLOTS, e1 - eb);
+            __syncthreads();
+            const int total = stream_setup(cells, src, eb, nl, lane, lbeg, lpre);
+            // the records of the next tile load while this one computes
+            double4 ra = make_double4(0.0, 0.0, 0.0, 0.0), rg = ra, ru = ra, rv = ra, rw = ra;
+            if (lane < total) {
+                const int js = stream_particle(lbeg, lpre, nl, lane);
+                ra = rec[5 * js]; rg = rec[5 * js + 1]; ru = rec[5 * js + 2];
+                rv = rec[5 * js + 3]; rw = rec[5 * js + 4];
+            }
+            for (int base = 0; base < total; base += P2P_BLOCK) {
+                __syncthreads();
+                if (base + lane < total) {
+                    sp[lane] = make_float4((float)(ra.x - A.cx), (float)(ra.y - A.cy),
+                                           (float)(ra.z - A.cz), (float)ra.w);
+                    sg[lane] = make_float4((float)rg.x, (float)rg.y, (float)rg.z,
+                                           (float)(ra.w * ra.w));
+                    sj0[lane] = make_float4((float)(ru.x - Jr[0]), (float)(ru.y - Jr[1]),
+                                            (float)(ru.z - Jr[2]), (float)(ru.w - Jr[3]));
+                    sj1[lane] = make_float4((float)(rv.x - Jr[4]), (float)(rv.y - Jr[5]),
+                                            (float)(rv.z - Jr[6]), (float)(rv.w - Jr[7]));
+                    sj8[lane] = (float)(rw.x - Jr[8]);
+                }
+                __syncthreads();
+                const int qn = base + P2P_BLOCK + lane;
+                if (qn < total) {
+                    const int js = stream_particle(lbeg, lpre, nl, qn);
+                    ra = rec[5 * js]; rg = rec[5 * js + 1]; ru = rec[5 * js + 2];
+                    rv = rec[5 * js + 3]; rw = rec[5 * js + 4];
+                }
+                const int cnt = min(P2P_BLOCK, total - base);
+                float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+                switch (R) {
+                case 8: estr_mixed_tile<KK, TRANSPOSED, 8>(sp, sg, sj0, sj1, sj8, cnt, s, xi, yi, zi, Ji, f0, f1, f2); break;
+                case 4: estr_mixed_tile<KK, TRANSPOSED, 4>(sp, sg, sj0, sj1, sj8, cnt, s, xi, yi, zi, Ji, f0, f1, f2); break;
+                case 2: estr_mixed_tile<KK, TRANSPOSED, 2>(sp, sg, sj0, sj1, sj8, cnt, s, xi, yi, zi, Ji, f0, f1, f2); break;
+                default: estr_mixed_tile<KK, TRANSPOSED, 1>(sp, sg, sj0, sj1, sj8, cnt, s, xi, yi, zi, Ji, f0, f1, f2); break;
+                }
+                d0 += f0; d1 += f1; d2 += f2;
+            }
+        }
+        if (R > 1) {   // add the slices of each target in slice order
+            __syncthreads();
+            red[0][lane] = d0; red[1][lane] = d1; red[2][lane] = d2;
+            __syncthreads();
+            if (lane < nt) {
+                for (int q = 1; q < R; ++q) {
+                    d0 += red[0][lane + q * nt];
+                    d1 += red[1][lane + q * nt];
+                    d2 += red[2][lane + q * nt];
+                }
+            }
+        }
+        if (lane < nt) {
+            const int64_t i = perm[tb + lane];
+            E[i] = d0;
+            E[cap + i] = d1;
+            E[2 * cap + i] = d2;
+        }
+    }
+}
+
 hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int transposed,
-                           const DeviceField &f, Scratch &s)
+                           int precision, const DeviceField &f, Scratch &s)
 {
     if (f.np == 0) return hipSuccess;
     fmm_estr_gather_kernel<<<gridn(f.np, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
@@ -1285,7 +1524,13 @@ hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int tr
     const int nleaf = (int)S.plan.leaves.size();
     if (nleaf == 0) return hipSuccess;
 #define VPMX_FESTR(KK)                                                                          \
-    if (transposed)                                                                             \
+    if (precision && transposed)                                                                \
+        fmm_estr_mixed_kernel<KK, true><<<nleaf, P2P_BLOCK, 0, st>>>(                           \
+            S.leaves, S.cells, S.p2p_off, S.p2p_src, s.srcE, s.perm, f.cap, f.E);               \
+    else if (precision)                                                                         \
+        fmm_estr_mixed_kernel<KK, false><<<nleaf, P2P_BLOCK, 0, st>>>(                          \
+            S.leaves, S.cells, S.p2p_off, S.p2p_src, s.srcE, s.perm, f.cap, f.E);               \
+    else if (transposed)                                                                        \
         fmm_estr_kernel<KK, true><<<nleaf, P2P_BLOCK, 0, st>>>(S.leaves, S.cells, S.p2p_off,     \
                                                              S.p2p_src, s.srcE, s.perm, f.cap,  \
                                                              f.E);                               \
@@ -1423,23 +1668,30 @@ __global__ __launch_bounds__(FMM_BLOCK) void csr_kernel(const uint64_t *__restri
     for (int64_t i = t; i < nkeys; i += (int64_t)gridDim.x * FMM_BLOCK) src[i] = (int32_t)(keys[i] & mask);
 }
 
-// Particle pairs of the P2P lists (integer atomics: order-independent).
-__global__ __launch_bounds__(FMM_BLOCK) void p2p_count_kernel(const int64_t *__restrict__ off,
-                                                              const int32_t *__restrict__ src,
+// Particle pairs of the P2P lists, one thread per sorted (target, source) key,
+// a wave-then-block sum and one integer atomic per block (order-independent).
+__global__ __launch_bounds__(FMM_BLOCK) void p2p_count_kernel(const uint64_t *__restrict__ keys,
+                                                              int64_t n, int nb,
                                                               const FmmCellDev *__restrict__ cells,
-                                                              int64_t ncell,
                                                               unsigned long long *__restrict__ total)
 {
-    const int64_t c = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
-    if (c >= ncell) return;
-    const FmmCellDev A = cells[c];
+    __shared__ unsigned long long part[FMM_BLOCK / 64];
+    const uint64_t mask = ((uint64_t)1 << nb) - 1;
     unsigned long long s = 0;
-    for (int64_t e = off[c]; e < off[c + 1]; e++) {
-        const FmmCellDev B = cells[src[e]];
-        s += (unsigned long long)(B.end - B.begin);
+    for (int64_t i = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * FMM_BLOCK) {
+        const uint64_t k = keys[i];
+        const FmmCellDev A = cells[k >> nb], B = cells[k & mask];
+        s += (unsigned long long)(A.end - A.begin) * (unsigned long long)(B.end - B.begin);
     }
-    s *= (unsigned long long)(A.end - A.begin);
-    if (s) atomicAdd(total, s);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < FMM_BLOCK / 64; ++w) t += part[w];
+        if (t) atomicAdd(total, t);
+    }
 }
 
 template <class T>
@@ -1567,8 +1819,10 @@ hipError_t fmm_traverse_device(hipStream_t st, FmmState &S, int64_t ncell, doubl
         VPMX_CHK(hipGetLastError());
     }
     VPMX_CHK(hipMemsetAsync(B.totals + 3, 0, sizeof(int64_t), st));
-    p2p_count_kernel<<<gridn(ncell, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
-        S.p2p_off, S.p2p_src, S.cells, ncell, (unsigned long long *)(B.totals + 3));
+    // B.keys_tmp holds the sorted P2P keys (the last list above)
+    if (np2p > 0)
+        p2p_count_kernel<<<(unsigned)std::min<int64_t>(gridn(np2p, FMM_BLOCK), 2048), FMM_BLOCK, 0,
+                           st>>>(B.keys_tmp, np2p, nb, S.cells, (unsigned long long *)(B.totals + 3));
     VPMX_CHK(hipGetLastError());
     VPMX_CHK(hipMemcpyAsync(B.h_totals + 3, B.totals + 3, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     VPMX_CHK(hipStreamSynchronize(st));

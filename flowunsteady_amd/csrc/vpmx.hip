@@ -921,7 +921,8 @@ int enqueue_estr(vpmx_ctx *c, const vpmx_uj_params *U)
     TimedScope ts(c, T_ESTR);
     if (U->fmm) {
         if (!c->fmm) return fail(c, VPMX_EINVAL, "Estr with the FMM needs an FMM UJ first");
-        VPMX_HIP(c, launch_fmm_estr(c->stream, *c->fmm, U->kernel, U->transposed, c->f, c->s));
+        VPMX_HIP(c, launch_fmm_estr(c->stream, *c->fmm, U->kernel, U->transposed, U->precision,
+                                    c->f, c->s));
     } else {
         VPMX_HIP(c, launch_estr_f64(c->stream, U->kernel, U->transposed, c->f, c->s));
     }

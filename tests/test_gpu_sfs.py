@@ -86,3 +86,26 @@ def test_dynamic_sfs_mixed_runs():
     print("C mixed vs fp64: mean", Ca.mean(), Cb.mean(), "Gamma err", e)
     assert e[1] < 5e-3
     assert abs(Cb.mean() - Ca.mean()) < 0.05 * abs(Ca.mean()) + 1e-3
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans", "gaussian"])
+@pytest.mark.parametrize("transposed", [True, False])
+def test_estr_fmm_mixed_vs_oracle(kernel, transposed):
+    """The mixed SFS stretching over the near-field lists (FP32 pair math,
+    FP64 accumulation) against the oracle's FP64 all-pairs Estr on the same J
+    (the J the mixed UJ produced), so only the Estr pair precision and the
+    neglected far field (below e^-24.5 zeta(0) per pair for the gaussian
+    kernels) separate them."""
+    import ctypes
+    from flowunsteady_amd import _native as N
+    X, G, s, _ = workloads.rotor_wake(n=1 << 14, seed=3)
+    pf = make_field(X, G, s, kernel=kernel, precision="mixed", transposed=transposed,
+                    fmm=vpm.FMM(p=8, ncrit=50, theta=0.4, nonzero_sigma=True))
+    P = pf._uj_params(sfs=True, fmm=True)
+    ctx = vpm._context(0)
+    ctx.check(N.lib().vpmx_uj(ctx.ptr, pf._ptr(), pf.np, ctypes.byref(pf.layout), ctypes.byref(P)))
+    J = vpm.get_J(pf)
+    E = O.estr_direct(kernel, transposed, X, J, X, G, s, J)
+    e = normwise(vpm.get_SFS(pf), E)
+    print(kernel, transposed, "mixed Estr fmm vs oracle", e)
+    assert e[0] < 2e-5 and e[1] < 5e-6
