@@ -55,6 +55,17 @@ struct FmmCellDev {
     double cx, cy, cz, R, smax;
 };
 
+// Near-field work item: entries [e0, e1) of target leaf c's P2P list.  A leaf
+// whose list is longer than P2P_CHUNK entries is cut into several items (one
+// workgroup each), which write FP64 partial sums from slot `part` (the leaf's
+// particles, one slot each) for a fixed-order reduction; part < 0: the item is
+// the leaf's whole list and stores directly.
+struct P2PChunk {
+    int64_t e0, e1, part;
+    int32_t c, pad;
+};
+constexpr int P2P_CHUNK = 512;
+
 // Scratch of the device dual-tree traversal.
 struct DttBufs {
     int64_t cap_front = 0, cap_next = 0, cap_m2l = 0, cap_p2p = 0, cap_count = 0, cap_tmpk = 0;
@@ -134,6 +145,11 @@ struct FmmDev {
     const uint16_t *m2l_tile_idx = nullptr;
     const double *m2l_bscale = nullptr, *m2l_lscale = nullptr;
     const int4 *m2l_erec = nullptr;
+    // near-field work items (fmm_p2p_chunks)
+    const P2PChunk *chunks = nullptr;
+    const int64_t *chunk_off = nullptr, *part_base = nullptr;
+    double *part = nullptr;
+    int64_t nchunk = 0, nslot = 0;
 };
 
 // Per-context FMM state: host plan, operator tables and device buffers.
@@ -151,6 +167,15 @@ struct FmmState {
     uint16_t *m2l_tile_idx = nullptr;
     double *m2l_bscale = nullptr, *m2l_lscale = nullptr;
     int4 *m2l_erec = nullptr;
+    // near-field work items: per leaf chunk count / partial slots (scanned in
+    // place into offsets, nleaf + 1 each), the items, the partial sums
+    int64_t cap_leaf = 0, cap_chunk = 0, cap_part = 0, nchunk = 0, nslot = 0;
+    int64_t *chunk_buf = nullptr;   // 4 (cap_leaf + 1): counts, offsets, slot counts, bases
+    int64_t *chunk_off = nullptr, *part_base = nullptr, *h_chunk_tot = nullptr;
+    P2PChunk *chunks = nullptr;
+    double *part = nullptr;
+    uint8_t *chunk_tmp = nullptr;
+    size_t chunk_tmp_bytes = 0;
     // device tree and lists
     int64_t cap_cells = 0, cap_m2l = 0, cap_p2p = 0, cap_coef = 0, cap_n = 0;
     FmmCellDev *cells = nullptr;
@@ -176,6 +201,10 @@ struct FmmState {
 // precision 1 runs the FP32 pair math with FP64 accumulation.
 hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int transposed,
                            int precision, const DeviceField &f, Scratch &s);
+
+// Near-field work items of the current lists (S.chunks, S.nchunk, S.nslot;
+// S.part sized for 14 values per slot).  nleaf = S.plan.leaves.size().
+hipError_t fmm_p2p_chunks(hipStream_t st, FmmState &S, int64_t nleaf);
 
 // Dual tree traversal on the device: fills m2l_off/m2l_src and p2p_off/p2p_src
 // (CSR by target cell, sources ascending), growing the buffers in S.

@@ -705,20 +705,50 @@ __global__ __launch_bounds__(P2P_BLOCK) void l2p_kernel(const int32_t *__restric
     }
 }
 
-// P2P near field, FP64: one block (one wave) per target leaf, one target per
-// lane, source leaves of the list staged through LDS.  Adds to U, J.
+// Stores of the near-field kernels, shared with the reduction of partial sums:
+// FP64 P2P sums d = (U, J) added to U, J.
+__device__ __forceinline__ void p2p_f64_finish(const double d[12], int64_t i, int64_t cap,
+                                               double *__restrict__ U, double *__restrict__ J)
+{
+    for (int k = 0; k < 3; ++k) U[k * cap + i] += d[k];
+    for (int k = 0; k < 9; ++k) J[k * cap + i] += d[3 + k];
+}
+
+// Mixed P2P sums d = (U, W, J0..7) before -1/(4 pi): U and J added to U, J
+// (W the Kronecker part of J, J8 from the zero trace).
+__device__ __forceinline__ void p2p_mixed_finish(const double d[14], int64_t i, int64_t cap,
+                                                 double *__restrict__ U, double *__restrict__ J)
+{
+    const double k = -INV4PI;
+    U[i] += k * d[0]; U[cap + i] += k * d[1]; U[2 * cap + i] += k * d[2];
+    J[i] += k * d[6];
+    J[cap + i] += k * (d[7] - d[5]);
+    J[2 * cap + i] += k * (d[8] + d[4]);
+    J[3 * cap + i] += k * (d[9] + d[5]);
+    J[4 * cap + i] += k * d[10];
+    J[5 * cap + i] += k * (d[11] - d[3]);
+    J[6 * cap + i] += k * (d[12] - d[4]);
+    J[7 * cap + i] += k * (d[13] + d[3]);
+    J[8 * cap + i] += -k * (d[6] + d[10]);
+}
+
+// P2P near field, FP64: one block (one wave) per work item (a target leaf and
+// a run of its list), one target per lane, source leaves staged through LDS.
+// Adds to U, J (or writes partial sums).
 template <int KK>
-__global__ __launch_bounds__(P2P_BLOCK) void p2p_f64_kernel(const int32_t *__restrict__ leaves,
+__global__ __launch_bounds__(P2P_BLOCK) void p2p_f64_kernel(const P2PChunk *__restrict__ chunks,
                                                             const FmmCellDev *__restrict__ cells,
-                                                            const int64_t *__restrict__ off,
                                                             const int32_t *__restrict__ src,
                                                             const double4 *__restrict__ rec,
                                                             const int32_t *__restrict__ perm,
                                                             int64_t cap, double *__restrict__ U,
-                                                            double *__restrict__ J)
+                                                            double *__restrict__ J,
+                                                            double *__restrict__ part,
+                                                            int64_t pstride)
 {
     __shared__ double4 tile[2 * P2P_BLOCK];
-    const int c = leaves[blockIdx.x];
+    const P2PChunk ch = chunks[blockIdx.x];
+    const int c = ch.c;
     const FmmCellDev A = cells[c];
     for (int tb = A.begin; tb < A.end; tb += P2P_BLOCK) {
         const int it = tb + threadIdx.x;
@@ -730,7 +760,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void p2p_f64_kernel(const int32_t *__res
         }
         double u0 = 0, u1 = 0, u2 = 0, j0 = 0, j1 = 0, j2 = 0, j3 = 0, j4 = 0, j5 = 0, j6 = 0,
                j7 = 0, j8 = 0;
-        for (int64_t e = off[c]; e < off[c + 1]; e++) {
+        for (int64_t e = ch.e0; e < ch.e1; e++) {
             const FmmCellDev B = cells[src[e]];
             for (int sb = B.begin; sb < B.end; sb += P2P_BLOCK) {
                 __syncthreads();
@@ -768,11 +798,11 @@ __global__ __launch_bounds__(P2P_BLOCK) void p2p_f64_kernel(const int32_t *__res
             }
         }
         if (active) {
-            const int64_t i = perm[it];
-            U[i] += u0; U[cap + i] += u1; U[2 * cap + i] += u2;
-            J[i] += j0; J[cap + i] += j1; J[2 * cap + i] += j2;
-            J[3 * cap + i] += j3; J[4 * cap + i] += j4; J[5 * cap + i] += j5;
-            J[6 * cap + i] += j6; J[7 * cap + i] += j7; J[8 * cap + i] += j8;
+            const double d[12] = {u0, u1, u2, j0, j1, j2, j3, j4, j5, j6, j7, j8};
+            if (ch.part < 0)
+                p2p_f64_finish(d, perm[it], cap, U, J);
+            else
+                for (int k = 0; k < 12; ++k) part[k * pstride + ch.part + (it - A.begin)] = d[k];
         }
     }
 }
@@ -841,23 +871,25 @@ __device__ __forceinline__ void p2p_mixed_tile(const float4 *tpos, const float4 
 
 template <int KK>
 __global__ __launch_bounds__(P2P_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
-void p2p_mixed_kernel(const int32_t *__restrict__ leaves,
+void p2p_mixed_kernel(const P2PChunk *__restrict__ chunks,
                                                               const FmmCellDev *__restrict__ cells,
-                                                              const int64_t *__restrict__ off,
                                                               const int32_t *__restrict__ src,
                                                               const double4 *__restrict__ rec,
                                                               const int32_t *__restrict__ perm,
                                                               int64_t cap, double *__restrict__ U,
-                                                              double *__restrict__ J)
+                                                              double *__restrict__ J,
+                                                              double *__restrict__ part,
+                                                              int64_t pstride)
 {
     __shared__ float4 tpos[P2P_BLOCK];
     __shared__ float4 tgam[P2P_BLOCK];
     __shared__ int32_t lbeg[P2P_SLOTS], lpre[P2P_SLOTS + 1];
     __shared__ double red[14][P2P_BLOCK];
     const int lane = threadIdx.x;
-    const int c = leaves[blockIdx.x];
+    const P2PChunk ch = chunks[blockIdx.x];
+    const int c = ch.c;
     const FmmCellDev A = cells[c];
-    const int64_t e0 = off[c], e1 = off[c + 1];
+    const int64_t e0 = ch.e0, e1 = ch.e1;
     for (int tb = A.begin; tb < A.end; tb += P2P_BLOCK) {
         const int nt = min(P2P_BLOCK, A.end - tb);
         const int R = nt <= 8 ? 8 : nt <= 16 ? 4 : nt <= 32 ? 2 : 1;
@@ -936,18 +968,12 @@ void p2p_mixed_kernel(const int32_t *__restrict__ leaves,
             }
         }
         if (lane < nt) {
-            const int64_t i = perm[tb + lane];
-            const double k = -INV4PI;
-            U[i] += k * dU0; U[cap + i] += k * dU1; U[2 * cap + i] += k * dU2;
-            J[i] += k * dJ0;
-            J[cap + i] += k * (dJ1 - dW2);
-            J[2 * cap + i] += k * (dJ2 + dW1);
-            J[3 * cap + i] += k * (dJ3 + dW2);
-            J[4 * cap + i] += k * dJ4;
-            J[5 * cap + i] += k * (dJ5 - dW0);
-            J[6 * cap + i] += k * (dJ6 - dW1);
-            J[7 * cap + i] += k * (dJ7 + dW0);
-            J[8 * cap + i] += -k * (dJ0 + dJ4);
+            const double d[14] = {dU0, dU1, dU2, dW0, dW1, dW2, dJ0, dJ1, dJ2, dJ3, dJ4, dJ5, dJ6, dJ7};
+            if (ch.part < 0)
+                p2p_mixed_finish(d, perm[tb + lane], cap, U, J);
+            else
+                for (int k = 0; k < 14; ++k)
+                    part[k * pstride + ch.part + (tb + lane - A.begin)] = d[k];
         }
     }
 }
@@ -1255,6 +1281,15 @@ void fmm_build_tables(FmmTables &T, int P)
 // below e^{-24.5} zeta(0) for the gaussian kernels, so the sum over the P2P
 // lists is the whole sum to that precision.
 
+// Estr store: the sums overwrite E.
+__device__ __forceinline__ void estr_finish(const double d[3], int64_t i, int64_t cap,
+                                            double *__restrict__ E)
+{
+    E[i] = d[0];
+    E[cap + i] = d[1];
+    E[2 * cap + i] = d[2];
+}
+
 // Sorted Estr source records (x, y, z, 1/sigma) (Gamma, 0) (J0..3) (J4..7) (J8, 0, 0, 0).
 __global__ __launch_bounds__(FMM_BLOCK) void fmm_estr_gather_kernel(
     const double *__restrict__ X, const double *__restrict__ G, const double *__restrict__ sg,
@@ -1272,16 +1307,18 @@ __global__ __launch_bounds__(FMM_BLOCK) void fmm_estr_gather_kernel(
 }
 
 template <int KK, bool TRANSPOSED>
-__global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_kernel(const int32_t *__restrict__ leaves,
+__global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_kernel(const P2PChunk *__restrict__ chunks,
                                                              const FmmCellDev *__restrict__ cells,
-                                                             const int64_t *__restrict__ off,
                                                              const int32_t *__restrict__ src,
                                                              const double4 *__restrict__ rec,
                                                              const int32_t *__restrict__ perm,
-                                                             int64_t cap, double *__restrict__ E)
+                                                             int64_t cap, double *__restrict__ E,
+                                                             double *__restrict__ part,
+                                                             int64_t pstride)
 {
     __shared__ double4 tile[5 * P2P_BLOCK];
-    const int c = leaves[blockIdx.x];
+    const P2PChunk ch = chunks[blockIdx.x];
+    const int c = ch.c;
     const FmmCellDev A = cells[c];
     for (int tb = A.begin; tb < A.end; tb += P2P_BLOCK) {
         const int it = tb + threadIdx.x;
@@ -1296,7 +1333,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_kernel(const int32_t *__re
             Ji[4] = q3.x; Ji[5] = q3.y; Ji[6] = q3.z; Ji[7] = q3.w; Ji[8] = q4.x;
         }
         double e0 = 0, e1 = 0, e2 = 0;
-        for (int64_t e = off[c]; e < off[c + 1]; e++) {
+        for (int64_t e = ch.e0; e < ch.e1; e++) {
             const FmmCellDev B = cells[src[e]];
             for (int sb = B.begin; sb < B.end; sb += P2P_BLOCK) {
                 __syncthreads();
@@ -1332,10 +1369,11 @@ __global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_kernel(const int32_t *__re
             }
         }
         if (active) {
-            const int64_t i = perm[it];
-            E[i] = e0;
-            E[cap + i] = e1;
-            E[2 * cap + i] = e2;
+            const double d[3] = {e0, e1, e2};
+            if (ch.part < 0)
+                estr_finish(d, perm[it], cap, E);
+            else
+                for (int k = 0; k < 3; ++k) part[k * pstride + ch.part + (it - A.begin)] = d[k];
         }
     }
 }
@@ -1411,19 +1449,20 @@ __device__ __forceinline__ void estr_mixed_tile(const float4 *sp, const float4 *
 
 template <int KK, bool TRANSPOSED>
 __global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_mixed_kernel(
-    const int32_t *__restrict__ leaves, const FmmCellDev *__restrict__ cells,
-    const int64_t *__restrict__ off, const int32_t *__restrict__ src,
-    const double4 *__restrict__ rec, const int32_t *__restrict__ perm, int64_t cap,
-    double *__restrict__ E)
+    const P2PChunk *__restrict__ chunks, const FmmCellDev *__restrict__ cells,
+    const int32_t *__restrict__ src, const double4 *__restrict__ rec,
+    const int32_t *__restrict__ perm, int64_t cap, double *__restrict__ E,
+    double *__restrict__ part, int64_t pstride)
 {
     __shared__ float4 sp[P2P_BLOCK], sg[P2P_BLOCK], sj0[P2P_BLOCK], sj1[P2P_BLOCK];
     __shared__ float sj8[P2P_BLOCK];
     __shared__ int32_t lbeg[P2P_SLOTS], lpre[P2P_SLOTS + 1];
     __shared__ double red[3][P2P_BLOCK];
     const int lane = threadIdx.x;
-    const int c = leaves[blockIdx.x];
+    const P2PChunk ch = chunks[blockIdx.x];
+    const int c = ch.c;
     const FmmCellDev A = cells[c];
-    const int64_t e0 = off[c], e1 = off[c + 1];
+    const int64_t e0 = ch.e0, e1 = ch.e1;
     for (int tb = A.begin; tb < A.end; tb += P2P_BLOCK) {
         const int nt = min(P2P_BLOCK, A.end - tb);
         const int R = nt <= 8 ? 8 : nt <= 16 ? 4 : nt <= 32 ? 2 : 1;
@@ -1505,12 +1544,57 @@ __global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_mixed_kernel(
             }
         }
         if (lane < nt) {
-            const int64_t i = perm[tb + lane];
-            E[i] = d0;
-            E[cap + i] = d1;
-            E[2 * cap + i] = d2;
+            const double d[3] = {d0, d1, d2};
+            if (ch.part < 0)
+                estr_finish(d, perm[tb + lane], cap, E);
+            else
+                for (int k = 0; k < 3; ++k) part[k * pstride + ch.part + (tb + lane - A.begin)] = d[k];
         }
     }
+}
+
+// Partial sums of the leaves cut into several work items, added in item order
+// and stored.  MODE 0: FP64 P2P (12 sums), 1: mixed P2P (14), 2: Estr (3).
+template <int MODE>
+__global__ __launch_bounds__(P2P_BLOCK) void p2p_part_reduce_kernel(
+    const int32_t *__restrict__ leaves, const int64_t *__restrict__ coff,
+    const int64_t *__restrict__ pbase, const FmmCellDev *__restrict__ cells,
+    const double *__restrict__ part, int64_t pstride, const int32_t *__restrict__ perm,
+    int64_t cap, double *__restrict__ U, double *__restrict__ J, double *__restrict__ E)
+{
+    constexpr int NV = MODE == 0 ? 12 : MODE == 1 ? 14 : 3;
+    const int64_t li = blockIdx.x;
+    const int64_t n = coff[li + 1] - coff[li];
+    if (n <= 1) return;
+    const FmmCellDev A = cells[leaves[li]];
+    const int nt = A.end - A.begin;
+    const int64_t pb = pbase[li];
+    for (int t = threadIdx.x; t < nt; t += P2P_BLOCK) {
+        double d[NV];
+        for (int k = 0; k < NV; ++k) d[k] = 0.0;
+        for (int64_t j = 0; j < n; ++j)
+            for (int k = 0; k < NV; ++k) d[k] += part[k * pstride + pb + j * nt + t];
+        const int64_t i = perm[A.begin + t];
+        if constexpr (MODE == 0)
+            p2p_f64_finish(d, i, cap, U, J);
+        else if constexpr (MODE == 1)
+            p2p_mixed_finish(d, i, cap, U, J);
+        else
+            estr_finish(d, i, cap, E);
+    }
+}
+
+template <int MODE>
+static hipError_t launch_part_reduce(hipStream_t st, const int32_t *leaves, int64_t nleaf,
+                                     const int64_t *coff, const int64_t *pbase,
+                                     const FmmCellDev *cells, const double *part, int64_t nslot,
+                                     const int32_t *perm, int64_t cap, double *U, double *J,
+                                     double *E)
+{
+    if (nslot == 0 || nleaf == 0) return hipSuccess;
+    p2p_part_reduce_kernel<MODE><<<(unsigned)nleaf, P2P_BLOCK, 0, st>>>(
+        leaves, coff, pbase, cells, part, nslot, perm, cap, U, J, E);
+    return hipGetLastError();
 }
 
 hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int transposed,
@@ -1521,23 +1605,23 @@ hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int tr
         f.X, f.G, f.sigma, f.J, s.perm, f.np, f.cap, s.srcE);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int nleaf = (int)S.plan.leaves.size();
-    if (nleaf == 0) return hipSuccess;
+    const int64_t nleaf = (int64_t)S.plan.leaves.size();
+    if (nleaf == 0 || S.nchunk == 0) return hipSuccess;
+    const unsigned g = (unsigned)S.nchunk;
 #define VPMX_FESTR(KK)                                                                          \
     if (precision && transposed)                                                                \
-        fmm_estr_mixed_kernel<KK, true><<<nleaf, P2P_BLOCK, 0, st>>>(                           \
-            S.leaves, S.cells, S.p2p_off, S.p2p_src, s.srcE, s.perm, f.cap, f.E);               \
+        fmm_estr_mixed_kernel<KK, true><<<g, P2P_BLOCK, 0, st>>>(                               \
+            S.chunks, S.cells, S.p2p_src, s.srcE, s.perm, f.cap, f.E, S.part, S.nslot);         \
     else if (precision)                                                                         \
-        fmm_estr_mixed_kernel<KK, false><<<nleaf, P2P_BLOCK, 0, st>>>(                          \
-            S.leaves, S.cells, S.p2p_off, S.p2p_src, s.srcE, s.perm, f.cap, f.E);               \
+        fmm_estr_mixed_kernel<KK, false><<<g, P2P_BLOCK, 0, st>>>(                              \
+            S.chunks, S.cells, S.p2p_src, s.srcE, s.perm, f.cap, f.E, S.part, S.nslot);         \
     else if (transposed)                                                                        \
-        fmm_estr_kernel<KK, true><<<nleaf, P2P_BLOCK, 0, st>>>(S.leaves, S.cells, S.p2p_off,     \
-                                                             S.p2p_src, s.srcE, s.perm, f.cap,  \
-                                                             f.E);                               \
+        fmm_estr_kernel<KK, true><<<g, P2P_BLOCK, 0, st>>>(S.chunks, S.cells, S.p2p_src, s.srcE, \
+                                                         s.perm, f.cap, f.E, S.part, S.nslot);  \
     else                                                                                        \
-        fmm_estr_kernel<KK, false><<<nleaf, P2P_BLOCK, 0, st>>>(S.leaves, S.cells, S.p2p_off,    \
-                                                              S.p2p_src, s.srcE, s.perm, f.cap, \
-                                                              f.E);
+        fmm_estr_kernel<KK, false><<<g, P2P_BLOCK, 0, st>>>(S.chunks, S.cells, S.p2p_src,        \
+                                                          s.srcE, s.perm, f.cap, f.E, S.part,   \
+                                                          S.nslot);
     switch (kernel) {
     case GAUSSIAN: VPMX_FESTR(GAUSSIAN); break;
     case GAUSSIANERF: VPMX_FESTR(GAUSSIANERF); break;
@@ -1545,7 +1629,10 @@ hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int tr
     default: return hipErrorInvalidValue;
     }
 #undef VPMX_FESTR
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_part_reduce<2>(st, S.leaves, nleaf, S.chunk_off, S.part_base, S.cells, S.part,
+                                 S.nslot, s.perm, f.cap, nullptr, nullptr, f.E);
 }
 
 // ---------------------------------------------------------------------------
@@ -1832,6 +1919,97 @@ hipError_t fmm_traverse_device(hipStream_t st, FmmState &S, int64_t ncell, doubl
     return hipSuccess;
 }
 
+// Near-field work items: a leaf's P2P list in runs of at most P2P_CHUNK
+// entries; leaves with several runs get partial-sum slots (their particle count
+// per run).  Counts, exclusive scans (offsets and bases, with the totals at
+// index nleaf), then the items.
+__global__ __launch_bounds__(FMM_BLOCK) void p2p_chunk_count_kernel(
+    const int32_t *__restrict__ leaves, int64_t nleaf, const int64_t *__restrict__ off,
+    const FmmCellDev *__restrict__ cells, int64_t *__restrict__ nch, int64_t *__restrict__ nslot)
+{
+    const int64_t li = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (li > nleaf) return;
+    if (li == nleaf) {
+        nch[li] = 0;
+        nslot[li] = 0;
+        return;
+    }
+    const int c = leaves[li];
+    const int64_t ne = off[c + 1] - off[c];
+    const int64_t n = ne > P2P_CHUNK ? (ne + P2P_CHUNK - 1) / P2P_CHUNK : 1;
+    nch[li] = n;
+    nslot[li] = n > 1 ? n * (int64_t)(cells[c].end - cells[c].begin) : 0;
+}
+
+__global__ __launch_bounds__(FMM_BLOCK) void p2p_chunk_fill_kernel(
+    const int32_t *__restrict__ leaves, int64_t nleaf, const int64_t *__restrict__ off,
+    const FmmCellDev *__restrict__ cells, const int64_t *__restrict__ coff,
+    const int64_t *__restrict__ pbase, P2PChunk *__restrict__ chunks)
+{
+    const int64_t li = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (li >= nleaf) return;
+    const int c = leaves[li];
+    const int64_t n = coff[li + 1] - coff[li], nt = cells[c].end - cells[c].begin;
+    for (int64_t j = 0; j < n; ++j) {
+        P2PChunk q;
+        q.c = c;
+        q.pad = 0;
+        q.e0 = off[c] + j * P2P_CHUNK;
+        q.e1 = min(off[c + 1], q.e0 + P2P_CHUNK);
+        q.part = n > 1 ? pbase[li] + j * nt : -1;
+        chunks[coff[li] + j] = q;
+    }
+}
+
+hipError_t fmm_p2p_chunks(hipStream_t st, FmmState &S, int64_t nleaf)
+{
+    S.nchunk = S.nslot = 0;
+    if (nleaf == 0) return hipSuccess;
+    if (nleaf > S.cap_leaf || !S.chunk_buf) {
+        const int64_t cap = std::max<int64_t>(nleaf, S.cap_leaf + S.cap_leaf / 2);
+        if (S.chunk_buf) (void)hipFree(S.chunk_buf);
+        S.chunk_buf = nullptr;
+        VPMX_CHK(hipMalloc((void **)&S.chunk_buf, 4 * (cap + 1) * sizeof(int64_t)));
+        S.cap_leaf = cap;
+    }
+    if (!S.h_chunk_tot)
+        VPMX_CHK(hipHostMalloc((void **)&S.h_chunk_tot, 2 * sizeof(int64_t), hipHostMallocDefault));
+    const int64_t L1 = S.cap_leaf + 1;
+    int64_t *ncnt = S.chunk_buf, *scnt = S.chunk_buf + 2 * L1;
+    S.chunk_off = S.chunk_buf + L1;
+    S.part_base = S.chunk_buf + 3 * L1;
+    p2p_chunk_count_kernel<<<gridn(nleaf + 1, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
+        S.leaves, nleaf, S.p2p_off, S.cells, ncnt, scnt);
+    VPMX_CHK(hipGetLastError());
+    size_t need = 0;
+    VPMX_CHK(rocprim::exclusive_scan(nullptr, need, ncnt, S.chunk_off, (int64_t)0,
+                                     (size_t)(nleaf + 1), rocprim::plus<int64_t>(), st));
+    if (need > S.chunk_tmp_bytes) {
+        if (S.chunk_tmp) (void)hipFree(S.chunk_tmp);
+        S.chunk_tmp = nullptr;
+        VPMX_CHK(hipMalloc((void **)&S.chunk_tmp, need));
+        S.chunk_tmp_bytes = need;
+    }
+    size_t tb = S.chunk_tmp_bytes;
+    VPMX_CHK(rocprim::exclusive_scan(S.chunk_tmp, tb, ncnt, S.chunk_off, (int64_t)0,
+                                     (size_t)(nleaf + 1), rocprim::plus<int64_t>(), st));
+    tb = S.chunk_tmp_bytes;
+    VPMX_CHK(rocprim::exclusive_scan(S.chunk_tmp, tb, scnt, S.part_base, (int64_t)0,
+                                     (size_t)(nleaf + 1), rocprim::plus<int64_t>(), st));
+    VPMX_CHK(hipMemcpyAsync(S.h_chunk_tot, S.chunk_off + nleaf, sizeof(int64_t),
+                            hipMemcpyDeviceToHost, st));
+    VPMX_CHK(hipMemcpyAsync(S.h_chunk_tot + 1, S.part_base + nleaf, sizeof(int64_t),
+                            hipMemcpyDeviceToHost, st));
+    VPMX_CHK(hipStreamSynchronize(st));
+    S.nchunk = S.h_chunk_tot[0];
+    S.nslot = S.h_chunk_tot[1];
+    VPMX_CHK(grow(&S.chunks, S.cap_chunk, std::max<int64_t>(S.nchunk, 1), st));
+    VPMX_CHK(grow(&S.part, S.cap_part, std::max<int64_t>(14 * S.nslot, 1), st));
+    p2p_chunk_fill_kernel<<<gridn(nleaf, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
+        S.leaves, nleaf, S.p2p_off, S.cells, S.chunk_off, S.part_base, S.chunks);
+    return hipGetLastError();
+}
+
 #undef VPMX_CHK
 
 // Host version of the key computation and the whole plan, for checking the
@@ -2009,14 +2187,14 @@ hipError_t launch_fmm_eval(hipStream_t st, const FmmDev &d, const FmmPlan &pl, i
                                                   d.d1, d.d2a, d.d2b, d.L, f.cap, f.U, f.J, reset);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+        const unsigned g = (unsigned)d.nchunk;
 #define VPMX_P2P(KK)                                                                              \
     if (precision == 0)                                                                           \
-        p2p_f64_kernel<KK><<<d.nleaf, P2P_BLOCK, 0, st>>>(d.leaves, d.cells, d.p2p_off, d.p2p_src, \
-                                                         d.rec, d.perm, f.cap, f.U, f.J);          \
+        p2p_f64_kernel<KK><<<g, P2P_BLOCK, 0, st>>>(d.chunks, d.cells, d.p2p_src, d.rec, d.perm,   \
+                                                   f.cap, f.U, f.J, d.part, d.nslot);             \
     else                                                                                          \
-        p2p_mixed_kernel<KK><<<d.nleaf, P2P_BLOCK, 0, st>>>(d.leaves, d.cells, d.p2p_off,          \
-                                                           d.p2p_src, d.rec, d.perm, f.cap, f.U,  \
-                                                           f.J);
+        p2p_mixed_kernel<KK><<<g, P2P_BLOCK, 0, st>>>(d.chunks, d.cells, d.p2p_src, d.rec, d.perm, \
+                                                     f.cap, f.U, f.J, d.part, d.nslot);
         switch (kernel) {
         case SINGULAR: VPMX_P2P(SINGULAR); break;
         case GAUSSIAN: VPMX_P2P(GAUSSIAN); break;
@@ -2026,6 +2204,12 @@ hipError_t launch_fmm_eval(hipStream_t st, const FmmDev &d, const FmmPlan &pl, i
         }
 #undef VPMX_P2P
         e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        e = precision == 0
+                ? launch_part_reduce<0>(st, d.leaves, d.nleaf, d.chunk_off, d.part_base, d.cells,
+                                        d.part, d.nslot, d.perm, f.cap, f.U, f.J, nullptr)
+                : launch_part_reduce<1>(st, d.leaves, d.nleaf, d.chunk_off, d.part_base, d.cells,
+                                        d.part, d.nslot, d.perm, f.cap, f.U, f.J, nullptr);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -664,7 +664,8 @@ void free_fmm(FmmState *S)
     void *ptrs[] = {S->mi, S->m2m_off, S->l2l_off, S->m2l_toff, S->rec1, S->rec2, S->d1, S->d2a,
                     S->d2b, S->m2m_idx, S->l2l_idx, S->m2l_tidx, S->m2m_coef, S->l2l_coef,
                     S->m2l_tcoef, S->m2l_chunks, S->m2l_cfirst, S->m2l_tile_idx, S->m2l_bscale,
-                    S->m2l_lscale, S->m2l_erec, S->cells, S->leaves, S->parents, S->kids, S->parent,
+                    S->m2l_lscale, S->m2l_erec, S->chunk_buf, S->chunks, S->part, S->chunk_tmp,
+                    S->cells, S->leaves, S->parents, S->kids, S->parent,
                     S->m2l_off, S->p2p_off, S->m2l_src, S->p2p_src, S->M, S->L, S->kin, S->kout,
                     S->ssig, S->sort_tmp};
     for (void *p : ptrs)
@@ -676,6 +677,7 @@ void free_fmm(FmmState *S)
         if (p) (void)hipFree(p);
     if (B.h_totals) (void)hipHostFree(B.h_totals);
     if (S->h_keys) (void)hipHostFree(S->h_keys);
+    if (S->h_chunk_tot) (void)hipHostFree(S->h_chunk_tot);
     if (S->h_ssig) (void)hipHostFree(S->h_ssig);
     delete S;
 }
@@ -809,6 +811,7 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
     // device: dual tree traversal -> M2L and P2P lists (CSR by target)
     DttResult dr;
     VPMX_HIP(c, fmm_traverse_device(st, S, nc, P->theta, sigcut, dr));
+    VPMX_HIP(c, fmm_p2p_chunks(st, S, (int64_t)pl.leaves.size()));
     const int64_t nm2l = dr.nm2l, np2p = dr.np2p, pairs = dr.p2p_pairs;
     const double t3 = now_ms();
 
@@ -853,6 +856,12 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
     d.m2l_bscale = S.m2l_bscale;
     d.m2l_lscale = S.m2l_lscale;
     d.m2l_erec = S.m2l_erec;
+    d.chunks = S.chunks;
+    d.chunk_off = S.chunk_off;
+    d.part_base = S.part_base;
+    d.part = S.part;
+    d.nchunk = S.nchunk;
+    d.nslot = S.nslot;
     const double ms_before = c->timer.total_ms[T_FMM];
     {
         TimedScope ts(c, T_FMM);

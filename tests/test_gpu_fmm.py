@@ -138,3 +138,47 @@ def test_geometric_criterion_matches_the_oracle():
     run(pf7)
     assert vpm._context(0).fmm_stats()["p2p_pairs"] >= st["p2p_pairs"]
 
+
+
+def clustered(n=6000, nout=40, seed=1):
+    """A dense cube of particles beside sparse outliers: with ncrit = 4 the
+    outliers' large leaves see thousands of the cluster's leaves in their near
+    field (484 list entries per leaf on average)."""
+    rng = np.random.default_rng(seed)
+    X = np.vstack([rng.uniform(-0.05, 0.05, (n, 3)) + 0.3, rng.uniform(-1, 1, (nout, 3))])
+    G = rng.normal(0, 1e-3, X.shape)
+    return X, G, np.full(len(X), 0.004)
+
+
+def test_fmm_long_near_lists():
+    """Near-field lists longer than one work item (512 source leaves) are cut
+    into several workgroups whose FP64 partial sums are added in order: FP64
+    against the oracle's CPU FMM, mixed against FP64, and the SFS stretching
+    over the same lists (FP64 and mixed) against the oracle's all-pairs Estr."""
+    import ctypes
+    from flowunsteady_amd import _native as N
+    X, G, s = clustered()
+    Uo, Jo, so = O.uj_fmm("gaussianerf", X, G, s, p=4, ncrit=4, theta=0.4, sigcut=7.0)
+    assert so["np2p"] > 400 * so["nleaf"]
+    out = {}
+    for prec in ("fp64", "mixed"):
+        pf = make_field(X, G, s, precision=prec,
+                        fmm=vpm.FMM(p=4, ncrit=4, theta=0.4, nonzero_sigma=True))
+        P = pf._uj_params(sfs=True, fmm=True)
+        ctx = vpm._context(0)
+        ctx.check(N.lib().vpmx_uj(ctx.ptr, pf._ptr(), pf.np, ctypes.byref(pf.layout),
+                                  ctypes.byref(P)))
+        st = ctx.fmm_stats()
+        assert int(st["p2p_pairs"]) == so["p2p_pairs"]
+        U, J, E = vpm.get_U(pf).copy(), vpm.get_J(pf).copy(), vpm.get_SFS(pf).copy()
+        Eo = O.estr_direct("gaussianerf", True, X, J, X, G, s, J)
+        out[prec] = (U, J)
+        ee = normwise(E, Eo)
+        print(prec, "estr vs oracle", ee)
+        assert ee[0] < (1e-9 if prec == "fp64" else 2e-5)
+    eu, ej = normwise(out["fp64"][0], Uo), normwise(out["fp64"][1], Jo)
+    print("fp64 vs oracle fmm", eu, ej)
+    assert eu[0] < 1e-12 and ej[0] < 1e-12
+    mu, mj = normwise(out["mixed"][0], out["fp64"][0]), normwise(out["mixed"][1], out["fp64"][1])
+    print("mixed vs fp64", mu, mj)
+    assert mu[0] < 5e-6 and mj[0] < 1e-5
