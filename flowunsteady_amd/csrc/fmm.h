@@ -46,7 +46,13 @@ constexpr int m2l_base(int P, int ta)
     for (int t = 0; t < ta; ++t) s += m2l_nrows(P, t);
     return s;
 }
-constexpr int m2l_ntiles(int P) { return m2l_base(P, m2l_ntile1(P)); }
+// Total tiles: sum over b-tiles of their a-tile counts (= sum of m2l_nrows).
+constexpr int m2l_ntiles(int P)
+{
+    int s = 0;
+    for (int tb = 0; tb < m2l_ntile1(P); ++tb) s += m2l_natile(P, tb);
+    return s;
+}
 // Tile count padded to whole batches of 8 (the index table's row length).
 constexpr int m2l_ntiles8(int P) { return (m2l_ntiles(P) + 7) / 8 * 8; }
 

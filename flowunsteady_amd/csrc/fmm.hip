@@ -477,7 +477,8 @@ __global__ __launch_bounds__(FMM_BLOCK) void m2l_prescale_kernel(double *__restr
 // indices of k - e_i and k - 2 e_i (K where absent: the zero slot) and k:
 // (j1x | j1y << 16, j1z | j2x << 16, j2y | j2z << 16, kx | ky << 8 | kz << 16).
 // The four sources' E are rows of m2l_erow(P) doubles: K + 1 (the zero slot at
-// K), rounded so that the rows start 16 banks apart.
+// K), rounded so that the rows start 16 banks apart (the layout measured
+// fastest among the ones tried).
 template <int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P <= 8 ? 2 : 1)))
 void m2l_mfma_kernel(
