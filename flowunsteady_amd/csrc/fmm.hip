@@ -26,6 +26,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -992,61 +993,119 @@ static int64_t lower_bound_key(const uint64_t *key, int64_t b, int64_t e, uint64
     return b;
 }
 
+// fn(begin, end) over [0, n) on up to 16 host threads (serially when n is small).
+template <class F>
+static void host_parallel_for(int64_t n, F fn)
+{
+    const int64_t grain = 4096;
+    int nt = (int)std::min<int64_t>((n + grain - 1) / grain,
+                                    std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    if (nt <= 1) {
+        fn((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (int t = 1; t < nt; t++) {
+        const int64_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b < e) th.emplace_back(fn, b, e);
+    }
+    fn((int64_t)0, std::min(n, chunk));
+    for (auto &x : th) x.join();
+}
+
+// Octree from the sorted keys, level by level: the cells of a level split in
+// parallel (each non-empty octant of a cell with more than ncrit particles is
+// a child, its range found by binary search in the parent's key range), and
+// the children are appended in the order of their parents and octants -- the
+// breadth-first order of a sequential queue, so the tree does not depend on
+// the thread count.
 void fmm_build_tree(FmmPlan &pl, const uint64_t *key, const double *ssig, int64_t n, int ncrit)
 {
     pl.cells.clear();
-    std::vector<uint64_t> prefix;
     FmmCellHost root{};
     root.begin = 0; root.end = n; root.level = 0; root.child0 = -1; root.nchild = 0;
+    root.prefix = 0;
     pl.cells.push_back(root);
-    prefix.push_back(0);
-    for (size_t ci = 0; ci < pl.cells.size(); ci++) {
-        const FmmCellHost C = pl.cells[ci];
-        if (C.end - C.begin <= ncrit || C.level >= KEYBITS) continue;
-        const int shift = 3 * (KEYBITS - C.level - 1);
-        const int64_t first = (int64_t)pl.cells.size();
-        int nch = 0;
-        for (uint64_t o = 0; o < 8; o++) {
-            const uint64_t cp = (prefix[ci] << 3) | o;
-            const int64_t b = lower_bound_key(key, C.begin, C.end, cp << shift);
-            const int64_t e = lower_bound_key(key, C.begin, C.end, (cp + 1) << shift);
-            if (e == b) continue;
-            FmmCellHost D{};
-            D.begin = b; D.end = e; D.level = C.level + 1; D.child0 = -1; D.nchild = 0;
-            D.prefix = cp;
-            pl.cells.push_back(D);
-            prefix.push_back(cp);
-            nch++;
+    struct Kids {
+        int n;
+        int64_t b[8], e[8];
+    };
+    std::vector<Kids> kids;
+    for (int64_t lb = 0, le = 1; lb < le; lb = le, le = (int64_t)pl.cells.size()) {
+        kids.assign(le - lb, Kids{});
+        host_parallel_for(le - lb, [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; i++) {
+                const FmmCellHost &C = pl.cells[lb + i];
+                Kids &K = kids[i];
+                K.n = 0;
+                if (C.end - C.begin <= ncrit || C.level >= KEYBITS) continue;
+                const int shift = 3 * (KEYBITS - C.level - 1);
+                for (uint64_t o = 0; o < 8; o++) {
+                    const uint64_t cp = (C.prefix << 3) | o;
+                    const int64_t b = lower_bound_key(key, C.begin, C.end, cp << shift);
+                    const int64_t e = lower_bound_key(key, C.begin, C.end, (cp + 1) << shift);
+                    if (e == b) continue;
+                    K.b[o] = b;
+                    K.e[o] = e;
+                    K.n |= 1 << o;
+                }
+            }
+        });
+        for (int64_t i = 0; i < le - lb; i++) {
+            if (!kids[i].n) continue;
+            const int64_t first = (int64_t)pl.cells.size();
+            const FmmCellHost P = pl.cells[lb + i];
+            int nch = 0;
+            for (int o = 0; o < 8; o++) {
+                if (!(kids[i].n >> o & 1)) continue;
+                FmmCellHost D{};
+                D.begin = kids[i].b[o]; D.end = kids[i].e[o]; D.level = P.level + 1;
+                D.child0 = -1; D.nchild = 0;
+                D.prefix = (P.prefix << 3) | (uint64_t)o;
+                pl.cells.push_back(D);
+                nch++;
+            }
+            pl.cells[lb + i].child0 = first;
+            pl.cells[lb + i].nchild = nch;
         }
-        pl.cells[ci].child0 = first;
-        pl.cells[ci].nchild = nch;
     }
     // geometry
-    for (size_t ci = 0; ci < pl.cells.size(); ci++) {
-        FmmCellHost &C = pl.cells[ci];
-        const uint64_t pre = prefix[ci];
-        uint64_t ix = 0, iy = 0, iz = 0;
-        for (int l = 0; l < C.level; l++) {
-            const uint64_t o = (pre >> (3 * (C.level - 1 - l))) & 7u;
-            ix = (ix << 1) | (o & 1u);
-            iy = (iy << 1) | ((o >> 1) & 1u);
-            iz = (iz << 1) | ((o >> 2) & 1u);
+    host_parallel_for((int64_t)pl.cells.size(), [&](int64_t i0, int64_t i1) {
+        for (int64_t ci = i0; ci < i1; ci++) {
+            FmmCellHost &C = pl.cells[ci];
+            const uint64_t pre = C.prefix;
+            uint64_t ix = 0, iy = 0, iz = 0;
+            for (int l = 0; l < C.level; l++) {
+                const uint64_t o = (pre >> (3 * (C.level - 1 - l))) & 7u;
+                ix = (ix << 1) | (o & 1u);
+                iy = (iy << 1) | ((o >> 1) & 1u);
+                iz = (iz << 1) | ((o >> 2) & 1u);
+            }
+            const double h = pl.ext / (double)((uint64_t)1 << C.level);
+            C.c[0] = pl.lo[0] + ((double)ix + 0.5) * h;
+            C.c[1] = pl.lo[1] + ((double)iy + 0.5) * h;
+            C.c[2] = pl.lo[2] + ((double)iz + 0.5) * h;
+            C.R = 0.8660254037844386 * h;
         }
-        const double h = pl.ext / (double)((uint64_t)1 << C.level);
-        C.c[0] = pl.lo[0] + ((double)ix + 0.5) * h;
-        C.c[1] = pl.lo[1] + ((double)iy + 0.5) * h;
-        C.c[2] = pl.lo[2] + ((double)iz + 0.5) * h;
-        C.R = 0.8660254037844386 * h;
-    }
-    // sigma max, bottom-up
-    for (int64_t ci = (int64_t)pl.cells.size() - 1; ci >= 0; ci--) {
-        FmmCellHost &C = pl.cells[ci];
-        double s = 0.0;
-        if (C.nchild == 0) {
+    });
+    // sigma max: leaves in parallel, then parents level by level from the
+    // deepest (cells of a level are contiguous in breadth-first order)
+    const int64_t nc = (int64_t)pl.cells.size();
+    host_parallel_for(nc, [&](int64_t i0, int64_t i1) {
+        for (int64_t ci = i0; ci < i1; ci++) {
+            FmmCellHost &C = pl.cells[ci];
+            if (C.nchild) continue;
+            double s = 0.0;
             for (int64_t j = C.begin; j < C.end; j++) s = fmax(s, ssig[j]);
-        } else {
-            for (int k = 0; k < C.nchild; k++) s = fmax(s, pl.cells[C.child0 + k].smax);
+            C.smax = s;
         }
+    });
+    for (int64_t ci = nc - 1; ci >= 0; ci--) {
+        FmmCellHost &C = pl.cells[ci];
+        if (C.nchild == 0) continue;
+        double s = 0.0;
+        for (int k = 0; k < C.nchild; k++) s = fmax(s, pl.cells[C.child0 + k].smax);
         C.smax = s;
     }
 }

@@ -767,16 +767,21 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
         if (C.nchild == 0) pl.leaves.push_back((int32_t)i);
         for (int k = 0; k < C.nchild; k++) pl.parent[C.child0 + k] = (int32_t)i;
     }
-    for (int lev = 0; lev <= maxlev; lev++) {
-        pl.level_parents[lev].first = (int)pl.parents.size();
-        pl.level_kids[lev].first = (int)pl.kids.size();
-        for (int64_t i = 0; i < nc; i++) {
-            if (pl.cells[i].level != lev) continue;
-            if (pl.cells[i].nchild) pl.parents.push_back((int32_t)i);
-            if (lev > 0) pl.kids.push_back((int32_t)i);
+    // cells are in breadth-first order, so one pass groups them by level
+    for (int64_t i = 0, cur = -1; i < nc; i++) {
+        const int lev = pl.cells[i].level;
+        for (; cur < lev; cur++) {
+            pl.level_parents[cur + 1].first = (int)pl.parents.size();
+            pl.level_kids[cur + 1].first = (int)pl.kids.size();
         }
-        pl.level_parents[lev].second = (int)pl.parents.size() - pl.level_parents[lev].first;
-        pl.level_kids[lev].second = (int)pl.kids.size() - pl.level_kids[lev].first;
+        if (pl.cells[i].nchild) {
+            pl.parents.push_back((int32_t)i);
+            pl.level_parents[lev].second++;
+        }
+        if (lev > 0) {
+            pl.kids.push_back((int32_t)i);
+            pl.level_kids[lev].second++;
+        }
     }
     // device: tables (once per p), tree, lists, expansions
     if (S.dev_tab_p != P->p) {
