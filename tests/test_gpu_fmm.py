@@ -75,7 +75,8 @@ def test_fmm_mixed_near_field(geom):
 
 
 @pytest.mark.parametrize("ncrit,theta,p", [(1, 0.4, 3), (7, 0.6, 5), (300, 0.3, 4), (50, 1e-9, 2),
-                                           (50, 0.4, 0), (64, 0.4, 12)])
+                                           (50, 0.4, 0), (64, 0.4, 12), (20, 0.5, 1), (30, 0.4, 7),
+                                           (40, 0.4, 9)])
 def test_fmm_parameters(ncrit, theta, p):
     X, G, s, _ = workloads.gaussian_blob(n=3000, seed=9)
     U, J = run(fmm_field(X, G, s, p=p, ncrit=ncrit, theta=theta, sigma_cut=7.0))
