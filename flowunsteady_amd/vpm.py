@@ -585,3 +585,21 @@ def monitor_enstrophy(pfield: ParticleField) -> float:
     G = get_Gamma(pfield)
     w = np.stack([J[:, 5] - J[:, 7], J[:, 6] - J[:, 2], J[:, 1] - J[:, 3]], axis=1)
     return 0.5 * float(np.sum(G * w))
+
+
+def save(pfield: ParticleField, file_name: str, *, path: str = "", add_num: bool = True,
+         num=None, overwrite_time=None) -> str:
+    """Write the field for visualisation and restart (FLOWVPM's `vpm.save`,
+    called at src/FLOWUnsteady_simulation.jl:438): XDMF + raw binary + a JSON
+    sidecar (pfield_io module); returns the XDMF file name."""
+    from . import pfield_io
+    return pfield_io.save(pfield, file_name, path=path, add_num=add_num, num=num,
+                          overwrite_time=overwrite_time)
+
+
+def read(pfield: ParticleField, file_name: str, *, overwrite: bool = True,
+         load_time: bool = False) -> ParticleField:
+    """Restart from a file written by `save` (FLOWVPM's `vpm.read!`, called at
+    src/FLOWUnsteady_simulation.jl:264 with overwrite=true, load_time=false)."""
+    from . import pfield_io
+    return pfield_io.read(pfield, file_name, overwrite=overwrite, load_time=load_time)
