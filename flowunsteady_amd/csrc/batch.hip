@@ -102,6 +102,7 @@ __global__ __launch_bounds__(BLK) void uj_mixed_items_kernel(const BatchItem *__
                                                              const int32_t *__restrict__ perm,
                                                              const double4 *__restrict__ spos,
                                                              const float4 *__restrict__ sgam,
+                                                             const double4 *__restrict__ tbnd,
                                                              int64_t ldo, double *__restrict__ U,
                                                              double *__restrict__ J, int reset)
 {
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(BLK) void uj_mixed_items_kernel(const BatchItem *__
     __shared__ float4 tgam[BLK];
     const BatchItem it = items[blockIdx.x];
     uj_mixed_block<K, BLK>(spos, it.t0, it.t1, perm, spos, sgam, it.s0, it.s1, ldo, U, J, reset,
-                           tpos, tgam);
+                           tpos, tgam, tbnd + 2 * it.tb, it.s0);
 }
 
 // Persistent grid: each workgroup takes the next item from the queue head until
@@ -121,6 +122,7 @@ __global__ __launch_bounds__(BLK) void uj_mixed_queue_kernel(const BatchItem *__
                                                              const int32_t *__restrict__ perm,
                                                              const double4 *__restrict__ spos,
                                                              const float4 *__restrict__ sgam,
+                                                             const double4 *__restrict__ tbnd,
                                                              int64_t ldo, double *__restrict__ U,
                                                              double *__restrict__ J, int reset)
 {
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(BLK) void uj_mixed_queue_kernel(const BatchItem *__
         if (k >= nitems) break;
         const BatchItem it = items[k];
         uj_mixed_block<K, BLK>(spos, it.t0, it.t1, perm, spos, sgam, it.s0, it.s1, ldo, U, J,
-                               reset, tpos, tgam);
+                               reset, tpos, tgam, tbnd + 2 * it.tb, it.s0);
     }
 }
 
@@ -162,10 +164,10 @@ hipError_t mixed_go(hipStream_t st, const DeviceField &f, const Scratch &s, Batc
         if (e != hipSuccess) return e;
         const int g = std::max(1, std::min(grid, n));
         uj_mixed_queue_kernel<K, BLK><<<g, BLK, 0, st>>>(b.items, n, b.counter, s.perm, s.spos,
-                                                         s.sgam, f.cap, f.U, f.J, reset);
+                                                         s.sgam, b.tbnd, f.cap, f.U, f.J, reset);
     } else {
-        uj_mixed_items_kernel<K, BLK><<<n, BLK, 0, st>>>(b.items, s.perm, s.spos, s.sgam, f.cap,
-                                                         f.U, f.J, reset);
+        uj_mixed_items_kernel<K, BLK><<<n, BLK, 0, st>>>(b.items, s.perm, s.spos, s.sgam, b.tbnd,
+                                                         f.cap, f.U, f.J, reset);
     }
     return hipGetLastError();
 }
@@ -260,6 +262,54 @@ hipError_t launch_batch_sort(hipStream_t st, const DeviceField &f, Scratch &s, c
     size_t bytes = b.sort_tmp_bytes;
     return rocprim::radix_sort_pairs(b.sort_tmp, bytes, b.kin, b.kout, s.vin, s.perm,
                                      (unsigned int)n, 0, 30 + fbits, st);
+}
+
+// Bounds of listed slot ranges (each at most 256 slots) of the sorted
+// concatenation: bbox centre, half diagonal, largest sigma -- the per-field
+// tiles of the batched pass.  One workgroup per tile.
+__global__ __launch_bounds__(256) void tile_bounds_list_kernel(const double4 *__restrict__ spos,
+                                                               const int32_t *__restrict__ perm,
+                                                               const double *__restrict__ sg,
+                                                               const int2 *__restrict__ tiles,
+                                                               double4 *__restrict__ tb)
+{
+    __shared__ double red[7][256];
+    const int t = threadIdx.x;
+    const int2 T = tiles[blockIdx.x];
+    const int s = T.x + t;
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    if (s < T.y) {
+        const double4 p = spos[s];
+        v[0] = v[3] = p.x;
+        v[1] = v[4] = p.y;
+        v[2] = v[5] = p.z;
+        v[6] = sg[perm[s]];
+    }
+    for (int k = 0; k < 7; ++k) red[k][t] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int k = 0; k < 3; ++k) red[k][t] = fmin(red[k][t], red[k][t + w]);
+            for (int k = 3; k < 7; ++k) red[k][t] = fmax(red[k][t], red[k][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double hx = 0.5 * (red[3][0] - red[0][0]), hy = 0.5 * (red[4][0] - red[1][0]),
+                     hz = 0.5 * (red[5][0] - red[2][0]);
+        tb[2 * blockIdx.x] = make_double4(red[0][0] + hx, red[1][0] + hy, red[2][0] + hz,
+                                          sqrt(hx * hx + hy * hy + hz * hz));
+        tb[2 * blockIdx.x + 1] = make_double4(red[6][0], 0.0, 0.0, 0.0);
+    }
+}
+
+hipError_t launch_batch_tile_bounds(hipStream_t st, const DeviceField &f, const Scratch &s,
+                                    const BatchState &b)
+{
+    if (b.ntiles == 0) return hipSuccess;
+    tile_bounds_list_kernel<<<(unsigned)b.ntiles, 256, 0, st>>>(s.spos, s.perm, f.sigma, b.tiles,
+                                                               b.tbnd);
+    return hipGetLastError();
 }
 
 hipError_t launch_batch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f,

@@ -73,7 +73,9 @@ struct Scratch {
 
 // One workgroup's work in the batched UJ: targets [t0, t1) against sources
 // [s0, s1), all in slots of the concatenated fields.
-struct BatchItem { int32_t t0, t1, s0, s1; };
+// Targets [t0, t1) of one field against its sources [s0, s1); the field's
+// BLK-slot tiles (from s0) have bounds from tile `tb` of the batch's list.
+struct BatchItem { int32_t t0, t1, s0, s1, tb, pad; };
 
 // Schedules of the batched UJ.
 enum BatchSchedule { SCHED_FLAT = 0, SCHED_QUEUE = 1 };
@@ -101,6 +103,10 @@ struct BatchState {
     int64_t icap = 0, nitems = 0;
     BatchItem *items = nullptr;
     int32_t *counter = nullptr;      // work-queue head
+    // per-field tiles of items_block slots (slot ranges) and their bounds
+    int64_t tcap = 0, ntiles = 0;
+    int2 *tiles = nullptr;
+    double4 *tbnd = nullptr;
 };
 
 // Instantiated hipGraph of one time step on the resident state and the
@@ -219,6 +225,9 @@ hipError_t launch_batch_sort(hipStream_t st, const DeviceField &f, Scratch &s,
 // fills fid from the offsets
 hipError_t launch_batch_fid(hipStream_t st, const BatchState &b, int64_t n);
 // pairwise pass over the work list with the batch's schedule
+// Bounds of the batch's per-field tiles (after the mixed pack).
+hipError_t launch_batch_tile_bounds(hipStream_t st, const DeviceField &f, const Scratch &s,
+                                    const BatchState &b);
 hipError_t launch_batch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f,
                                  const Scratch &s, BatchState &b, int reset, int sched,
                                  int block, int grid);

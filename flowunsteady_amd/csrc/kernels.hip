@@ -299,7 +299,7 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_split_kernel(const double4 
     const int64_t s0 = (int64_t)g * seglen;
     const int64_t s1 = s0 + seglen < n ? s0 + seglen : n;
     double d[14];
-    uj_mixed_sums<K, UJ_BLOCK>(tgt, t0, nt, spos, sgam, s0, s1, tb, tpos, tgam, d);
+    uj_mixed_sums<K, UJ_BLOCK>(tgt, t0, nt, spos, sgam, s0, s1, tb, 0, tpos, tgam, d);
     const int64_t s = t0 + threadIdx.x;
     if (s < nt)
         for (int k = 0; k < 14; ++k) part[((int64_t)g * 14 + k) * pld + s] = d[k];

@@ -105,18 +105,20 @@ __device__ __forceinline__ void uj_f64_block(const double4 *__restrict__ src, in
 // first target in FP64 and rounded to FP32; each tile's FP32 partials are
 // folded into FP64.  d = (U0..2, W0..2, J0..7) before the -1/(4 pi) factor.
 //
-// With tile bounds `tb` (per BLK-source tile of the sorted field: (centre,
-// radius), (largest sigma, -, -, -); the block's own targets are tile t0/BLK),
-// a tile whose every pair is in the far field -- gap >= 7.001 sigma_max for the
-// gaussian kernels, gap > 0 for the singular one -- runs mixed_pair_far without
-// per-pair tests, the same arithmetic the per-pair path takes for such pairs.
-// Tiles must then start at multiples of BLK (s0 a multiple of BLK).
+// With tile bounds `tb` (per BLK-slot tile counted from slot `torg`: (centre,
+// radius), (largest sigma, -, -, -); the block's own targets are tile
+// (t0 - torg)/BLK), a tile whose every pair is in the far field -- gap >=
+// 7.001 sigma_max for the gaussian kernels, gap > 0 for the singular one --
+// runs mixed_pair_far without per-pair tests, the same arithmetic the per-pair
+// path takes for such pairs.  t0 - torg and s0 - torg must then be multiples
+// of BLK.
 template <int K, int BLK>
 __device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, int64_t t0,
                                               int64_t t1, const double4 *__restrict__ spos,
                                               const float4 *__restrict__ sgam, int64_t s0,
                                               int64_t s1, const double4 *__restrict__ tb,
-                                              float4 *tpos, float4 *tgam, double d[14])
+                                              int64_t torg, float4 *tpos, float4 *tgam,
+                                              double d[14])
 {
     const int t = threadIdx.x;
     const int64_t s = t0 + t;
@@ -130,7 +132,7 @@ __device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, i
         zi = (float)(p.z - o.z);
     }
     double4 T = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (tb) T = tb[2 * (t0 / BLK)];
+    if (tb) T = tb[2 * ((t0 - torg) / BLK)];
     for (int k = 0; k < 14; ++k) d[k] = 0.0;
 
     for (int64_t base = s0; base < s1; base += BLK) {
@@ -147,13 +149,14 @@ __device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, i
         bool far = false;
         if constexpr (K == GAUSSIANERF || K == GAUSSIAN || K == SINGULAR) {
             if (tb && cnt == BLK) {
-                const double4 S = tb[2 * (base / BLK)];
+                const int64_t ts = (base - torg) / BLK;
+                const double4 S = tb[2 * ts];
                 const double ex = S.x - T.x, ey = S.y - T.y, ez = S.z - T.z;
                 const double gap = sqrt(ex * ex + ey * ey + ez * ez) - S.w - T.w;
                 if constexpr (K == SINGULAR)
                     far = gap > 0.0;
                 else
-                    far = gap >= 7.001 * tb[2 * (base / BLK) + 1].x;
+                    far = gap >= 7.001 * tb[2 * ts + 1].x;
             }
         }
         MixedAcc acc;
@@ -208,10 +211,11 @@ __device__ __forceinline__ void uj_mixed_block(const double4 *__restrict__ tgt, 
                                                int64_t s1, int64_t ldo,
                                                double *__restrict__ U, double *__restrict__ J,
                                                int reset, float4 *tpos, float4 *tgam,
-                                               const double4 *__restrict__ tb = nullptr)
+                                               const double4 *__restrict__ tb = nullptr,
+                                               int64_t torg = 0)
 {
     double d[14];
-    uj_mixed_sums<K, BLK>(tgt, t0, t1, spos, sgam, s0, s1, tb, tpos, tgam, d);
+    uj_mixed_sums<K, BLK>(tgt, t0, t1, spos, sgam, s0, s1, tb, torg, tpos, tgam, d);
     const int64_t s = t0 + threadIdx.x;
     if (s < t1) uj_mixed_store(d, perm[s], ldo, U, J, reset);
 }

@@ -264,7 +264,8 @@ int ensure_batch(vpmx_ctx *c, int64_t nf, int64_t np)
 void free_batch(BatchState &B)
 {
     for (void *p : {(void *)B.doff, (void *)B.box, (void *)B.uinf, (void *)B.fid, (void *)B.kin,
-                    (void *)B.kout, (void *)B.sort_tmp, (void *)B.items, (void *)B.counter})
+                    (void *)B.kout, (void *)B.sort_tmp, (void *)B.items, (void *)B.counter,
+                    (void *)B.tiles, (void *)B.tbnd})
         if (p) (void)hipFree(p);
     B = BatchState();
 }
@@ -290,12 +291,21 @@ int ensure_items(vpmx_ctx *c, int block)
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         return B.off[a + 1] - B.off[a] > B.off[b + 1] - B.off[b];
     });
+    // per-field tiles of `block` slots, field by field (their bounds feed the
+    // far-field tile test of the mixed pass)
+    std::vector<int2> tiles;
+    std::vector<int32_t> tbase(nf);
+    for (int k = 0; k < nf; ++k) {
+        tbase[k] = (int32_t)tiles.size();
+        for (int64_t t = B.off[k]; t < B.off[k + 1]; t += block)
+            tiles.push_back(make_int2((int)t, (int)std::min<int64_t>(t + block, B.off[k + 1])));
+    }
     std::vector<BatchItem> items;
     for (int k : order) {
         const int64_t o = B.off[k], e = B.off[k + 1];
         for (int64_t t = o; t < e; t += block)
             items.push_back({(int32_t)t, (int32_t)std::min<int64_t>(t + block, e), (int32_t)o,
-                             (int32_t)e});
+                             (int32_t)e, tbase[k], 0});
     }
     if ((int64_t)items.size() > B.icap) {
         const int64_t cap = std::max<int64_t>((int64_t)items.size(), 2 * B.icap);
@@ -303,8 +313,19 @@ int ensure_items(vpmx_ctx *c, int block)
         B.icap = cap;
         c->gen++;
     }
+    if ((int64_t)tiles.size() > B.tcap) {
+        const int64_t cap = std::max<int64_t>((int64_t)tiles.size(), 2 * B.tcap);
+        VPMX_TRY(dalloc(c, &B.tiles, cap));
+        VPMX_TRY(dalloc(c, &B.tbnd, 2 * cap));
+        B.tcap = cap;
+        c->gen++;
+    }
+    B.ntiles = (int64_t)tiles.size();
     if (!items.empty())
         VPMX_HIP(c, hipMemcpyAsync(B.items, items.data(), items.size() * sizeof(BatchItem),
+                                   hipMemcpyHostToDevice, c->stream));
+    if (!tiles.empty())
+        VPMX_HIP(c, hipMemcpyAsync(B.tiles, tiles.data(), tiles.size() * sizeof(int2),
                                    hipMemcpyHostToDevice, c->stream));
     VPMX_TRY(sync(c));
     B.nitems = (int64_t)items.size();
@@ -918,6 +939,7 @@ int enqueue_uj(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool sort = true
     } else {
         if (sort) VPMX_TRY(enqueue_sort(c));
         VPMX_HIP(c, launch_pack_mixed(c->stream, P->kernel, f, c->s));
+        if (B.nfields) VPMX_HIP(c, launch_batch_tile_bounds(c->stream, f, c->s, B));
         TimedScope ts(c, T_UJ);
         if (B.nfields)
             VPMX_HIP(c, launch_batch_uj_mixed(c->stream, P->kernel, f, c->s, B, reset, B.sched,
@@ -1703,9 +1725,11 @@ int vpmx_batch_autotune(vpmx_ctx *c, const vpmx_uj_params *P, int32_t reps, doub
         return fail(c, VPMX_EHIP, "hipEventCreate failed");
     }
     auto launch = [&]() -> hipError_t {
-        return mixed ? launch_batch_uj_mixed(c->stream, P->kernel, c->f, c->s, B, 1, B.sched,
-                                             B.block, B.grid_eff)
-                     : launch_batch_uj_f64(c->stream, P->kernel, c->f, c->s, B, 1);
+        if (!mixed) return launch_batch_uj_f64(c->stream, P->kernel, c->f, c->s, B, 1);
+        const hipError_t e = launch_batch_tile_bounds(c->stream, c->f, c->s, B);
+        return e != hipSuccess ? e
+                               : launch_batch_uj_mixed(c->stream, P->kernel, c->f, c->s, B, 1,
+                                                       B.sched, B.block, B.grid_eff);
     };
     int rc = VPMX_OK, best = -1;
     double best_ms = INFINITY;
