@@ -213,7 +213,8 @@ hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int tr
 hipError_t fmm_p2p_chunks(hipStream_t st, FmmState &S, int64_t nleaf);
 
 // Dual tree traversal on the device: fills m2l_off/m2l_src and p2p_off/p2p_src
-// (CSR by target cell, sources ascending), growing the buffers in S.
+// (CSR by target cell, each target's sources in emission order), growing the
+// buffers in S.
 hipError_t fmm_traverse_device(hipStream_t st, FmmState &S, int64_t ncell, double theta,
                                double sigcut, DttResult &res);
 

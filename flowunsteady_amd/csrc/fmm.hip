@@ -1702,8 +1702,10 @@ hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int tr
 // the same rules as fmm_traverse (M2L, P2P, or a split into child pairs);
 // exclusive scans of the per-pair output counts give each pair fixed output
 // slots, so the lists are deterministic without atomics.  The M2L and P2P
-// lists are then radix-sorted by (target, source), which is exactly the
-// host traversal's order.
+// lists are then radix-sorted (stable) on the target bits of their
+// (target, source) keys only: grouped by target for the CSR, each target's
+// sources in emission order (the host traversal orders them by source; the
+// lists hold the same pairs).
 
 // Multipole acceptance with IEEE-rounded operations in the host's order (no
 // FMA contraction), so device and host decisions agree bit for bit.
@@ -1940,7 +1942,7 @@ hipError_t fmm_traverse_device(hipStream_t st, FmmState &S, int64_t ncell, doubl
     const int64_t kmax = std::max(nm2l, np2p);
     VPMX_CHK(grow(&B.keys_tmp, B.cap_tmpk, std::max<int64_t>(kmax, 1), st));
     size_t need = 0;
-    VPMX_CHK(rocprim::radix_sort_keys(nullptr, need, B.m2l_keys, B.keys_tmp, (unsigned int)kmax, 0,
+    VPMX_CHK(rocprim::radix_sort_keys(nullptr, need, B.m2l_keys, B.keys_tmp, (unsigned int)kmax, nb,
                                       2 * nb, st));
     if (need > B.tmp_bytes) {
         if (B.tmp) (void)hipFree(B.tmp);
@@ -1957,7 +1959,7 @@ hipError_t fmm_traverse_device(hipStream_t st, FmmState &S, int64_t ncell, doubl
     for (auto &li : lists) {
         if (li.n > 0) {
             size_t tb = B.tmp_bytes;
-            VPMX_CHK(rocprim::radix_sort_keys(B.tmp, tb, li.keys, B.keys_tmp, (unsigned int)li.n, 0,
+            VPMX_CHK(rocprim::radix_sort_keys(B.tmp, tb, li.keys, B.keys_tmp, (unsigned int)li.n, nb,
                                               2 * nb, st));
         }
         const int64_t work = std::max(li.n, ncell + 1);
