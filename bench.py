@@ -437,8 +437,8 @@ def main():
                     help="only run one pair-kernel launch and a reference copy (for PMC runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
-                                   "described in DESIGN.md but not built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "vpmx runs on one GPU only: the particle field is not "
+                                   "sharded across GPUs", "n_gpus": args.gpus}))
         sys.exit(2)
 
     if args.traffic_probe:
