@@ -31,6 +31,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# CPU-baseline protocol: the oracle's OpenMP threads are pinned one per core,
+# in order (set before the oracle library, and with it libgomp, is loaded).
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 METRIC = "particle interactions/s (UJ direct+FMM, N=1M) mixed vs FP64 at stated rel. err"
 
 
@@ -214,23 +219,25 @@ def batched_section(ctx, log, nfields=64, seed=5, reps=2, steps=2):
             "ring8k_fp64_rk3_relax_step_ms": small}
 
 
-def fmm_section(ctx, n, orders, log, cpu=True, nref=1024):
+def fmm_section(ctx, n, orders, log, cpu=True, nref=65536):
     """UJ by the FMM on the rotor-wake workload (3 helical tip filaments +
     root line, Gamma = 0.05 t + N(0, 5e-4), sigma = 0.03), ncrit = 50,
-    theta = 0.4, mixed near field, FP64 expansions; errors against the mixed
-    direct sum at `nref` sampled targets."""
+    theta = 0.4, mixed near field, FP64 expansions; errors against the FP64
+    direct sum at `nref` sampled targets (the FP64 direct path equals the
+    oracle to 1e-12)."""
     from flowunsteady_amd import _native as N
     from flowunsteady_amd import workloads
     X, G, s, v = workloads.rotor_wake(n=n, seed=3)
     fld = Field(ctx, X, G, s, v)
     fld.upload()
-    idx = np.sort(np.random.default_rng(7).choice(n, nref, replace=False))
+    idx = np.sort(np.random.default_rng(7).choice(n, min(nref, n), replace=False))
     t0 = time.perf_counter()
-    Uref, Jref = uj_at(ctx, X[idx], N.GAUSSIANERF, N.MIXED)
-    log(f"fmm reference: {nref} targets direct mixed in {time.perf_counter() - t0:.2f} s")
+    Uref, Jref = uj_at(ctx, X[idx], N.GAUSSIANERF, N.FP64)
+    log(f"fmm reference: {len(idx)} targets direct FP64 in {time.perf_counter() - t0:.2f} s")
     res = {"workload": f"rotor wake N={n} (3 helices radius 1 pitch 0.3 40 turns + root line), "
                        "gaussian-erf, ncrit=50, theta=0.4, mixed P2P, FP64 expansions",
-           "N": n, "error_reference": f"direct mixed UJ at {nref} sampled targets", "orders": {}}
+           "N": n, "error_reference": f"direct FP64 UJ at {len(idx)} sampled targets",
+           "orders": {}}
     P = N.UJParams()
     P.kernel, P.precision, P.transposed, P.reset = N.GAUSSIANERF, N.MIXED, 1, 1
     P.fmm, P.ncrit, P.theta, P.sigma_cut = 1, 50, 0.4, -1.0   # core-aware criterion
@@ -262,11 +269,14 @@ def fmm_section(ctx, n, orders, log, cpu=True, nref=1024):
     if cpu:
         from oracle import oracle as O
         p = 8 if 8 in orders else orders[-1]
-        sample = np.random.default_rng(11).choice(n, 256, replace=False)
+        sample = np.sort(np.random.default_rng(11).choice(n, 256, replace=False))
         t0 = time.perf_counter()
-        _, _, st = O.uj_fmm("gaussianerf", X, G, s, p=p, ncrit=50, theta=0.4, sigcut=7.0,
-                            sample=sample)
+        Uc, Jc, st = O.uj_fmm("gaussianerf", X, G, s, p=p, ncrit=50, theta=0.4, sigcut=7.0,
+                              sample=sample)
         wall = time.perf_counter() - t0
+        # the CPU FMM's own error at its sampled targets, against FP64 direct
+        Us, Js = uj_at(ctx, X[sample], N.GAUSSIANERF, N.FP64)
+        cu, cj = normwise(Uc[sample], Us), normwise(Jc[sample], Js)
         setup = st["t_tree"] + st["t_traverse"] + st["t_p2m"] + st["t_m2m"]
         full = (setup + st["t_m2l"] * st["nm2l"] / max(st["nm2l_active"], 1)
                 + st["t_l2l"] * st["ncell"] / max(st["ncell_active"], 1)
@@ -281,14 +291,39 @@ def fmm_section(ctx, n, orders, log, cpu=True, nref=1024):
                       f"leaves holding 256 sampled targets (and their ancestors) and each "
                       f"scaled by its work count (M2L pairs, cells, leaves, P2P pairs) to the "
                       f"full {st['nleaf']}-leaf tree; {wall:.1f} s of CPU work",
+            "rel_err_U": {"max": cu[0], "rms": cu[1]}, "rel_err_J": {"max": cj[0], "rms": cj[1]},
+            "error_reference": "direct FP64 UJ at the 256 sampled targets",
+            "host": cpu_info(),
             "gpu_speedup": full / gpu}
         log("cpu fmm baseline:", res["cpu_baseline"])
     return res
 
 
-def cpu_baseline(X, G, s, budget_s=12.0):
+def cpu_info():
+    """The host side of the CPU baseline: CPU model, logical CPUs, the CPUs
+    this process may run on, and the OpenMP settings the oracle runs with."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "OMP_PLACES": os.environ.get("OMP_PLACES")}
+
+
+def cpu_baseline(X, G, s, budget_s=15.0, runs=5):
     """Oracle UJ_direct (OpenMP, FP64) on a bounded target sample against all N
-    sources; interactions/s = sample * N / t."""
+    sources, timed `runs` times; interactions/s = sample * N / median time."""
     from oracle import oracle as O
     n = len(X)
     threads = O.num_threads()
@@ -296,15 +331,20 @@ def cpu_baseline(X, G, s, budget_s=12.0):
     t0 = time.perf_counter()
     O.uj_direct("gaussianerf", X[:m], X, G, s)
     t = time.perf_counter() - t0
-    m2 = int(min(n, max(m, m * budget_s / max(t, 1e-3))))
+    m2 = int(min(n, max(m, m * budget_s / runs / max(t, 1e-3))))
     rng = np.random.default_rng(0)
     idx = np.sort(rng.choice(n, m2, replace=False))
-    t0 = time.perf_counter()
-    O.uj_direct("gaussianerf", X[idx], X, G, s)
-    t = time.perf_counter() - t0
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        O.uj_direct("gaussianerf", X[idx], X, G, s)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
     return {"value": m2 * n / t, "unit": "interactions/s", "cores": threads, "kind": "port",
+            "runs": runs, "seconds": {"median": t, "min": min(ts), "max": max(ts)},
+            "host": cpu_info(),
             "sample": f"{m2} targets x {n} sources, gaussian-erf U+J, FP64 OpenMP oracle "
-                      f"(C restatement of UJ_direct), {t:.1f} s"}
+                      f"(C restatement of UJ_direct), median of {runs} runs of {t:.2f} s"}
 
 
 def full_step_run(n, steps, log, precisions=("mixed", "fp64"), every=10):

@@ -23,7 +23,7 @@ RELAX_NONE, RELAX_PEDRIZZETTI, RELAX_CORRECTED = 0, 1, 2
 # SFS
 SFS_NONE, SFS_CONSTANT, SFS_DYNAMIC = 0, 1, 2
 # timing classes
-T_UJ, T_ESTR, T_UPDATE, T_FMM, T_GRAPH = 0, 1, 2, 3, 4
+T_UJ, T_ESTR, T_UPDATE, T_FMM, T_GRAPH, T_DSIG = 0, 1, 2, 3, 4, 5
 # schedules of the batched pairwise pass
 SCHED_FLAT, SCHED_QUEUE = 0, 1
 # viscous schemes

@@ -73,13 +73,72 @@ __device__ __forceinline__ double zeta_f64(double rho)
 }
 
 // ---------------------------------------------------------------------------
+// Core-size derivatives for the dynamic SFS procedure (docs/src/theory/rvpm.md:276-290),
+// every core scaled together (D = sigma d/dsigma):
+//   h  = D g(r/sigma)  = -rho g'(rho) = -4 pi rho^3 zeta(rho)
+//   h' = dh/drho       = -4 pi (3 rho^2 zeta + rho^3 zeta')
+//   zD = sigma^3 D [zeta(r/sigma)/sigma^3] = -(3 zeta + rho zeta')
+// U and J carry g only through g(r/sigma)/r^3, so D[U] and D[J] are the pair
+// algebra of U and J with (g, g') replaced by (h, h'); in radial-coefficient
+// form A_D = h/r^3 = -4 pi zeta/sigma^3 and B_D = -4 pi zeta'/(rho sigma^5),
+// both finite at r = 0 and negligible beyond the core (no far field).
+template <int K>
+__device__ __forceinline__ void h_dh_f64(double rho, double &h, double &dh)
+{
+    if constexpr (K == GAUSSIANERF) {
+        const double e = exp(-0.5 * rho * rho);
+        const double r2 = rho * rho;
+        h = -SQRT2_PI * r2 * rho * e;
+        dh = -SQRT2_PI * r2 * (3.0 - r2) * e;
+    } else if constexpr (K == GAUSSIAN) {
+        const double r3 = rho * rho * rho;
+        const double e = exp(-r3);
+        h = -3.0 * r3 * e;
+        dh = -9.0 * rho * rho * (1.0 - r3) * e;
+    } else if constexpr (K == WINCKELMANS) {
+        const double r2 = rho * rho, q = r2 + 1.0;
+        const double q72 = q * q * q * sqrt(q);
+        h = -7.5 * r2 * rho / q72;
+        dh = -7.5 * (3.0 * r2 - 7.0 * r2 * r2 / q) / q72;
+    } else {
+        h = 0.0;
+        dh = 0.0;
+    }
+}
+
+// zeta(rho) and zD(rho) together (one exp).
+template <int K>
+__device__ __forceinline__ void zeta_zD_f64(double rho, double &z, double &zD)
+{
+    if constexpr (K == GAUSSIANERF) {
+        z = 0.06349363593424097 * exp(-0.5 * rho * rho);
+        zD = (rho * rho - 3.0) * z;
+    } else if constexpr (K == GAUSSIAN) {
+        const double r3 = rho * rho * rho;
+        z = 0.238732414637843 * exp(-r3);
+        zD = 3.0 * (r3 - 1.0) * z;
+    } else if constexpr (K == WINCKELMANS) {
+        const double q = rho * rho + 1.0;
+        z = 0.5968310365946075 / (q * q * q * sqrt(q));
+        zD = (4.0 - 7.0 / q) * z;
+    } else {
+        z = 0.0;
+        zD = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // FP32 forms used by the mixed-precision kernels.
 //
-// gaussianerf, rho >= 1.5: erfc(x) = t P(t) e^{-x^2}, t = 1/(1 + p x), the
-//   five-term rational approximation 7.1.26 of Abramowitz & Stegun (1964),
-//   |error| <= 1.5e-7.  With x = rho/sqrt2 its e^{-x^2} is the same
-//   e^{-rho^2/2} that g' needs, so one hardware exp serves both:
-//   g = 1 - e (t P(t) + sqrt(2/pi) rho).
+// gaussianerf, 1.5 <= rho < 7: erfc(x) = t P(t) e^{-x^2}, t = 1/(1 + p x),
+//   x = rho/sqrt2, P of degree 4 -- the form of Hastings' approximation
+//   (Abramowitz & Stegun 7.1.26), with p and the five coefficients refitted
+//   here as a weighted minimax approximation of erfc(x) e^{x^2} on exactly
+//   this range, weighted by e^{-x^2} / |rho g' - 3 g| (the factor by which an
+//   error in erfc reaches B).  Its e^{-x^2} is the same e^{-rho^2/2} that g'
+//   needs, so one hardware exp serves both: g = 1 - e (t P(t) + sqrt(2/pi) rho).
+//   Evaluated in FP32 it puts at most 2.0e-7 relative error into B and 8e-8
+//   into A (A&S 7.1.26 on the same range: 5.5e-7 and 2.1e-7).
 // gaussianerf, rho < 1.5: Taylor series of g/rho^3 and of
 //   sigma^5 B = (g'/rho^2 - 3 g/rho^3)/rho^2 obtained by integrating the series
 //   of g' = sqrt(2/pi) rho^2 e^{-rho^2/2} term by term:
@@ -192,10 +251,11 @@ VPMX_HD float series_gauss_hB(float u, float rho)
     return h * rho;
 }
 
-// Abramowitz & Stegun 7.1.26 constants (Hastings' approximation).
-constexpr float AS_P = 0.3275911f;
-constexpr float AS_A1 = 0.254829592f, AS_A2 = -0.284496736f, AS_A3 = 1.421413741f,
-                AS_A4 = -1.453152027f, AS_A5 = 1.061405429f;
+// Constants of the erfc fit above (p, then the coefficients of t^1 .. t^5).
+constexpr float ERFC_P = 0.5199999809265137f;
+constexpr float ERFC_C1 = 0.2891683578491211f, ERFC_C2 = 0.33505475521087646f,
+                ERFC_C3 = 0.08860228955745697f, ERFC_C4 = 0.495405912399292f,
+                ERFC_C5 = -0.20866064727306366f;
 constexpr float LOG2E = 1.4426950408889634f;
 
 // Radial coefficients (A, B) of one pair in FP32, full evaluation (near field).
@@ -216,8 +276,8 @@ VPMX_HD void radial_near_f32(float r2, float rinv, float p0, float p1,
             float rinv2 = rinv * rinv, rinv3 = rinv2 * rinv;
             float rho = r2 * rinv * p0;
             float e = fexp2(-0.5f * LOG2E * rho2);
-            float t = frcp(fmaf(AS_P * (float)INV_SQRT2, rho, 1.0f));
-            float P = fmaf(fmaf(fmaf(fmaf(AS_A5, t, AS_A4), t, AS_A3), t, AS_A2), t, AS_A1) * t;
+            float t = frcp(fmaf(ERFC_P * (float)INV_SQRT2, rho, 1.0f));
+            float P = fmaf(fmaf(fmaf(fmaf(ERFC_C5, t, ERFC_C4), t, ERFC_C3), t, ERFC_C2), t, ERFC_C1) * t;
             float g = fmaf(-e, fmaf((float)SQRT2_PI, rho, P), 1.0f);
             float dg = (float)SQRT2_PI * rho2 * e;
             A = g * rinv3;
@@ -249,6 +309,38 @@ VPMX_HD void radial_near_f32(float r2, float rinv, float p0, float p1,
         float rinv2 = rinv * rinv;
         A = rinv2 * rinv;
         B = -3.0f * A * rinv2;
+    }
+}
+
+// Radial coefficients (A_D, B_D) of the core-size derivative of one pair in
+// FP32 (see h_dh_f64); p0, p1 as for radial_near_f32.
+//   gaussianerf  A_D = -sqrt(2/pi) e^{-rho^2/2} / sigma^3,  B_D = sqrt(2/pi) e^{-rho^2/2} / sigma^5
+//   gaussian     A_D = -3 e^{-rho^3} / sigma^3,             B_D = 9 rho e^{-rho^3} / sigma^5
+//   winckelmans  A_D = -15/2 sigma^4 s^-7,                 B_D = 105/2 sigma^4 s^-9,  s^2 = r^2 + sigma^2
+template <int K>
+VPMX_HD void radial_dsig_f32(float r2, float p0, float p1, float &A, float &B)
+{
+    if constexpr (K == GAUSSIANERF) {
+        const float is3 = p0 * p1;
+        const float e = (float)SQRT2_PI * fexp2(-0.5f * LOG2E * r2 * p1);
+        A = -e * is3;
+        B = e * is3 * p1;
+    } else if constexpr (K == GAUSSIAN) {
+        const float is3 = p0 * p1;
+        const float rho = sqrtf(r2 * p1);
+        const float e = fexp2(-LOG2E * rho * rho * rho);
+        A = -3.0f * e * is3;
+        B = 9.0f * rho * e * is3 * p1;
+    } else if constexpr (K == WINCKELMANS) {
+        const float si = frsq(r2 + p0);
+        const float si2 = si * si;
+        const float si7 = si2 * si2 * si2 * si;
+        const float s4 = p0 * p0;
+        A = -7.5f * s4 * si7;
+        B = 52.5f * s4 * si7 * si2;
+    } else {
+        A = 0.f;
+        B = 0.f;
     }
 }
 
@@ -301,6 +393,22 @@ __device__ __forceinline__ void mixed_pair_d(float dx, float dy, float dz, float
     } else {
         radial_near_f32<K>(r2, rinv, a.w, b.w, A, B);
     }
+    A = self ? 0.f : A;
+    B = self ? 0.f : B;
+    mixed_accumulate(dx, dy, dz, A, B, b, acc);
+}
+
+// The core-size derivative of the same pair (D[U], D[J] in the U, J slots of
+// the accumulator).  Coincident pairs contribute nothing.
+template <int K>
+__device__ __forceinline__ void mixed_pair_dsig(const float4 a, const float4 b, float xi,
+                                                float yi, float zi, MixedAcc &acc)
+{
+    const float dx = xi - a.x, dy = yi - a.y, dz = zi - a.z;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const bool self = !(r2 > 0.f);
+    float A, B;
+    radial_dsig_f32<K>(r2, a.w, b.w, A, B);
     A = self ? 0.f : A;
     B = self ? 0.f : B;
     mixed_accumulate(dx, dy, dz, A, B, b, acc);

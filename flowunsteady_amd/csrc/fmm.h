@@ -204,9 +204,15 @@ struct FmmState {
 };
 
 // Estr over the near-field lists of the last FMM evaluation (J must be current);
-// precision 1 runs the FP32 pair math with FP64 accumulation.
+// precision 1 runs the FP32 pair math with FP64 accumulation.  With DJ
+// (9 x cap, the core-size derivative of J) also D[E] into DE.
 hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int transposed,
-                           int precision, const DeviceField &f, Scratch &s);
+                           int precision, const DeviceField &f, Scratch &s,
+                           const double *DJ = nullptr, double *DE = nullptr);
+// Core-size derivatives D[U] (3 x cap) and D[J] (9 x cap) over the near-field
+// lists of the last FMM evaluation (overwritten; the far field has none).
+hipError_t launch_fmm_dsig(hipStream_t st, const FmmState &S, int kernel, int precision,
+                           const DeviceField &f, const Scratch &s, double *DU, double *DJ);
 
 // Near-field work items of the current lists (S.chunks, S.nchunk, S.nslot;
 // S.part sized for 14 values per slot).  nleaf = S.plan.leaves.size().

@@ -736,8 +736,9 @@ __device__ __forceinline__ void p2p_mixed_finish(const double d[14], int64_t i, 
 
 // P2P near field, FP64: one block (one wave) per work item (a target leaf and
 // a run of its list), one target per lane, source leaves staged through LDS.
-// Adds to U, J (or writes partial sums).
-template <int KK>
+// Adds to U, J (or writes partial sums).  DS: the core-size derivatives D[U],
+// D[J] instead (g, g' -> h, h'), added to U, J as given.
+template <int KK, bool DS = false>
 __global__ __launch_bounds__(P2P_BLOCK) void p2p_f64_kernel(const P2PChunk *__restrict__ chunks,
                                                             const FmmCellDev *__restrict__ cells,
                                                             const int32_t *__restrict__ src,
@@ -780,7 +781,10 @@ __global__ __launch_bounds__(P2P_BLOCK) void p2p_f64_kernel(const P2PChunk *__re
                     if (r2 > 0.0) {
                         const double r = sqrt(r2);
                         double g, dg;
-                        g_dg_f64<KK>(r * b.w, g, dg);
+                        if constexpr (DS)
+                            h_dh_f64<KK>(r * b.w, g, dg);
+                        else
+                            g_dg_f64<KK>(r * b.w, g, dg);
                         const double rinv = 1.0 / r;
                         const double r3i = rinv * rinv * rinv;
                         const double G1 = a.w, G2 = b.x, G3 = b.y;
@@ -856,7 +860,7 @@ __device__ __forceinline__ int stream_particle(const int32_t *lbeg, const int32_
     return lbeg[lo] + (q - lpre[lo]);
 }
 
-template <int KK, int R>
+template <int KK, int R, bool DS>
 __device__ __forceinline__ void p2p_mixed_tile(const float4 *tpos, const float4 *tgam, int cnt,
                                                int s, float xi, float yi, float zi, MixedAcc &acc)
 {
@@ -864,14 +868,23 @@ __device__ __forceinline__ void p2p_mixed_tile(const float4 *tpos, const float4 
 #pragma unroll 4
         for (int i = 0; i < P2P_BLOCK / R; ++i) {
             const int j = i * R + s;
-            mixed_pair<KK>(tpos[j], tgam[j], xi, yi, zi, acc);
+            if constexpr (DS)
+                mixed_pair_dsig<KK>(tpos[j], tgam[j], xi, yi, zi, acc);
+            else
+                mixed_pair<KK>(tpos[j], tgam[j], xi, yi, zi, acc);
         }
     } else {
-        for (int j = s; j < cnt; j += R) mixed_pair<KK>(tpos[j], tgam[j], xi, yi, zi, acc);
+        for (int j = s; j < cnt; j += R) {
+            if constexpr (DS)
+                mixed_pair_dsig<KK>(tpos[j], tgam[j], xi, yi, zi, acc);
+            else
+                mixed_pair<KK>(tpos[j], tgam[j], xi, yi, zi, acc);
+        }
     }
 }
 
-template <int KK>
+// DS: the core-size derivatives D[U], D[J] in the same sums (mixed_pair_dsig).
+template <int KK, bool DS = false>
 __global__ __launch_bounds__(P2P_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void p2p_mixed_kernel(const P2PChunk *__restrict__ chunks,
                                                               const FmmCellDev *__restrict__ cells,
@@ -941,10 +954,10 @@ void p2p_mixed_kernel(const P2PChunk *__restrict__ chunks,
                 const int cnt = min(P2P_BLOCK, total - base);
                 MixedAcc acc;
                 switch (R) {
-                case 8: p2p_mixed_tile<KK, 8>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
-                case 4: p2p_mixed_tile<KK, 4>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
-                case 2: p2p_mixed_tile<KK, 2>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
-                default: p2p_mixed_tile<KK, 1>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
+                case 8: p2p_mixed_tile<KK, 8, DS>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
+                case 4: p2p_mixed_tile<KK, 4, DS>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
+                case 2: p2p_mixed_tile<KK, 2, DS>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
+                default: p2p_mixed_tile<KK, 1, DS>(tpos, tgam, cnt, s, xi, yi, zi, acc); break;
                 }
                 dU0 += acc.U0; dU1 += acc.U1; dU2 += acc.U2;
                 dW0 += acc.W0; dW1 += acc.W1; dW2 += acc.W2;
@@ -1341,99 +1354,149 @@ void fmm_build_tables(FmmTables &T, int P)
 // below e^{-24.5} zeta(0) for the gaussian kernels, so the sum over the P2P
 // lists is the whole sum to that precision.
 
-// Estr store: the sums overwrite E.
-__device__ __forceinline__ void estr_finish(const double d[3], int64_t i, int64_t cap,
-                                            double *__restrict__ E)
+// Estr store: the sums overwrite E (and with six sums, D[E] into DE).
+template <int NV>
+__device__ __forceinline__ void estr_finish(const double *d, int64_t i, int64_t cap,
+                                            double *__restrict__ E, double *__restrict__ DE)
 {
     E[i] = d[0];
     E[cap + i] = d[1];
     E[2 * cap + i] = d[2];
+    if constexpr (NV == 6) {
+        DE[i] = d[3];
+        DE[cap + i] = d[4];
+        DE[2 * cap + i] = d[5];
+    }
 }
 
-// Sorted Estr source records (x, y, z, 1/sigma) (Gamma, 0) (J0..3) (J4..7) (J8, 0, 0, 0).
+// Sorted Estr source records (x, y, z, 1/sigma) (Gamma, 0) (J0..3) (J4..7)
+// (J8, 0, 0, 0); with DYN the last is (J8, DJ0..2) followed by (DJ3..6)
+// (DJ7, DJ8, 0, 0): 7 per particle.
+template <bool DYN>
 __global__ __launch_bounds__(FMM_BLOCK) void fmm_estr_gather_kernel(
     const double *__restrict__ X, const double *__restrict__ G, const double *__restrict__ sg,
-    const double *__restrict__ Jm, const int32_t *__restrict__ perm, int64_t n, int64_t cap,
-    double4 *__restrict__ out)
+    const double *__restrict__ Jm, const double *__restrict__ DJ,
+    const int32_t *__restrict__ perm, int64_t n, int64_t cap, double4 *__restrict__ out)
 {
+    constexpr int NR = DYN ? 7 : 5;
     const int64_t s = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
     if (s >= n) return;
     const int64_t i = perm[s];
-    out[5 * s] = make_double4(X[i], X[cap + i], X[2 * cap + i], 1.0 / sg[i]);
-    out[5 * s + 1] = make_double4(G[i], G[cap + i], G[2 * cap + i], 0.0);
-    out[5 * s + 2] = make_double4(Jm[i], Jm[cap + i], Jm[2 * cap + i], Jm[3 * cap + i]);
-    out[5 * s + 3] = make_double4(Jm[4 * cap + i], Jm[5 * cap + i], Jm[6 * cap + i], Jm[7 * cap + i]);
-    out[5 * s + 4] = make_double4(Jm[8 * cap + i], 0.0, 0.0, 0.0);
+    double4 *o = out + NR * s;
+    o[0] = make_double4(X[i], X[cap + i], X[2 * cap + i], 1.0 / sg[i]);
+    o[1] = make_double4(G[i], G[cap + i], G[2 * cap + i], 0.0);
+    o[2] = make_double4(Jm[i], Jm[cap + i], Jm[2 * cap + i], Jm[3 * cap + i]);
+    o[3] = make_double4(Jm[4 * cap + i], Jm[5 * cap + i], Jm[6 * cap + i], Jm[7 * cap + i]);
+    if constexpr (DYN) {
+        o[4] = make_double4(Jm[8 * cap + i], DJ[i], DJ[cap + i], DJ[2 * cap + i]);
+        o[5] = make_double4(DJ[3 * cap + i], DJ[4 * cap + i], DJ[5 * cap + i], DJ[6 * cap + i]);
+        o[6] = make_double4(DJ[7 * cap + i], DJ[8 * cap + i], 0.0, 0.0);
+    } else {
+        o[4] = make_double4(Jm[8 * cap + i], 0.0, 0.0, 0.0);
+    }
 }
 
-template <int KK, bool TRANSPOSED>
+// The nine J (and with DYN DJ) entries of a record.
+template <bool DYN>
+__device__ __forceinline__ void estr_rec_j(const double4 *r, double *Jv, double *DJv)
+{
+    const double4 q2 = r[2], q3 = r[3], q4 = r[4];
+    Jv[0] = q2.x; Jv[1] = q2.y; Jv[2] = q2.z; Jv[3] = q2.w;
+    Jv[4] = q3.x; Jv[5] = q3.y; Jv[6] = q3.z; Jv[7] = q3.w; Jv[8] = q4.x;
+    if constexpr (DYN) {
+        const double4 q5 = r[5], q6 = r[6];
+        DJv[0] = q4.y; DJv[1] = q4.z; DJv[2] = q4.w;
+        DJv[3] = q5.x; DJv[4] = q5.y; DJv[5] = q5.z; DJv[6] = q5.w;
+        DJv[7] = q6.x; DJv[8] = q6.y;
+    }
+}
+
+template <bool TRANSPOSED, typename T>
+__device__ __forceinline__ void stretch_of(const T *dJ, T gx, T gy, T gz, T &S0, T &S1, T &S2)
+{
+    if constexpr (TRANSPOSED) {   // S_k = sum_l dJ[l,k] G_l
+        S0 = dJ[0] * gx + dJ[1] * gy + dJ[2] * gz;
+        S1 = dJ[3] * gx + dJ[4] * gy + dJ[5] * gz;
+        S2 = dJ[6] * gx + dJ[7] * gy + dJ[8] * gz;
+    } else {                      // S_k = sum_l dJ[k,l] G_l
+        S0 = dJ[0] * gx + dJ[3] * gy + dJ[6] * gz;
+        S1 = dJ[1] * gx + dJ[4] * gy + dJ[7] * gz;
+        S2 = dJ[2] * gx + dJ[5] * gy + dJ[8] * gz;
+    }
+}
+
+// Estr over the near-field lists, FP64; DYN adds D[E] (zeta_zD_f64).
+template <int KK, bool TRANSPOSED, bool DYN>
 __global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_kernel(const P2PChunk *__restrict__ chunks,
                                                              const FmmCellDev *__restrict__ cells,
                                                              const int32_t *__restrict__ src,
                                                              const double4 *__restrict__ rec,
                                                              const int32_t *__restrict__ perm,
                                                              int64_t cap, double *__restrict__ E,
+                                                             double *__restrict__ DE,
                                                              double *__restrict__ part,
                                                              int64_t pstride)
 {
-    __shared__ double4 tile[5 * P2P_BLOCK];
+    constexpr int NR = DYN ? 7 : 5;
+    constexpr int NV = DYN ? 6 : 3;
+    __shared__ double4 tile[NR * P2P_BLOCK];
     const P2PChunk ch = chunks[blockIdx.x];
     const int c = ch.c;
     const FmmCellDev A = cells[c];
     for (int tb = A.begin; tb < A.end; tb += P2P_BLOCK) {
         const int it = tb + threadIdx.x;
         const bool active = it < A.end;
-        double xi = 0, yi = 0, zi = 0, Ji[9];
-        for (int k = 0; k < 9; ++k) Ji[k] = 0.0;
+        double xi = 0, yi = 0, zi = 0, Ji[9], DJi[9];
+        for (int k = 0; k < 9; ++k) Ji[k] = DJi[k] = 0.0;
         if (active) {
-            const double4 p = rec[5 * it], q2 = rec[5 * it + 2], q3 = rec[5 * it + 3],
-                          q4 = rec[5 * it + 4];
+            const double4 p = rec[NR * it];
             xi = p.x; yi = p.y; zi = p.z;
-            Ji[0] = q2.x; Ji[1] = q2.y; Ji[2] = q2.z; Ji[3] = q2.w;
-            Ji[4] = q3.x; Ji[5] = q3.y; Ji[6] = q3.z; Ji[7] = q3.w; Ji[8] = q4.x;
+            estr_rec_j<DYN>(rec + NR * it, Ji, DJi);
         }
-        double e0 = 0, e1 = 0, e2 = 0;
+        double e0 = 0, e1 = 0, e2 = 0, f0 = 0, f1 = 0, f2 = 0;
         for (int64_t e = ch.e0; e < ch.e1; e++) {
             const FmmCellDev B = cells[src[e]];
             for (int sb = B.begin; sb < B.end; sb += P2P_BLOCK) {
                 __syncthreads();
                 const int js = sb + threadIdx.x;
                 if (js < B.end)
-                    for (int k = 0; k < 5; ++k) tile[5 * threadIdx.x + k] = rec[5 * js + k];
+                    for (int k = 0; k < NR; ++k) tile[NR * threadIdx.x + k] = rec[NR * js + k];
                 __syncthreads();
                 const int cnt = min(P2P_BLOCK, B.end - sb);
                 for (int jj = 0; jj < cnt; ++jj) {
-                    const double4 a = tile[5 * jj], gm = tile[5 * jj + 1];
-                    const double4 q2 = tile[5 * jj + 2], q3 = tile[5 * jj + 3], q4 = tile[5 * jj + 4];
+                    const double4 a = tile[NR * jj], gm = tile[NR * jj + 1];
+                    double Jj[9], DJj[9];
+                    estr_rec_j<DYN>(tile + NR * jj, Jj, DJj);
                     const double d1 = xi - a.x, d2 = yi - a.y, d3 = zi - a.z;
                     const double r = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
-                    const double zs = zeta_f64<KK>(r * a.w) * (a.w * a.w * a.w);
-                    double dJ[9];
-                    dJ[0] = Ji[0] - q2.x; dJ[1] = Ji[1] - q2.y; dJ[2] = Ji[2] - q2.z;
-                    dJ[3] = Ji[3] - q2.w; dJ[4] = Ji[4] - q3.x; dJ[5] = Ji[5] - q3.y;
-                    dJ[6] = Ji[6] - q3.z; dJ[7] = Ji[7] - q3.w; dJ[8] = Ji[8] - q4.x;
-                    double S0, S1, S2;
-                    if constexpr (TRANSPOSED) {
-                        S0 = dJ[0] * gm.x + dJ[1] * gm.y + dJ[2] * gm.z;
-                        S1 = dJ[3] * gm.x + dJ[4] * gm.y + dJ[5] * gm.z;
-                        S2 = dJ[6] * gm.x + dJ[7] * gm.y + dJ[8] * gm.z;
+                    const double is3 = a.w * a.w * a.w;
+                    double dJ[9], S0, S1, S2;
+                    for (int k = 0; k < 9; ++k) dJ[k] = Ji[k] - Jj[k];
+                    stretch_of<TRANSPOSED>(dJ, gm.x, gm.y, gm.z, S0, S1, S2);
+                    if constexpr (DYN) {
+                        double z, zD;
+                        zeta_zD_f64<KK>(r * a.w, z, zD);
+                        const double zs = z * is3, zDs = zD * is3;
+                        double T0, T1, T2;
+                        for (int k = 0; k < 9; ++k) dJ[k] = DJi[k] - DJj[k];
+                        stretch_of<TRANSPOSED>(dJ, gm.x, gm.y, gm.z, T0, T1, T2);
+                        e0 += zs * S0; e1 += zs * S1; e2 += zs * S2;
+                        f0 += zDs * S0 + zs * T0;
+                        f1 += zDs * S1 + zs * T1;
+                        f2 += zDs * S2 + zs * T2;
                     } else {
-                        S0 = dJ[0] * gm.x + dJ[3] * gm.y + dJ[6] * gm.z;
-                        S1 = dJ[1] * gm.x + dJ[4] * gm.y + dJ[7] * gm.z;
-                        S2 = dJ[2] * gm.x + dJ[5] * gm.y + dJ[8] * gm.z;
+                        const double zs = zeta_f64<KK>(r * a.w) * is3;
+                        e0 += zs * S0; e1 += zs * S1; e2 += zs * S2;
                     }
-                    e0 += zs * S0;
-                    e1 += zs * S1;
-                    e2 += zs * S2;
                 }
             }
         }
         if (active) {
-            const double d[3] = {e0, e1, e2};
+            const double d[6] = {e0, e1, e2, f0, f1, f2};
             if (ch.part < 0)
-                estr_finish(d, perm[it], cap, E);
+                estr_finish<NV>(d, perm[it], cap, E, DE);
             else
-                for (int k = 0; k < 3; ++k) part[k * pstride + ch.part + (it - A.begin)] = d[k];
+                for (int k = 0; k < NV; ++k) part[k * pstride + ch.part + (it - A.begin)] = d[k];
         }
     }
 }
@@ -1442,36 +1505,43 @@ __global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_kernel(const P2PChunk *__r
 // relative to the target leaf's centre and on J relative to the J of the
 // leaf's first particle (J_i - J_j then keeps its relative precision), FP32
 // partials per source tile folded into FP64.  Same streaming of the source
-// leaves and replication of small target leaves as p2p_mixed_kernel.
+// leaves and replication of small target leaves as p2p_mixed_kernel.  DYN
+// adds D[E] from DJ, taken relative to the first particle's DJ the same way.
 
-// zeta(r / sigma) / sigma^3 in FP32; p0 = 1/sigma, p1 = 1/sigma^2.
-template <int KK>
-__device__ __forceinline__ float zeta_s3_f32(float r2, float p0, float p1)
+// zeta(r / sigma) / sigma^3 and zD(r / sigma) / sigma^3 in FP32; p0 = 1/sigma,
+// p1 = 1/sigma^2.
+template <int KK, bool DYN>
+__device__ __forceinline__ void zeta_s3_f32(float r2, float p0, float p1, float &zs, float &zDs)
 {
     const float is3 = p0 * p1;
     if constexpr (KK == GAUSSIANERF) {
-        return 0.06349363593424097f * fexp2(-0.5f * LOG2E * r2 * p1) * is3;   // (2 pi)^-3/2
+        zs = 0.06349363593424097f * fexp2(-0.5f * LOG2E * r2 * p1) * is3;   // (2 pi)^-3/2
+        if constexpr (DYN) zDs = fmaf(r2, p1, -3.0f) * zs;                 // (rho^2 - 3) zeta
     } else if constexpr (KK == GAUSSIAN) {
-        const float rho2 = r2 * p1;
-        return 0.238732414637843f * fexp2(-LOG2E * rho2 * sqrtf(rho2)) * is3;   // 3/(4 pi)
+        const float rho2 = r2 * p1, rho3 = rho2 * sqrtf(rho2);
+        zs = 0.238732414637843f * fexp2(-LOG2E * rho3) * is3;               // 3/(4 pi)
+        if constexpr (DYN) zDs = 3.0f * (rho3 - 1.0f) * zs;                 // 3 (rho^3 - 1) zeta
     } else {   // WINCKELMANS: 15/(8 pi) (rho^2 + 1)^-7/2
         const float si = frsq(fmaf(r2, p1, 1.0f)), si2 = si * si;
-        return 0.5968310365946075f * si2 * si2 * si2 * si * is3;
+        zs = 0.5968310365946075f * si2 * si2 * si2 * si * is3;
+        if constexpr (DYN) zDs = fmaf(-7.0f, si2, 4.0f) * zs;              // (7 rho^2/q - 3) zeta
     }
 }
 
-template <int KK, bool TRANSPOSED>
-__device__ __forceinline__ void estr_mixed_pair(const float4 a, const float4 g, const float4 q0,
-                                                const float4 q1, float q8, float xi, float yi,
-                                                float zi, const float (&Ji)[9], float &e0,
-                                                float &e1, float &e2)
+// LDS tile of the mixed Estr: per source position (x, y, z relative, 1/sigma),
+// (Gamma, 1/sigma^2), J - Jr in 9 floats, and with DYN DJ - DJr in 9 floats.
+struct EstrTile {
+    float4 sp[P2P_BLOCK], sg[P2P_BLOCK], sj0[P2P_BLOCK], sj1[P2P_BLOCK];
+    float sj8[P2P_BLOCK];
+    float4 sd0[P2P_BLOCK], sd1[P2P_BLOCK];
+    float sd8[P2P_BLOCK];
+};
+
+template <bool TRANSPOSED>
+__device__ __forceinline__ void stretch_f32(float d0, float d1, float d2, float d3, float d4,
+                                            float d5, float d6, float d7, float d8, const float4 g,
+                                            float &S0, float &S1, float &S2)
 {
-    const float dx = xi - a.x, dy = yi - a.y, dz = zi - a.z;
-    const float zs = zeta_s3_f32<KK>(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), a.w, g.w);
-    const float d0 = Ji[0] - q0.x, d1 = Ji[1] - q0.y, d2 = Ji[2] - q0.z;
-    const float d3 = Ji[3] - q0.w, d4 = Ji[4] - q1.x, d5 = Ji[5] - q1.y;
-    const float d6 = Ji[6] - q1.z, d7 = Ji[7] - q1.w, d8 = Ji[8] - q8;
-    float S0, S1, S2;
     if constexpr (TRANSPOSED) {
         S0 = fmaf(d0, g.x, fmaf(d1, g.y, d2 * g.z));
         S1 = fmaf(d3, g.x, fmaf(d4, g.y, d5 * g.z));
@@ -1481,43 +1551,63 @@ __device__ __forceinline__ void estr_mixed_pair(const float4 a, const float4 g, 
         S1 = fmaf(d1, g.x, fmaf(d4, g.y, d7 * g.z));
         S2 = fmaf(d2, g.x, fmaf(d5, g.y, d8 * g.z));
     }
-    e0 = fmaf(zs, S0, e0);
-    e1 = fmaf(zs, S1, e1);
-    e2 = fmaf(zs, S2, e2);
 }
 
-template <int KK, bool TRANSPOSED, int R>
-__device__ __forceinline__ void estr_mixed_tile(const float4 *sp, const float4 *sg,
-                                                const float4 *sj0, const float4 *sj1,
-                                                const float *sj8, int cnt, int s, float xi,
-                                                float yi, float zi, const float (&Ji)[9],
-                                                float &e0, float &e1, float &e2)
+template <int KK, bool TRANSPOSED, bool DYN>
+__device__ __forceinline__ void estr_mixed_pair(const EstrTile &T, int j, float xi, float yi,
+                                                float zi, const float (&Ji)[9],
+                                                const float (&DJi)[9], float (&e)[6])
 {
-    if (cnt == P2P_BLOCK) {
-#pragma unroll 4
-        for (int i = 0; i < P2P_BLOCK / R; ++i) {
-            const int j = i * R + s;
-            estr_mixed_pair<KK, TRANSPOSED>(sp[j], sg[j], sj0[j], sj1[j], sj8[j], xi, yi, zi, Ji,
-                                            e0, e1, e2);
-        }
-    } else {
-        for (int j = s; j < cnt; j += R)
-            estr_mixed_pair<KK, TRANSPOSED>(sp[j], sg[j], sj0[j], sj1[j], sj8[j], xi, yi, zi, Ji,
-                                            e0, e1, e2);
+    const float4 a = T.sp[j], g = T.sg[j], q0 = T.sj0[j], q1 = T.sj1[j];
+    const float dx = xi - a.x, dy = yi - a.y, dz = zi - a.z;
+    float zs, zDs = 0.f;
+    zeta_s3_f32<KK, DYN>(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), a.w, g.w, zs, zDs);
+    float S0, S1, S2;
+    stretch_f32<TRANSPOSED>(Ji[0] - q0.x, Ji[1] - q0.y, Ji[2] - q0.z, Ji[3] - q0.w, Ji[4] - q1.x,
+                            Ji[5] - q1.y, Ji[6] - q1.z, Ji[7] - q1.w, Ji[8] - T.sj8[j], g, S0, S1,
+                            S2);
+    e[0] = fmaf(zs, S0, e[0]);
+    e[1] = fmaf(zs, S1, e[1]);
+    e[2] = fmaf(zs, S2, e[2]);
+    if constexpr (DYN) {
+        const float4 r0 = T.sd0[j], r1 = T.sd1[j];
+        float T0, T1, T2;
+        stretch_f32<TRANSPOSED>(DJi[0] - r0.x, DJi[1] - r0.y, DJi[2] - r0.z, DJi[3] - r0.w,
+                                DJi[4] - r1.x, DJi[5] - r1.y, DJi[6] - r1.z, DJi[7] - r1.w,
+                                DJi[8] - T.sd8[j], g, T0, T1, T2);
+        e[3] = fmaf(zDs, S0, fmaf(zs, T0, e[3]));
+        e[4] = fmaf(zDs, S1, fmaf(zs, T1, e[4]));
+        e[5] = fmaf(zDs, S2, fmaf(zs, T2, e[5]));
     }
 }
 
-template <int KK, bool TRANSPOSED>
+template <int KK, bool TRANSPOSED, bool DYN, int R>
+__device__ __forceinline__ void estr_mixed_tile(const EstrTile &T, int cnt, int s, float xi,
+                                                float yi, float zi, const float (&Ji)[9],
+                                                const float (&DJi)[9], float (&e)[6])
+{
+    if (cnt == P2P_BLOCK) {
+#pragma unroll 4
+        for (int i = 0; i < P2P_BLOCK / R; ++i)
+            estr_mixed_pair<KK, TRANSPOSED, DYN>(T, i * R + s, xi, yi, zi, Ji, DJi, e);
+    } else {
+        for (int j = s; j < cnt; j += R)
+            estr_mixed_pair<KK, TRANSPOSED, DYN>(T, j, xi, yi, zi, Ji, DJi, e);
+    }
+}
+
+template <int KK, bool TRANSPOSED, bool DYN>
 __global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_mixed_kernel(
     const P2PChunk *__restrict__ chunks, const FmmCellDev *__restrict__ cells,
     const int32_t *__restrict__ src, const double4 *__restrict__ rec,
     const int32_t *__restrict__ perm, int64_t cap, double *__restrict__ E,
-    double *__restrict__ part, int64_t pstride)
+    double *__restrict__ DE, double *__restrict__ part, int64_t pstride)
 {
-    __shared__ float4 sp[P2P_BLOCK], sg[P2P_BLOCK], sj0[P2P_BLOCK], sj1[P2P_BLOCK];
-    __shared__ float sj8[P2P_BLOCK];
+    constexpr int NR = DYN ? 7 : 5;
+    constexpr int NV = DYN ? 6 : 3;
+    __shared__ EstrTile T;
     __shared__ int32_t lbeg[P2P_SLOTS], lpre[P2P_SLOTS + 1];
-    __shared__ double red[3][P2P_BLOCK];
+    __shared__ double red[NV][P2P_BLOCK];
     const int lane = threadIdx.x;
     const P2PChunk ch = chunks[blockIdx.x];
     const int c = ch.c;
@@ -1529,92 +1619,96 @@ __global__ __launch_bounds__(P2P_BLOCK) void fmm_estr_mixed_kernel(
         const int sl = lane / nt;                  // slice of this lane
         const int t = lane - sl * nt;              // target of this lane
         const bool active = sl < R;
-        // reference J: the block's first target
-        double Jr[9];
+        // reference J (and DJ): the block's first target
+        double Jr[9], DJr[9];
+        for (int k = 0; k < 9; ++k) DJr[k] = 0.0;
+        estr_rec_j<DYN>(rec + NR * tb, Jr, DJr);
+        float xi, yi, zi, Ji[9], DJi[9];
         {
-            const double4 u = rec[5 * tb + 2], v = rec[5 * tb + 3], w = rec[5 * tb + 4];
-            Jr[0] = u.x; Jr[1] = u.y; Jr[2] = u.z; Jr[3] = u.w;
-            Jr[4] = v.x; Jr[5] = v.y; Jr[6] = v.z; Jr[7] = v.w; Jr[8] = w.x;
-        }
-        float xi, yi, zi, Ji[9];
-        {
-            const double4 a = rec[5 * (tb + t)], u = rec[5 * (tb + t) + 2],
-                          v = rec[5 * (tb + t) + 3], w = rec[5 * (tb + t) + 4];
+            double Jt[9], DJt[9];
+            for (int k = 0; k < 9; ++k) DJt[k] = 0.0;
+            const double4 a = rec[NR * (tb + t)];
+            estr_rec_j<DYN>(rec + NR * (tb + t), Jt, DJt);
             xi = (float)(a.x - A.cx); yi = (float)(a.y - A.cy); zi = (float)(a.z - A.cz);
-            Ji[0] = (float)(u.x - Jr[0]); Ji[1] = (float)(u.y - Jr[1]);
-            Ji[2] = (float)(u.z - Jr[2]); Ji[3] = (float)(u.w - Jr[3]);
-            Ji[4] = (float)(v.x - Jr[4]); Ji[5] = (float)(v.y - Jr[5]);
-            Ji[6] = (float)(v.z - Jr[6]); Ji[7] = (float)(v.w - Jr[7]);
-            Ji[8] = (float)(w.x - Jr[8]);
+            for (int k = 0; k < 9; ++k) {
+                Ji[k] = (float)(Jt[k] - Jr[k]);
+                DJi[k] = (float)(DJt[k] - DJr[k]);
+            }
         }
         const int s = active ? sl : 0;
-        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+        double d[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int64_t eb = e0; eb < e1; eb += P2P_SLOTS) {
             const int nl = (int)min<int64_t>(P2P_SLOTS, e1 - eb);
             __syncthreads();
             const int total = stream_setup(cells, src, eb, nl, lane, lbeg, lpre);
             // the records of the next tile load while this one computes
-            double4 ra = make_double4(0.0, 0.0, 0.0, 0.0), rg = ra, ru = ra, rv = ra, rw = ra;
+            double4 rr[NR];
+            for (int k = 0; k < NR; ++k) rr[k] = make_double4(0.0, 0.0, 0.0, 0.0);
             if (lane < total) {
                 const int js = stream_particle(lbeg, lpre, nl, lane);
-                ra = rec[5 * js]; rg = rec[5 * js + 1]; ru = rec[5 * js + 2];
-                rv = rec[5 * js + 3]; rw = rec[5 * js + 4];
+                for (int k = 0; k < NR; ++k) rr[k] = rec[NR * js + k];
             }
             for (int base = 0; base < total; base += P2P_BLOCK) {
                 __syncthreads();
                 if (base + lane < total) {
-                    sp[lane] = make_float4((float)(ra.x - A.cx), (float)(ra.y - A.cy),
-                                           (float)(ra.z - A.cz), (float)ra.w);
-                    sg[lane] = make_float4((float)rg.x, (float)rg.y, (float)rg.z,
-                                           (float)(ra.w * ra.w));
-                    sj0[lane] = make_float4((float)(ru.x - Jr[0]), (float)(ru.y - Jr[1]),
-                                            (float)(ru.z - Jr[2]), (float)(ru.w - Jr[3]));
-                    sj1[lane] = make_float4((float)(rv.x - Jr[4]), (float)(rv.y - Jr[5]),
-                                            (float)(rv.z - Jr[6]), (float)(rv.w - Jr[7]));
-                    sj8[lane] = (float)(rw.x - Jr[8]);
+                    const double4 ra = rr[0], rg = rr[1], ru = rr[2], rv = rr[3], rw = rr[4];
+                    T.sp[lane] = make_float4((float)(ra.x - A.cx), (float)(ra.y - A.cy),
+                                             (float)(ra.z - A.cz), (float)ra.w);
+                    T.sg[lane] = make_float4((float)rg.x, (float)rg.y, (float)rg.z,
+                                             (float)(ra.w * ra.w));
+                    T.sj0[lane] = make_float4((float)(ru.x - Jr[0]), (float)(ru.y - Jr[1]),
+                                              (float)(ru.z - Jr[2]), (float)(ru.w - Jr[3]));
+                    T.sj1[lane] = make_float4((float)(rv.x - Jr[4]), (float)(rv.y - Jr[5]),
+                                              (float)(rv.z - Jr[6]), (float)(rv.w - Jr[7]));
+                    T.sj8[lane] = (float)(rw.x - Jr[8]);
+                    if constexpr (DYN) {
+                        const double4 rx = rr[5], ry = rr[6];
+                        T.sd0[lane] = make_float4((float)(rw.y - DJr[0]), (float)(rw.z - DJr[1]),
+                                                  (float)(rw.w - DJr[2]), (float)(rx.x - DJr[3]));
+                        T.sd1[lane] = make_float4((float)(rx.y - DJr[4]), (float)(rx.z - DJr[5]),
+                                                  (float)(rx.w - DJr[6]), (float)(ry.x - DJr[7]));
+                        T.sd8[lane] = (float)(ry.y - DJr[8]);
+                    }
                 }
                 __syncthreads();
                 const int qn = base + P2P_BLOCK + lane;
                 if (qn < total) {
                     const int js = stream_particle(lbeg, lpre, nl, qn);
-                    ra = rec[5 * js]; rg = rec[5 * js + 1]; ru = rec[5 * js + 2];
-                    rv = rec[5 * js + 3]; rw = rec[5 * js + 4];
+                    for (int k = 0; k < NR; ++k) rr[k] = rec[NR * js + k];
                 }
                 const int cnt = min(P2P_BLOCK, total - base);
-                float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+                float f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 switch (R) {
-                case 8: estr_mixed_tile<KK, TRANSPOSED, 8>(sp, sg, sj0, sj1, sj8, cnt, s, xi, yi, zi, Ji, f0, f1, f2); break;
-                case 4: estr_mixed_tile<KK, TRANSPOSED, 4>(sp, sg, sj0, sj1, sj8, cnt, s, xi, yi, zi, Ji, f0, f1, f2); break;
-                case 2: estr_mixed_tile<KK, TRANSPOSED, 2>(sp, sg, sj0, sj1, sj8, cnt, s, xi, yi, zi, Ji, f0, f1, f2); break;
-                default: estr_mixed_tile<KK, TRANSPOSED, 1>(sp, sg, sj0, sj1, sj8, cnt, s, xi, yi, zi, Ji, f0, f1, f2); break;
+                case 8: estr_mixed_tile<KK, TRANSPOSED, DYN, 8>(T, cnt, s, xi, yi, zi, Ji, DJi, f); break;
+                case 4: estr_mixed_tile<KK, TRANSPOSED, DYN, 4>(T, cnt, s, xi, yi, zi, Ji, DJi, f); break;
+                case 2: estr_mixed_tile<KK, TRANSPOSED, DYN, 2>(T, cnt, s, xi, yi, zi, Ji, DJi, f); break;
+                default: estr_mixed_tile<KK, TRANSPOSED, DYN, 1>(T, cnt, s, xi, yi, zi, Ji, DJi, f); break;
                 }
-                d0 += f0; d1 += f1; d2 += f2;
+                for (int k = 0; k < NV; ++k) d[k] += f[k];
             }
         }
         if (R > 1) {   // add the slices of each target in slice order
             __syncthreads();
-            red[0][lane] = d0; red[1][lane] = d1; red[2][lane] = d2;
+            for (int k = 0; k < NV; ++k) red[k][lane] = d[k];
             __syncthreads();
             if (lane < nt) {
-                for (int q = 1; q < R; ++q) {
-                    d0 += red[0][lane + q * nt];
-                    d1 += red[1][lane + q * nt];
-                    d2 += red[2][lane + q * nt];
-                }
+                for (int q = 1; q < R; ++q)
+                    for (int k = 0; k < NV; ++k) d[k] += red[k][lane + q * nt];
             }
         }
         if (lane < nt) {
-            const double d[3] = {d0, d1, d2};
             if (ch.part < 0)
-                estr_finish(d, perm[tb + lane], cap, E);
+                estr_finish<NV>(d, perm[tb + lane], cap, E, DE);
             else
-                for (int k = 0; k < 3; ++k) part[k * pstride + ch.part + (tb + lane - A.begin)] = d[k];
+                for (int k = 0; k < NV; ++k)
+                    part[k * pstride + ch.part + (tb + lane - A.begin)] = d[k];
         }
     }
 }
 
 // Partial sums of the leaves cut into several work items, added in item order
-// and stored.  MODE 0: FP64 P2P (12 sums), 1: mixed P2P (14), 2: Estr (3).
+// and stored.  MODE 0: FP64 P2P (12 sums), 1: mixed P2P (14), 2: Estr (3),
+// 3: Estr and D[E] (6).
 template <int MODE>
 __global__ __launch_bounds__(P2P_BLOCK) void p2p_part_reduce_kernel(
     const int32_t *__restrict__ leaves, const int64_t *__restrict__ coff,
@@ -1622,7 +1716,7 @@ __global__ __launch_bounds__(P2P_BLOCK) void p2p_part_reduce_kernel(
     const double *__restrict__ part, int64_t pstride, const int32_t *__restrict__ perm,
     int64_t cap, double *__restrict__ U, double *__restrict__ J, double *__restrict__ E)
 {
-    constexpr int NV = MODE == 0 ? 12 : MODE == 1 ? 14 : 3;
+    constexpr int NV = MODE == 0 ? 12 : MODE == 1 ? 14 : MODE == 2 ? 3 : 6;
     const int64_t li = blockIdx.x;
     const int64_t n = coff[li + 1] - coff[li];
     if (n <= 1) return;
@@ -1639,8 +1733,10 @@ __global__ __launch_bounds__(P2P_BLOCK) void p2p_part_reduce_kernel(
             p2p_f64_finish(d, i, cap, U, J);
         else if constexpr (MODE == 1)
             p2p_mixed_finish(d, i, cap, U, J);
+        else if constexpr (MODE == 2)
+            estr_finish<3>(d, i, cap, E, nullptr);
         else
-            estr_finish(d, i, cap, E);
+            estr_finish<6>(d, i, cap, E, U);   // U carries DE here
     }
 }
 
@@ -1658,30 +1754,40 @@ static hipError_t launch_part_reduce(hipStream_t st, const int32_t *leaves, int6
 }
 
 hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int transposed,
-                           int precision, const DeviceField &f, Scratch &s)
+                           int precision, const DeviceField &f, Scratch &s, const double *DJ,
+                           double *DE)
 {
     if (f.np == 0) return hipSuccess;
-    fmm_estr_gather_kernel<<<gridn(f.np, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
-        f.X, f.G, f.sigma, f.J, s.perm, f.np, f.cap, s.srcE);
+    const bool dyn = DJ != nullptr;
+    if (dyn)
+        fmm_estr_gather_kernel<true><<<gridn(f.np, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
+            f.X, f.G, f.sigma, f.J, DJ, s.perm, f.np, f.cap, s.srcE);
+    else
+        fmm_estr_gather_kernel<false><<<gridn(f.np, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
+            f.X, f.G, f.sigma, f.J, DJ, s.perm, f.np, f.cap, s.srcE);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int64_t nleaf = (int64_t)S.plan.leaves.size();
     if (nleaf == 0 || S.nchunk == 0) return hipSuccess;
     const unsigned g = (unsigned)S.nchunk;
-#define VPMX_FESTR(KK)                                                                          \
-    if (precision && transposed)                                                                \
-        fmm_estr_mixed_kernel<KK, true><<<g, P2P_BLOCK, 0, st>>>(                               \
-            S.chunks, S.cells, S.p2p_src, s.srcE, s.perm, f.cap, f.E, S.part, S.nslot);         \
-    else if (precision)                                                                         \
-        fmm_estr_mixed_kernel<KK, false><<<g, P2P_BLOCK, 0, st>>>(                              \
-            S.chunks, S.cells, S.p2p_src, s.srcE, s.perm, f.cap, f.E, S.part, S.nslot);         \
-    else if (transposed)                                                                        \
-        fmm_estr_kernel<KK, true><<<g, P2P_BLOCK, 0, st>>>(S.chunks, S.cells, S.p2p_src, s.srcE, \
-                                                         s.perm, f.cap, f.E, S.part, S.nslot);  \
-    else                                                                                        \
-        fmm_estr_kernel<KK, false><<<g, P2P_BLOCK, 0, st>>>(S.chunks, S.cells, S.p2p_src,        \
-                                                          s.srcE, s.perm, f.cap, f.E, S.part,   \
-                                                          S.nslot);
+#define VPMX_FESTR3(KK, TR, DY)                                                                  \
+    if (precision)                                                                               \
+        fmm_estr_mixed_kernel<KK, TR, DY><<<g, P2P_BLOCK, 0, st>>>(                             \
+            S.chunks, S.cells, S.p2p_src, s.srcE, s.perm, f.cap, f.E, DE, S.part, S.nslot);      \
+    else                                                                                         \
+        fmm_estr_kernel<KK, TR, DY><<<g, P2P_BLOCK, 0, st>>>(S.chunks, S.cells, S.p2p_src,      \
+                                                            s.srcE, s.perm, f.cap, f.E, DE,      \
+                                                            S.part, S.nslot);
+#define VPMX_FESTR(KK)                                                                           \
+    if (dyn && transposed) {                                                                     \
+        VPMX_FESTR3(KK, true, true)                                                              \
+    } else if (dyn) {                                                                            \
+        VPMX_FESTR3(KK, false, true)                                                             \
+    } else if (transposed) {                                                                     \
+        VPMX_FESTR3(KK, true, false)                                                             \
+    } else {                                                                                     \
+        VPMX_FESTR3(KK, false, false)                                                            \
+    }
     switch (kernel) {
     case GAUSSIAN: VPMX_FESTR(GAUSSIAN); break;
     case GAUSSIANERF: VPMX_FESTR(GAUSSIANERF); break;
@@ -1689,10 +1795,57 @@ hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int tr
     default: return hipErrorInvalidValue;
     }
 #undef VPMX_FESTR
+#undef VPMX_FESTR3
     e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (dyn)
+        return launch_part_reduce<3>(st, S.leaves, nleaf, S.chunk_off, S.part_base, S.cells,
+                                     S.part, S.nslot, s.perm, f.cap, DE, nullptr, f.E);
     return launch_part_reduce<2>(st, S.leaves, nleaf, S.chunk_off, S.part_base, S.cells, S.part,
                                  S.nslot, s.perm, f.cap, nullptr, nullptr, f.E);
+}
+
+// Core-size derivatives over the near-field lists: the P2P kernels in their
+// DS mode, adding into DU, DJ cleared first.
+__global__ __launch_bounds__(FMM_BLOCK) void fmm_zero_kernel(double *__restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+hipError_t launch_fmm_dsig(hipStream_t st, const FmmState &S, int kernel, int precision,
+                           const DeviceField &f, const Scratch &s, double *DU, double *DJ)
+{
+    if (f.np == 0) return hipSuccess;
+    fmm_zero_kernel<<<gridn(3 * f.cap, FMM_BLOCK), FMM_BLOCK, 0, st>>>(DU, 3 * f.cap);
+    fmm_zero_kernel<<<gridn(9 * f.cap, FMM_BLOCK), FMM_BLOCK, 0, st>>>(DJ, 9 * f.cap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t nleaf = (int64_t)S.plan.leaves.size();
+    if (nleaf == 0 || S.nchunk == 0) return hipSuccess;
+    const unsigned g = (unsigned)S.nchunk;
+#define VPMX_DS(KK)                                                                             \
+    if (precision == 0)                                                                         \
+        p2p_f64_kernel<KK, true><<<g, P2P_BLOCK, 0, st>>>(S.chunks, S.cells, S.p2p_src, s.src64, \
+                                                         s.perm, f.cap, DU, DJ, S.part, S.nslot); \
+    else                                                                                        \
+        p2p_mixed_kernel<KK, true><<<g, P2P_BLOCK, 0, st>>>(S.chunks, S.cells, S.p2p_src,       \
+                                                           s.src64, s.perm, f.cap, DU, DJ,       \
+                                                           S.part, S.nslot);
+    switch (kernel) {
+    case GAUSSIAN: VPMX_DS(GAUSSIAN); break;
+    case GAUSSIANERF: VPMX_DS(GAUSSIANERF); break;
+    case WINCKELMANS: VPMX_DS(WINCKELMANS); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef VPMX_DS
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return precision == 0
+               ? launch_part_reduce<0>(st, S.leaves, nleaf, S.chunk_off, S.part_base, S.cells,
+                                       S.part, S.nslot, s.perm, f.cap, DU, DJ, nullptr)
+               : launch_part_reduce<1>(st, S.leaves, nleaf, S.chunk_off, S.part_base, S.cells,
+                                       S.part, S.nslot, s.perm, f.cap, DU, DJ, nullptr);
 }
 
 // ---------------------------------------------------------------------------

@@ -12,12 +12,12 @@
 namespace vpmx {
 
 // Timed kernel classes (vpmx_get_timing's `which`).
-enum TimedKind { T_UJ = 0, T_ESTR = 1, T_UPDATE = 2, T_FMM = 3, T_GRAPH = 4, T_NKIND = 5 };
+enum TimedKind { T_UJ = 0, T_ESTR = 1, T_UPDATE = 2, T_FMM = 3, T_GRAPH = 4, T_DSIG = 5, T_NKIND = 6 };
 
 struct Timer {
     bool on = false;
-    double total_ms[T_NKIND] = {0, 0, 0, 0, 0};
-    int64_t count[T_NKIND] = {0, 0, 0, 0, 0};
+    double total_ms[T_NKIND] = {0, 0, 0, 0, 0, 0};
+    int64_t count[T_NKIND] = {0, 0, 0, 0, 0, 0};
     struct Pending { int kind; hipEvent_t a, b; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
@@ -29,19 +29,23 @@ struct DeviceField {
     int64_t np = 0, cap = 0;
     double *X = nullptr, *G = nullptr, *sigma = nullptr, *U = nullptr, *J = nullptr;
     double *E = nullptr, *C = nullptr, *M = nullptr;   // C: 3 components
-    double *D = nullptr;       // dynamic SFS: test-filter stretching and Estr (6)
-    double *sig0 = nullptr;    // dynamic SFS: saved sigma
+    // dynamic SFS: core-size derivatives (D = sigma d/dsigma of every core)
+    // D[U] (3), D[J] (9), D[E] (3)
+    double *D = nullptr;
     double *vol = nullptr;     // particle volume (sigma^3 when the layout has none)
     double *V = nullptr;       // viscous: PSE rate of Gamma (3), allocated on first use
     uint8_t *is_static = nullptr;
     bool any_static = false;
+    double *DU() const { return D; }
+    double *DJ() const { return D + 3 * cap; }
+    double *DE() const { return D + 12 * cap; }
 };
 
 // Scratch of the pairwise passes, grown with the field.
 struct Scratch {
     int64_t cap = 0;
     double4 *src64 = nullptr;        // FP64 UJ source records, 2 per particle
-    double4 *srcE = nullptr;         // FP64 Estr source records, 5 per particle
+    double4 *srcE = nullptr;         // FP64 Estr source records, 7 per particle
     double4 *spos = nullptr;         // Morton-sorted (x, y, z, p0) FP64
     float4 *sgam = nullptr;          // Morton-sorted (Gx, Gy, Gz, p1) FP32
     double4 *tbnd = nullptr;         // per 256-slot tile of the sorted field:
@@ -161,17 +165,18 @@ hipError_t launch_uj_mixed_targets(hipStream_t st, int kernel, const DeviceField
                                    const Scratch &s, const double4 *tgt, int64_t nt,
                                    const int32_t *tperm, int64_t ldo, double *U, double *J,
                                    int reset);
+// Estr into f.E; with DJ (9 x cap) also its core-size derivative into DE.
 hipError_t launch_estr_f64(hipStream_t st, int kernel, int transposed, const DeviceField &f,
-                           Scratch &s);
+                           Scratch &s, const double *DJ = nullptr, double *DE = nullptr);
+// Core-size derivatives D[U] (3 x cap), D[J] (9 x cap) of every particle from
+// the records of the UJ that just ran (FP64 all-pairs / mixed near pairs).
+hipError_t launch_dsig_f64(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
+                           double *DU, double *DJ);
+hipError_t launch_dsig_mixed(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
+                             double *DU, double *DJ);
 hipError_t launch_sfs_coeff(hipStream_t st, const DeviceField &f, double Cs, int clip);
-// dynamic SFS: sig0 <- sigma, sigma <- alpha sigma
-hipError_t launch_sigma_scale(hipStream_t st, const DeviceField &f, double alpha);
-// sigma <- sig0
-hipError_t launch_sigma_restore(hipStream_t st, const DeviceField &f);
-// D[0..2] <- stretching(J, Gamma), D[3..5] <- E   (test filter)
-hipError_t launch_dyn_stash(hipStream_t st, const DeviceField &f, int transposed);
-// C from the test/domain differences, Lagrangian average, clipping
-hipError_t launch_dyn_coeff(hipStream_t st, const DeviceField &f, int transposed, double zeta0,
+// dynamic SFS: C from f.DJ(), f.DE(), Lagrangian average, bounds, clipping
+hipError_t launch_dyn_coeff(hipStream_t st, const DeviceField &f, double alpha, double zeta0,
                             double rlxf, double minC, double maxC, int clip);
 hipError_t launch_fill(hipStream_t st, double *p, int64_t n, double v);
 hipError_t launch_update(hipStream_t st, const DeviceField &f, int transposed, int sfs,

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(EW_BLOCK) void pack_f64_kernel(const double *__rest
     out[2 * i + 1] = make_double4(G[cap + i], G[2 * cap + i], s, 1.0 / s);
 }
 
-template <int K>
+template <int K, bool DS>
 __global__ __launch_bounds__(UJ_BLOCK) void uj_f64_kernel(const double4 *__restrict__ src,
                                                           int64_t ns,
                                                           const double *__restrict__ X,
@@ -57,12 +57,12 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_f64_kernel(const double4 *__restr
 {
     __shared__ double4 tile[2 * UJ_BLOCK];
     const int64_t t0 = (int64_t)blockIdx.x * UJ_BLOCK;
-    uj_f64_block<K, UJ_BLOCK>(src, 0, ns, X, t0, nt, cap, U, J, reset, tile);
+    uj_f64_block<K, UJ_BLOCK, DS>(src, 0, ns, X, t0, nt, cap, U, J, reset, tile);
 }
 
 // Split source range: workgroup (block b, segment g) sums sources
 // [g seglen, (g+1) seglen) for target block b into part[(g 12 + k) pld + i].
-template <int K>
+template <int K, bool DS>
 __global__ __launch_bounds__(UJ_BLOCK) void uj_f64_split_kernel(const double4 *__restrict__ src,
                                                                 int64_t ns,
                                                                 const double *__restrict__ X,
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_f64_split_kernel(const double4 *_
     const int64_t s0 = (int64_t)g * seglen;
     const int64_t s1 = s0 + seglen < ns ? s0 + seglen : ns;
     double d[12];
-    uj_f64_sums<K, UJ_BLOCK>(src, s0, s1, X, t0, nt, cap, tile, d);
+    uj_f64_sums<K, UJ_BLOCK, DS>(src, s0, s1, X, t0, nt, cap, tile, d);
     const int64_t i = t0 + threadIdx.x;
     if (i < nt)
         for (int k = 0; k < 12; ++k) part[((int64_t)g * 12 + k) * pld + i] = d[k];
@@ -128,7 +128,7 @@ hipError_t launch_pack_f64(hipStream_t st, const DeviceField &f, Scratch &s)
     return hipGetLastError();
 }
 
-template <int K>
+template <int K, bool DS = false>
 static hipError_t uj_f64_go(hipStream_t st, const double4 *src, int64_t ns, const double *X,
                             int64_t nt, int64_t cap, double *U, double *J, int reset,
                             double *part)
@@ -136,10 +136,10 @@ static hipError_t uj_f64_go(hipStream_t st, const double4 *src, int64_t ns, cons
     const int64_t nb = (nt + UJ_BLOCK - 1) / UJ_BLOCK;
     const Split sp = split_for(nb, ns);
     if (sp.nseg == 1 || !part) {
-        uj_f64_kernel<K><<<(unsigned)nb, UJ_BLOCK, 0, st>>>(src, ns, X, nt, cap, U, J, reset);
+        uj_f64_kernel<K, DS><<<(unsigned)nb, UJ_BLOCK, 0, st>>>(src, ns, X, nt, cap, U, J, reset);
         return hipGetLastError();
     }
-    uj_f64_split_kernel<K><<<(unsigned)(nb * sp.nseg), UJ_BLOCK, 0, st>>>(src, ns, X, nt, cap,
+    uj_f64_split_kernel<K, DS><<<(unsigned)(nb * sp.nseg), UJ_BLOCK, 0, st>>>(src, ns, X, nt, cap,
                                                                           sp.seglen, sp.nseg,
                                                                           part, nt);
     hipError_t e = hipGetLastError();
@@ -175,6 +175,20 @@ hipError_t launch_uj_f64(hipStream_t st, int kernel, const DeviceField &f, const
 {
     if (f.np == 0) return hipSuccess;
     return uj_f64_any(st, kernel, s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset, s.kpart);
+}
+
+// Core-size derivatives D[U], D[J] of every particle into DU (3 x cap) and DJ
+// (9 x cap), FP64, from the source records of the UJ that just ran.
+hipError_t launch_dsig_f64(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
+                           double *DU, double *DJ)
+{
+    if (f.np == 0) return hipSuccess;
+    switch (kernel) {
+    case GAUSSIAN: return uj_f64_go<GAUSSIAN, true>(st, s.src64, f.np, f.X, f.np, f.cap, DU, DJ, 1, s.kpart);
+    case GAUSSIANERF: return uj_f64_go<GAUSSIANERF, true>(st, s.src64, f.np, f.X, f.np, f.cap, DU, DJ, 1, s.kpart);
+    case WINCKELMANS: return uj_f64_go<WINCKELMANS, true>(st, s.src64, f.np, f.X, f.np, f.cap, DU, DJ, 1, s.kpart);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 // ===========================================================================
@@ -261,7 +275,7 @@ __global__ __launch_bounds__(EW_BLOCK) void pack_mixed_kernel(const double *__re
     sgam[s] = make_float4((float)G[i], (float)G[cap + i], (float)G[2 * cap + i], (float)p1);
 }
 
-template <int K>
+template <int K, bool DS>
 __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_kernel(const double4 *__restrict__ tgt,
                                                             int64_t nt,
                                                             const int32_t *__restrict__ perm,
@@ -275,13 +289,13 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_kernel(const double4 *__res
     __shared__ float4 tpos[UJ_BLOCK];
     __shared__ float4 tgam[UJ_BLOCK];
     const int64_t t0 = (int64_t)blockIdx.x * UJ_BLOCK;
-    uj_mixed_block<K, UJ_BLOCK>(tgt, t0, nt, perm, spos, sgam, 0, n, cap, U, J, reset, tpos, tgam,
-                                tb);
+    uj_mixed_block<K, UJ_BLOCK, DS>(tgt, t0, nt, perm, spos, sgam, 0, n, cap, U, J, reset, tpos,
+                                    tgam, tb);
 }
 
 // Split source range of the mixed kernel: workgroup (block b, segment g) writes
 // its 14 sums to part[(g 14 + k) pld + slot].
-template <int K>
+template <int K, bool DS>
 __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_split_kernel(const double4 *__restrict__ tgt,
                                                                   int64_t nt,
                                                                   const double4 *__restrict__ spos,
@@ -299,7 +313,7 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_split_kernel(const double4 
     const int64_t s0 = (int64_t)g * seglen;
     const int64_t s1 = s0 + seglen < n ? s0 + seglen : n;
     double d[14];
-    uj_mixed_sums<K, UJ_BLOCK>(tgt, t0, nt, spos, sgam, s0, s1, tb, 0, tpos, tgam, d);
+    uj_mixed_sums<K, UJ_BLOCK, DS>(tgt, t0, nt, spos, sgam, s0, s1, tb, 0, tpos, tgam, d);
     const int64_t s = t0 + threadIdx.x;
     if (s < nt)
         for (int k = 0; k < 14; ++k) part[((int64_t)g * 14 + k) * pld + s] = d[k];
@@ -417,7 +431,8 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed2_split_kernel(const double4
 
 // Mixed launches.  two = 1: the two-float target kernel (probes).  tb: tile
 // bounds of the sorted field when the targets are its own slots (self UJ).
-template <int K>
+// DS: the core-size derivatives (self UJ only).
+template <int K, bool DS = false>
 static hipError_t uj_mixed_go(hipStream_t st, bool two, const double4 *tgt, int64_t nt,
                               const int32_t *tperm, const double4 *spos, const float4 *sgam,
                               int64_t n, int64_t ldo, double *U, double *J, int reset,
@@ -430,8 +445,8 @@ static hipError_t uj_mixed_go(hipStream_t st, bool two, const double4 *tgt, int6
             uj_mixed2_kernel<K><<<(unsigned)nb, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, spos, sgam, n,
                                                                   ldo, U, J);
         else
-            uj_mixed_kernel<K><<<(unsigned)nb, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, spos, sgam, n,
-                                                                 ldo, U, J, reset, tb);
+            uj_mixed_kernel<K, DS><<<(unsigned)nb, UJ_BLOCK, 0, st>>>(tgt, nt, tperm, spos, sgam, n,
+                                                                     ldo, U, J, reset, tb);
         return hipGetLastError();
     }
     const unsigned g = (unsigned)(nb * sp.nseg);
@@ -439,8 +454,8 @@ static hipError_t uj_mixed_go(hipStream_t st, bool two, const double4 *tgt, int6
         uj_mixed2_split_kernel<K><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, spos, sgam, n, sp.seglen,
                                                          sp.nseg, part, nt);
     else
-        uj_mixed_split_kernel<K><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, spos, sgam, n, sp.seglen,
-                                                        sp.nseg, tb, part, nt);
+        uj_mixed_split_kernel<K, DS><<<g, UJ_BLOCK, 0, st>>>(tgt, nt, spos, sgam, n, sp.seglen,
+                                                            sp.nseg, tb, part, nt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     uj_mixed_split_reduce_kernel<<<grid_for(nt, EW_BLOCK), EW_BLOCK, 0, st>>>(
@@ -586,6 +601,24 @@ hipError_t launch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f, con
                         s.tbnd);
 }
 
+// Core-size derivatives on the mixed path, from the sorted records and tile
+// bounds of the UJ that just ran.
+hipError_t launch_dsig_mixed(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
+                             double *DU, double *DJ)
+{
+    if (f.np == 0) return hipSuccess;
+    switch (kernel) {
+    case GAUSSIAN:
+        return uj_mixed_go<GAUSSIAN, true>(st, false, s.spos, f.np, s.perm, s.spos, s.sgam, f.np, f.cap, DU, DJ, 1, s.tbnd, s.kpart);
+    case GAUSSIANERF:
+        return uj_mixed_go<GAUSSIANERF, true>(st, false, s.spos, f.np, s.perm, s.spos, s.sgam, f.np, f.cap, DU, DJ, 1, s.tbnd, s.kpart);
+    case WINCKELMANS:
+        return uj_mixed_go<WINCKELMANS, true>(st, false, s.spos, f.np, s.perm, s.spos, s.sgam, f.np, f.cap, DU, DJ, 1, s.tbnd, s.kpart);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_uj_mixed_targets(hipStream_t st, int kernel, const DeviceField &f,
                                    const Scratch &s, const double4 *tgt, int64_t nt,
                                    const int32_t *tperm, int64_t ldo, double *U, double *J,
@@ -599,96 +632,158 @@ hipError_t launch_uj_mixed_targets(hipStream_t st, int kernel, const DeviceField
 // Estr (SFS stretching), FP64
 // ===========================================================================
 
-// Source record: (x, y, z, 1/sigma) (Gx, Gy, Gz, 0) (J0..J3) (J4..J7) (J8, 0, 0, 0)
+// Source record: (x, y, z, 1/sigma) (Gx, Gy, Gz, 0) (J0..J3) (J4..J7) (J8, 0, 0, 0);
+// with the core-size derivatives DJ (dynamic SFS) the last one is
+// (J8, DJ0, DJ1, DJ2) followed by (DJ3..DJ6) (DJ7, DJ8, 0, 0): 7 per particle.
+template <bool DYN>
 __global__ __launch_bounds__(EW_BLOCK) void pack_estr_kernel(const double *__restrict__ X,
                                                              const double *__restrict__ G,
                                                              const double *__restrict__ sg,
                                                              const double *__restrict__ Jm,
+                                                             const double *__restrict__ DJ,
                                                              int64_t n, int64_t cap,
                                                              double4 *__restrict__ out)
 {
+    constexpr int NR = DYN ? 7 : 5;
     const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
-    out[5 * i] = make_double4(X[i], X[cap + i], X[2 * cap + i], 1.0 / sg[i]);
-    out[5 * i + 1] = make_double4(G[i], G[cap + i], G[2 * cap + i], 0.0);
-    out[5 * i + 2] = make_double4(Jm[i], Jm[cap + i], Jm[2 * cap + i], Jm[3 * cap + i]);
-    out[5 * i + 3] = make_double4(Jm[4 * cap + i], Jm[5 * cap + i], Jm[6 * cap + i], Jm[7 * cap + i]);
-    out[5 * i + 4] = make_double4(Jm[8 * cap + i], 0.0, 0.0, 0.0);
+    double4 *o = out + NR * i;
+    o[0] = make_double4(X[i], X[cap + i], X[2 * cap + i], 1.0 / sg[i]);
+    o[1] = make_double4(G[i], G[cap + i], G[2 * cap + i], 0.0);
+    o[2] = make_double4(Jm[i], Jm[cap + i], Jm[2 * cap + i], Jm[3 * cap + i]);
+    o[3] = make_double4(Jm[4 * cap + i], Jm[5 * cap + i], Jm[6 * cap + i], Jm[7 * cap + i]);
+    if constexpr (DYN) {
+        o[4] = make_double4(Jm[8 * cap + i], DJ[i], DJ[cap + i], DJ[2 * cap + i]);
+        o[5] = make_double4(DJ[3 * cap + i], DJ[4 * cap + i], DJ[5 * cap + i], DJ[6 * cap + i]);
+        o[6] = make_double4(DJ[7 * cap + i], DJ[8 * cap + i], 0.0, 0.0);
+    } else {
+        o[4] = make_double4(Jm[8 * cap + i], 0.0, 0.0, 0.0);
+    }
 }
 
-template <int K, bool TRANSPOSED>
+// The nine J entries (and with DYN the nine DJ entries) of an Estr record.
+template <bool DYN>
+__device__ __forceinline__ void estr_rec_j(const double4 *r, double *Jv, double *DJv)
+{
+    const double4 q2 = r[2], q3 = r[3], q4 = r[4];
+    Jv[0] = q2.x; Jv[1] = q2.y; Jv[2] = q2.z; Jv[3] = q2.w;
+    Jv[4] = q3.x; Jv[5] = q3.y; Jv[6] = q3.z; Jv[7] = q3.w; Jv[8] = q4.x;
+    if constexpr (DYN) {
+        const double4 q5 = r[5], q6 = r[6];
+        DJv[0] = q4.y; DJv[1] = q4.z; DJv[2] = q4.w;
+        DJv[3] = q5.x; DJv[4] = q5.y; DJv[5] = q5.z; DJv[6] = q5.w;
+        DJv[7] = q6.x; DJv[8] = q6.y;
+    }
+}
+
+// S_k = sum_l dJ[l,k] G_l (transposed) or sum_l dJ[k,l] G_l (classic).
+template <bool TRANSPOSED>
+__device__ __forceinline__ void stretch_of(const double *dJ, double gx, double gy, double gz,
+                                           double &S0, double &S1, double &S2)
+{
+    if constexpr (TRANSPOSED) {
+        S0 = dJ[0] * gx + dJ[1] * gy + dJ[2] * gz;
+        S1 = dJ[3] * gx + dJ[4] * gy + dJ[5] * gz;
+        S2 = dJ[6] * gx + dJ[7] * gy + dJ[8] * gz;
+    } else {
+        S0 = dJ[0] * gx + dJ[3] * gy + dJ[6] * gz;
+        S1 = dJ[1] * gx + dJ[4] * gy + dJ[7] * gz;
+        S2 = dJ[2] * gx + dJ[5] * gy + dJ[8] * gz;
+    }
+}
+
+// Estr of every particle; with DYN also its core-size derivative
+//   D[E]_i = sum_j zD_j (J_i - J_j)-stretching + zeta_j (DJ_i - DJ_j)-stretching
+// into DE (zeta_zD_f64).
+template <int K, bool TRANSPOSED, bool DYN>
 __global__ __launch_bounds__(UJ_BLOCK) void estr_f64_kernel(const double4 *__restrict__ src,
                                                             int64_t n, int64_t cap,
-                                                            double *__restrict__ E)
+                                                            double *__restrict__ E,
+                                                            double *__restrict__ DE)
 {
-    __shared__ double4 tile[5 * UJ_BLOCK];
+    constexpr int NR = DYN ? 7 : 5;
+    __shared__ double4 tile[NR * UJ_BLOCK];
     const int t = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * UJ_BLOCK + t;
     const bool active = i < n;
-    double xi = 0, yi = 0, zi = 0, Ji[9];
-    for (int k = 0; k < 9; ++k) Ji[k] = 0.0;
+    double xi = 0, yi = 0, zi = 0, Ji[9], DJi[9];
+    for (int k = 0; k < 9; ++k) Ji[k] = DJi[k] = 0.0;
     if (active) {
-        const double4 p = src[5 * i];
+        const double4 p = src[NR * i];
         xi = p.x; yi = p.y; zi = p.z;
-        const double4 q2 = src[5 * i + 2], q3 = src[5 * i + 3], q4 = src[5 * i + 4];
-        Ji[0] = q2.x; Ji[1] = q2.y; Ji[2] = q2.z; Ji[3] = q2.w;
-        Ji[4] = q3.x; Ji[5] = q3.y; Ji[6] = q3.z; Ji[7] = q3.w; Ji[8] = q4.x;
+        estr_rec_j<DYN>(src + NR * i, Ji, DJi);
     }
-    double e0 = 0, e1 = 0, e2 = 0;
+    double e0 = 0, e1 = 0, e2 = 0, f0 = 0, f1 = 0, f2 = 0;
     for (int64_t base = 0; base < n; base += UJ_BLOCK) {
         __syncthreads();
         const int64_t js = base + t;
         if (js < n) {
-            for (int k = 0; k < 5; ++k) tile[5 * t + k] = src[5 * js + k];
+            for (int k = 0; k < NR; ++k) tile[NR * t + k] = src[NR * js + k];
         }
         __syncthreads();
         const int cnt = (int)((n - base) < UJ_BLOCK ? (n - base) : UJ_BLOCK);
         for (int jj = 0; jj < cnt; ++jj) {
-            const double4 a = tile[5 * jj], gm = tile[5 * jj + 1];
-            const double4 q2 = tile[5 * jj + 2], q3 = tile[5 * jj + 3], q4 = tile[5 * jj + 4];
+            const double4 a = tile[NR * jj], gm = tile[NR * jj + 1];
+            double Jj[9], DJj[9];
+            estr_rec_j<DYN>(tile + NR * jj, Jj, DJj);
             const double d1 = xi - a.x, d2 = yi - a.y, d3 = zi - a.z;
             const double r = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
-            const double zs = zeta_f64<K>(r * a.w) * (a.w * a.w * a.w);
-            double dJ[9];
-            dJ[0] = Ji[0] - q2.x; dJ[1] = Ji[1] - q2.y; dJ[2] = Ji[2] - q2.z;
-            dJ[3] = Ji[3] - q2.w; dJ[4] = Ji[4] - q3.x; dJ[5] = Ji[5] - q3.y;
-            dJ[6] = Ji[6] - q3.z; dJ[7] = Ji[7] - q3.w; dJ[8] = Ji[8] - q4.x;
-            double S0, S1, S2;
-            if constexpr (TRANSPOSED) {   // S_k = sum_l dJ[l,k] G_l
-                S0 = dJ[0] * gm.x + dJ[1] * gm.y + dJ[2] * gm.z;
-                S1 = dJ[3] * gm.x + dJ[4] * gm.y + dJ[5] * gm.z;
-                S2 = dJ[6] * gm.x + dJ[7] * gm.y + dJ[8] * gm.z;
-            } else {                      // S_k = sum_l dJ[k,l] G_l
-                S0 = dJ[0] * gm.x + dJ[3] * gm.y + dJ[6] * gm.z;
-                S1 = dJ[1] * gm.x + dJ[4] * gm.y + dJ[7] * gm.z;
-                S2 = dJ[2] * gm.x + dJ[5] * gm.y + dJ[8] * gm.z;
+            const double is3 = a.w * a.w * a.w;
+            double dJ[9], S0, S1, S2;
+            for (int k = 0; k < 9; ++k) dJ[k] = Ji[k] - Jj[k];
+            stretch_of<TRANSPOSED>(dJ, gm.x, gm.y, gm.z, S0, S1, S2);
+            if constexpr (DYN) {
+                double z, zD;
+                zeta_zD_f64<K>(r * a.w, z, zD);
+                const double zs = z * is3, zDs = zD * is3;
+                double T0, T1, T2;
+                for (int k = 0; k < 9; ++k) dJ[k] = DJi[k] - DJj[k];
+                stretch_of<TRANSPOSED>(dJ, gm.x, gm.y, gm.z, T0, T1, T2);
+                e0 += zs * S0; e1 += zs * S1; e2 += zs * S2;
+                f0 += zDs * S0 + zs * T0;
+                f1 += zDs * S1 + zs * T1;
+                f2 += zDs * S2 + zs * T2;
+            } else {
+                const double zs = zeta_f64<K>(r * a.w) * is3;
+                e0 += zs * S0; e1 += zs * S1; e2 += zs * S2;
             }
-            e0 += zs * S0;
-            e1 += zs * S1;
-            e2 += zs * S2;
         }
     }
     if (!active) return;
     E[i] = e0;
     E[cap + i] = e1;
     E[2 * cap + i] = e2;
+    if constexpr (DYN) {
+        DE[i] = f0;
+        DE[cap + i] = f1;
+        DE[2 * cap + i] = f2;
+    }
 }
 
 hipError_t launch_estr_f64(hipStream_t st, int kernel, int transposed, const DeviceField &f,
-                           Scratch &s)
+                           Scratch &s, const double *DJ, double *DE)
 {
     const int64_t n = f.np;
     if (n == 0) return hipSuccess;
-    pack_estr_kernel<<<grid_for(n, EW_BLOCK), EW_BLOCK, 0, st>>>(f.X, f.G, f.sigma, f.J, n, f.cap,
-                                                                 s.srcE);
+    const bool dyn = DJ != nullptr;
+    if (dyn)
+        pack_estr_kernel<true><<<grid_for(n, EW_BLOCK), EW_BLOCK, 0, st>>>(f.X, f.G, f.sigma, f.J,
+                                                                           DJ, n, f.cap, s.srcE);
+    else
+        pack_estr_kernel<false><<<grid_for(n, EW_BLOCK), EW_BLOCK, 0, st>>>(f.X, f.G, f.sigma, f.J,
+                                                                            DJ, n, f.cap, s.srcE);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const unsigned g = grid_for(n, UJ_BLOCK);
 #define VPMX_ESTR(KK)                                                                        \
-    if (transposed)                                                                          \
-        estr_f64_kernel<KK, true><<<g, UJ_BLOCK, 0, st>>>(s.srcE, n, f.cap, f.E);            \
+    if (dyn && transposed)                                                                   \
+        estr_f64_kernel<KK, true, true><<<g, UJ_BLOCK, 0, st>>>(s.srcE, n, f.cap, f.E, DE);  \
+    else if (dyn)                                                                            \
+        estr_f64_kernel<KK, false, true><<<g, UJ_BLOCK, 0, st>>>(s.srcE, n, f.cap, f.E, DE); \
+    else if (transposed)                                                                     \
+        estr_f64_kernel<KK, true, false><<<g, UJ_BLOCK, 0, st>>>(s.srcE, n, f.cap, f.E, DE); \
     else                                                                                     \
-        estr_f64_kernel<KK, false><<<g, UJ_BLOCK, 0, st>>>(s.srcE, n, f.cap, f.E);
+        estr_f64_kernel<KK, false, false><<<g, UJ_BLOCK, 0, st>>>(s.srcE, n, f.cap, f.E, DE);
     switch (kernel) {
     case GAUSSIAN: VPMX_ESTR(GAUSSIAN); break;
     case GAUSSIANERF: VPMX_ESTR(GAUSSIANERF); break;
@@ -717,80 +812,33 @@ __global__ __launch_bounds__(EW_BLOCK) void sfs_coeff_kernel(int64_t n, int64_t 
 }
 
 // ---- dynamic SFS procedure (docs/src/theory/rvpm.md:264-296) ---------------
-// C_d = <Gamma . L> / <Gamma . m> with the sigma-derivatives of the resolved
-// stretching (L) and of the SFS model (m) taken as differences between a test
-// filter alpha*sigma and the domain filter sigma; <.> is a per-particle
-// Lagrangian average  a <- w a_new + (1 - w) a_old  (first sample taken as is).
-__global__ __launch_bounds__(EW_BLOCK) void sigma_scale_kernel(int64_t n, double *__restrict__ sg,
-                                                               double *__restrict__ sig0,
-                                                               double alpha)
-{
-    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const double s = sg[i];
-    sig0[i] = s;
-    sg[i] = alpha * s;
-}
-
-__global__ __launch_bounds__(EW_BLOCK) void sigma_restore_kernel(int64_t n, double *__restrict__ sg,
-                                                                 const double *__restrict__ sig0)
-{
-    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
-    if (i < n) sg[i] = sig0[i];
-}
-
-__device__ __forceinline__ void stretch3(const double *__restrict__ J, int64_t cap, int64_t i,
-                                         double g0, double g1, double g2, int transposed,
-                                         double &s0, double &s1, double &s2)
-{
-    double Jm[9];
-    for (int k = 0; k < 9; ++k) Jm[k] = J[k * cap + i];
-    if (transposed) {
-        s0 = Jm[0] * g0 + Jm[1] * g1 + Jm[2] * g2;
-        s1 = Jm[3] * g0 + Jm[4] * g1 + Jm[5] * g2;
-        s2 = Jm[6] * g0 + Jm[7] * g1 + Jm[8] * g2;
-    } else {
-        s0 = Jm[0] * g0 + Jm[3] * g1 + Jm[6] * g2;
-        s1 = Jm[1] * g0 + Jm[4] * g1 + Jm[7] * g2;
-        s2 = Jm[2] * g0 + Jm[5] * g1 + Jm[8] * g2;
-    }
-}
-
-__global__ __launch_bounds__(EW_BLOCK) void dyn_stash_kernel(int64_t n, int64_t cap,
-                                                             const double *__restrict__ G,
-                                                             const double *__restrict__ J,
-                                                             const double *__restrict__ E,
-                                                             double *__restrict__ D, int transposed)
-{
-    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    double s0, s1, s2;
-    stretch3(J, cap, i, G[i], G[cap + i], G[2 * cap + i], transposed, s0, s1, s2);
-    D[i] = s0;
-    D[cap + i] = s1;
-    D[2 * cap + i] = s2;
-    D[3 * cap + i] = E[i];
-    D[4 * cap + i] = E[cap + i];
-    D[5 * cap + i] = E[2 * cap + i];
-}
-
+// C_d = <Gamma . L> / <Gamma . m> per particle with the exact core-size
+// derivatives DJ = D[J] and DE = D[E] (D = sigma d/dsigma of every core):
+//   num = (3 alpha - 2)/sigma  Gamma^T DJ Gamma
+//       (L = (3/sigma)(Gamma.grad)(u - ubar) + (Gamma.grad) dubar/dsigma with the
+//        unresolved u - ubar taken at the test-filter level alpha sigma to first
+//        order in 1 - alpha: u - ubar ~ (alpha - 1) D[ubar]),
+//   den = sigma^2/zeta0  Gamma . DE      (m = sigma^3/zeta0 dE/dsigma),
+// each averaged along the particle's path as a <- w a_new + (1 - w) a_old (the
+// first sample taken as is), C = <num>/<den> bounded to [minC, maxC], then
+// clipped to 0 where C Gamma.E < 0.  Static particles keep their C.
 __global__ __launch_bounds__(EW_BLOCK) void dyn_coeff_kernel(
     int64_t n, int64_t cap, const double *__restrict__ G, const double *__restrict__ sg,
-    const double *__restrict__ J, const double *__restrict__ E, const double *__restrict__ D,
-    double *__restrict__ C, const uint8_t *__restrict__ st, int transposed, double zeta0,
-    double w, double minC, double maxC, int clip)
+    const double *__restrict__ E, const double *__restrict__ DJ, const double *__restrict__ DE,
+    double *__restrict__ C, const uint8_t *__restrict__ st, double alpha, double zeta0, double w,
+    double minC, double maxC, int clip)
 {
     const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) return;
     if (st && st[i]) return;
-    const double g0 = G[i], g1 = G[cap + i], g2 = G[2 * cap + i];
-    double s0, s1, s2;
-    stretch3(J, cap, i, g0, g1, g2, transposed, s0, s1, s2);
-    const double e0 = E[i], e1 = E[cap + i], e2 = E[2 * cap + i];
-    const double nume = g0 * (D[i] - s0) + g1 * (D[cap + i] - s1) + g2 * (D[2 * cap + i] - s2);
+    const double g[3] = {G[i], G[cap + i], G[2 * cap + i]};
+    double q = 0.0;
+    for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) q += g[k] * DJ[(k + 3 * l) * cap + i] * g[l];
     const double sig = sg[i];
-    const double deno = (g0 * (D[3 * cap + i] - e0) + g1 * (D[4 * cap + i] - e1) +
-                         g2 * (D[5 * cap + i] - e2)) * (sig * sig * sig / zeta0);
+    const double nume = (3.0 * alpha - 2.0) / sig * q;
+    const double deno = sig * sig / zeta0 *
+                        (g[0] * DE[i] + g[1] * DE[cap + i] + g[2] * DE[2 * cap + i]);
     const double n_old = C[cap + i], d_old = C[2 * cap + i];
     double na = nume, da = deno;
     if (n_old != 0.0 || d_old != 0.0) {
@@ -801,39 +849,18 @@ __global__ __launch_bounds__(EW_BLOCK) void dyn_coeff_kernel(
     C[2 * cap + i] = da;
     double c = da != 0.0 ? na / da : 0.0;
     c = fmin(fmax(c, minC), maxC);
-    if (clip && c * (g0 * e0 + g1 * e1 + g2 * e2) < 0.0) c = 0.0;
+    const double ge = g[0] * E[i] + g[1] * E[cap + i] + g[2] * E[2 * cap + i];
+    if (clip && c * ge < 0.0) c = 0.0;
     C[i] = c;
 }
 
-hipError_t launch_sigma_scale(hipStream_t st, const DeviceField &f, double alpha)
-{
-    if (f.np == 0) return hipSuccess;
-    sigma_scale_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(f.np, f.sigma, f.sig0, alpha);
-    return hipGetLastError();
-}
-
-hipError_t launch_sigma_restore(hipStream_t st, const DeviceField &f)
-{
-    if (f.np == 0) return hipSuccess;
-    sigma_restore_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(f.np, f.sigma, f.sig0);
-    return hipGetLastError();
-}
-
-hipError_t launch_dyn_stash(hipStream_t st, const DeviceField &f, int transposed)
-{
-    if (f.np == 0) return hipSuccess;
-    dyn_stash_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(f.np, f.cap, f.G, f.J, f.E, f.D,
-                                                                    transposed);
-    return hipGetLastError();
-}
-
-hipError_t launch_dyn_coeff(hipStream_t st, const DeviceField &f, int transposed, double zeta0,
+hipError_t launch_dyn_coeff(hipStream_t st, const DeviceField &f, double alpha, double zeta0,
                             double rlxf, double minC, double maxC, int clip)
 {
     if (f.np == 0) return hipSuccess;
     dyn_coeff_kernel<<<grid_for(f.np, EW_BLOCK), EW_BLOCK, 0, st>>>(
-        f.np, f.cap, f.G, f.sigma, f.J, f.E, f.D, f.C, f.any_static ? f.is_static : nullptr,
-        transposed, zeta0, rlxf, minC, maxC, clip);
+        f.np, f.cap, f.G, f.sigma, f.E, f.DJ(), f.DE(), f.C,
+        f.any_static ? f.is_static : nullptr, alpha, zeta0, rlxf, minC, maxC, clip);
     return hipGetLastError();
 }
 

@@ -15,7 +15,8 @@ namespace vpmx {
 // pack_f64_kernel.  Targets [t0, t1) read X (SoA, stride cap); sources
 // [s0, s1).  Reference pair algebra: src/FLOWUnsteady_processing_force.jl:888-905
 // extended to J = grad U.  `tile` holds 2*BLK records.  d = (U0..2, J0..8).
-template <int K, int BLK>
+// DS: the core-size derivatives D[U], D[J] instead (g, g' -> h, h').
+template <int K, int BLK, bool DS = false>
 __device__ __forceinline__ void uj_f64_sums(const double4 *__restrict__ src, int64_t s0,
                                             int64_t s1, const double *__restrict__ X, int64_t t0,
                                             int64_t t1, int64_t cap, double4 *tile, double d[12])
@@ -48,7 +49,10 @@ __device__ __forceinline__ void uj_f64_sums(const double4 *__restrict__ src, int
             if (r2 > 0.0) {   // the reference skips r == 0
                 const double r = sqrt(r2);
                 double g, dg;
-                g_dg_f64<K>(r * b.w, g, dg);
+                if constexpr (DS)
+                    h_dh_f64<K>(r * b.w, g, dg);
+                else
+                    g_dg_f64<K>(r * b.w, g, dg);
                 const double rinv = 1.0 / r;
                 const double r3i = rinv * rinv * rinv;
                 const double G1 = a.w, G2 = b.x, G3 = b.y;
@@ -86,7 +90,7 @@ __device__ __forceinline__ void uj_f64_store(const double d[12], int64_t i, int6
     for (int k = 0; k < 9; ++k) J[k * ldo + i] = reset ? d[3 + k] : J[k * ldo + i] + d[3 + k];
 }
 
-template <int K, int BLK>
+template <int K, int BLK, bool DS = false>
 __device__ __forceinline__ void uj_f64_block(const double4 *__restrict__ src, int64_t s0,
                                              int64_t s1, const double *__restrict__ X,
                                              int64_t t0, int64_t t1, int64_t cap,
@@ -94,7 +98,7 @@ __device__ __forceinline__ void uj_f64_block(const double4 *__restrict__ src, in
                                              int reset, double4 *tile)
 {
     double d[12];
-    uj_f64_sums<K, BLK>(src, s0, s1, X, t0, t1, cap, tile, d);
+    uj_f64_sums<K, BLK, DS>(src, s0, s1, X, t0, t1, cap, tile, d);
     const int64_t i = t0 + threadIdx.x;
     if (i < t1) uj_f64_store(d, i, cap, U, J, reset);
 }
@@ -112,7 +116,11 @@ __device__ __forceinline__ void uj_f64_block(const double4 *__restrict__ src, in
 // runs mixed_pair_far without per-pair tests, the same arithmetic the per-pair
 // path takes for such pairs.  t0 - torg and s0 - torg must then be multiples
 // of BLK.
-template <int K, int BLK>
+//
+// DS: the core-size derivatives D[U], D[J] in the same 14 sums
+// (mixed_pair_dsig); far tiles are skipped, their pairs being beyond the
+// cores (e^{-24.5} of the core value for the gaussian kernels).
+template <int K, int BLK, bool DS = false>
 __device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, int64_t t0,
                                               int64_t t1, const double4 *__restrict__ spos,
                                               const float4 *__restrict__ sgam, int64_t s0,
@@ -160,7 +168,18 @@ __device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, i
             }
         }
         MixedAcc acc;
-        if (far) {
+        if constexpr (DS) {
+            if (!far) {
+                if (cnt == BLK) {
+#pragma unroll 4
+                    for (int jj = 0; jj < BLK; ++jj)
+                        mixed_pair_dsig<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
+                } else {
+                    for (int jj = 0; jj < cnt; ++jj)
+                        mixed_pair_dsig<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
+                }
+            }
+        } else if (far) {
 #pragma unroll 4
             for (int jj = 0; jj < BLK; ++jj)
                 mixed_pair_far<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
@@ -203,7 +222,7 @@ __device__ __forceinline__ void uj_mixed_store(const double d[14], int64_t i, in
 }
 
 // One block: sums over [s0, s1) and the store through perm (stride ldo).
-template <int K, int BLK>
+template <int K, int BLK, bool DS = false>
 __device__ __forceinline__ void uj_mixed_block(const double4 *__restrict__ tgt, int64_t t0,
                                                int64_t t1, const int32_t *__restrict__ perm,
                                                const double4 *__restrict__ spos,
@@ -215,7 +234,7 @@ __device__ __forceinline__ void uj_mixed_block(const double4 *__restrict__ tgt, 
                                                int64_t torg = 0)
 {
     double d[14];
-    uj_mixed_sums<K, BLK>(tgt, t0, t1, spos, sgam, s0, s1, tb, torg, tpos, tgam, d);
+    uj_mixed_sums<K, BLK, DS>(tgt, t0, t1, spos, sgam, s0, s1, tb, torg, tpos, tgam, d);
     const int64_t s = t0 + threadIdx.x;
     if (s < t1) uj_mixed_store(d, perm[s], ldo, U, J, reset);
 }

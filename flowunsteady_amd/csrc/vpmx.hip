@@ -134,8 +134,7 @@ int dalloc(vpmx_ctx *c, T **p, size_t count)
 
 void free_field(DeviceField &f)
 {
-    for (double **p : {&f.X, &f.G, &f.sigma, &f.U, &f.J, &f.E, &f.C, &f.M, &f.D, &f.sig0, &f.vol,
-                       &f.V}) {
+    for (double **p : {&f.X, &f.G, &f.sigma, &f.U, &f.J, &f.E, &f.C, &f.M, &f.D, &f.vol, &f.V}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
@@ -184,14 +183,13 @@ int ensure_capacity(vpmx_ctx *c, int64_t np)
     VPMX_TRY(dalloc(c, &f.J, 9 * cap));
     VPMX_TRY(dalloc(c, &f.E, 3 * cap));
     VPMX_TRY(dalloc(c, &f.C, 3 * cap));
-    VPMX_TRY(dalloc(c, &f.D, 6 * cap));
-    VPMX_TRY(dalloc(c, &f.sig0, cap));
+    VPMX_TRY(dalloc(c, &f.D, 15 * cap));
     VPMX_TRY(dalloc(c, &f.M, 7 * cap));
     VPMX_TRY(dalloc(c, &f.vol, cap));
     VPMX_TRY(dalloc(c, &f.is_static, cap));
     f.cap = cap;
     VPMX_TRY(dalloc(c, &s.src64, 2 * cap));
-    VPMX_TRY(dalloc(c, &s.srcE, 5 * cap));
+    VPMX_TRY(dalloc(c, &s.srcE, 7 * cap));
     VPMX_TRY(dalloc(c, &s.spos, cap));
     VPMX_TRY(dalloc(c, &s.sgam, cap));
     VPMX_TRY(dalloc(c, &s.tbnd, 2 * ((cap + 255) / 256)));
@@ -951,29 +949,40 @@ int enqueue_uj(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool sort = true
 }
 
 // Estr with the scheme of the UJ that just ran (direct all-pairs, or the FMM's
-// near-field lists).
-int enqueue_estr(vpmx_ctx *c, const vpmx_uj_params *U)
+// near-field lists); dyn: also its core-size derivative into f.DE() from f.DJ().
+int enqueue_estr(vpmx_ctx *c, const vpmx_uj_params *U, bool dyn = false)
 {
     TimedScope ts(c, T_ESTR);
+    DeviceField &f = c->f;
+    const double *DJ = dyn ? f.DJ() : nullptr;
+    double *DE = dyn ? f.DE() : nullptr;
     if (U->fmm) {
         if (!c->fmm) return fail(c, VPMX_EINVAL, "Estr with the FMM needs an FMM UJ first");
-        VPMX_HIP(c, launch_fmm_estr(c->stream, *c->fmm, U->kernel, U->transposed, U->precision,
-                                    c->f, c->s));
+        VPMX_HIP(c, launch_fmm_estr(c->stream, *c->fmm, U->kernel, U->transposed, U->precision, f,
+                                    c->s, DJ, DE));
     } else {
-        VPMX_HIP(c, launch_estr_f64(c->stream, U->kernel, U->transposed, c->f, c->s));
+        VPMX_HIP(c, launch_estr_f64(c->stream, U->kernel, U->transposed, f, c->s, DJ, DE));
     }
     return VPMX_OK;
 }
 
-// Dynamic procedure, before the stage's UJ: UJ and Estr with the test filter
-// alpha*sigma, stretching and Estr stashed in D, sigma restored.
-int enqueue_dynamic_test_filter(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S)
+// Dynamic procedure: core-size derivatives D[U], D[J] of the field, with the
+// scheme and records of the stage's UJ (which must have just run).  The FMM's
+// far field uses the singular kernel and does not depend on sigma, so with
+// the FMM they are near-field sums over its P2P lists.
+int enqueue_dsig(vpmx_ctx *c, const vpmx_uj_params *U)
 {
-    VPMX_HIP(c, launch_sigma_scale(c->stream, c->f, S->alpha));
-    VPMX_TRY(enqueue_uj(c, U, 1, false));
-    VPMX_TRY(enqueue_estr(c, U));
-    VPMX_HIP(c, launch_dyn_stash(c->stream, c->f, U->transposed));
-    VPMX_HIP(c, launch_sigma_restore(c->stream, c->f));
+    TimedScope ts(c, T_DSIG);
+    DeviceField &f = c->f;
+    if (U->fmm) {
+        if (!c->fmm) return fail(c, VPMX_EINVAL, "the dynamic SFS with the FMM needs an FMM UJ first");
+        VPMX_HIP(c, launch_fmm_dsig(c->stream, *c->fmm, U->kernel, U->precision, f, c->s, f.DU(),
+                                    f.DJ()));
+    } else if (U->precision == VPMX_FP64) {
+        VPMX_HIP(c, launch_dsig_f64(c->stream, U->kernel, f, c->s, f.DU(), f.DJ()));
+    } else {
+        VPMX_HIP(c, launch_dsig_mixed(c->stream, U->kernel, f, c->s, f.DU(), f.DJ()));
+    }
     return VPMX_OK;
 }
 
@@ -996,15 +1005,15 @@ int enqueue_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S
     const int32_t *fid = c->b.nfields ? c->b.fid : nullptr;
     if (rk3) VPMX_HIP(c, launch_fill(c->stream, f.M, 7 * f.cap, 0.0));
     for (int s = 0; s < nst; ++s) {
-        if (S->sfs == VPMX_SFS_DYNAMIC) VPMX_TRY(enqueue_dynamic_test_filter(c, U, S));
         VPMX_TRY(enqueue_uj(c, U, 1, false));
-        if (sfs) {
+        if (S->sfs == VPMX_SFS_DYNAMIC) {
+            VPMX_TRY(enqueue_dsig(c, U));
+            VPMX_TRY(enqueue_estr(c, U, true));
+            VPMX_HIP(c, launch_dyn_coeff(c->stream, f, S->alpha, zeta0, S->sfs_rlxf, S->minC,
+                                         S->maxC, S->clip_backscatter));
+        } else if (sfs) {
             VPMX_TRY(enqueue_estr(c, U));
-            if (S->sfs == VPMX_SFS_DYNAMIC)
-                VPMX_HIP(c, launch_dyn_coeff(c->stream, f, U->transposed, zeta0, S->sfs_rlxf,
-                                             S->minC, S->maxC, S->clip_backscatter));
-            else
-                VPMX_HIP(c, launch_sfs_coeff(c->stream, f, S->Cs, S->clip_backscatter));
+            VPMX_HIP(c, launch_sfs_coeff(c->stream, f, S->Cs, S->clip_backscatter));
         }
         if (S->viscous == VPMX_VISCOUS_PSE)
             VPMX_HIP(c, launch_pse(c->stream, U->kernel, f, S->nu, c->s.vrec));

@@ -144,9 +144,15 @@ def DynamicSFS(alpha: float = 0.999, rlxf: float = 0.005, minC: float = 0.0, max
                clip_backscatter: bool = True) -> SubFilterScale:
     """Dynamic model coefficient (vpm.DynamicSFS(Estr, pseudo3level_positive;
     alpha, maxC, clippings=[clipping_backscatter]), examples/rotorhover/rotorhover.jl:53-55):
-    C = <Gamma . (S(alpha sigma) - S(sigma))> / <Gamma . (E(alpha sigma) - E(sigma)) sigma^3/zeta0>
-    with a per-particle Lagrangian average, bounded to [minC, maxC] (minC = 0:
-    positive), then backscatter clipping (docs/src/theory/rvpm.md:264-296)."""
+    C = <Gamma . L> / <Gamma . m> (docs/src/theory/rvpm.md:264-296) with exact
+    core-size derivatives D = sigma d/dsigma,
+      Gamma . L = (3 alpha - 2)/sigma Gamma^T D[J] Gamma
+        (L's unresolved u - ubar taken at the test filter alpha sigma, to first
+         order in 1 - alpha: u - ubar ~ (alpha - 1) D[ubar]),
+      Gamma . m = sigma^2/zeta0 Gamma . D[E],
+    each with a per-particle Lagrangian average (weight rlxf of the newest
+    sample), C bounded to [minC, maxC] (minC = 0: positive), then backscatter
+    clipping."""
     return SubFilterScale("DynamicSFS", N.SFS_DYNAMIC, 0.0, bool(clip_backscatter),
                           float(alpha), float(rlxf), float(minC), float(maxC))
 
@@ -437,24 +443,72 @@ def UJ_fmm_gpu(pfield: ParticleField, *, reset: bool = True, **kwargs):
 UJ_fmm = UJ_fmm_gpu
 
 
-def Vvpm_on_Xs(pfield: ParticleField, Xs) -> np.ndarray:
-    """Velocity induced by the particle field at points Xs (m, 3), as
-    FLOWUnsteady's Vvpm_on_Xs (src/FLOWUnsteady_simulation.jl:494-570) returns
-    it, evaluated without appending probes to the field (the probes carry no
-    circulation, so only the m x N target-source pairs are computed)."""
+def mirror_particles(pfield: ParticleField, mirror_X, mirror_normal):
+    """Append the mirror image of every particle as a static particle, the
+    method of images of Vvpm_on_Xs (src/FLOWUnsteady_simulation.jl:518-532):
+    X_m = X - 2 ((X - X_0) . n) n and, as the reference writes it,
+    Gamma_m = 2 (Gamma . n) Gamma / |Gamma| - Gamma, keeping sigma, vol,
+    circulation and C.  A particle with Gamma = 0 gets Gamma_m = 0 (the
+    reference's formula divides by |Gamma| there)."""
+    n = pfield.np
+    X0 = np.asarray(mirror_X, float).reshape(3)
+    nv = np.asarray(mirror_normal, float).reshape(3)
+    P = pfield.particles
+    X = P[X_INDEX, :n].T.copy()
+    G = P[GAMMA_INDEX, :n].T.copy()
+    Xm = X - 2.0 * ((X - X0) @ nv)[:, None] * nv[None, :]
+    gn = np.linalg.norm(G, axis=1)
+    safe = np.where(gn > 0.0, gn, 1.0)
+    Gm = np.where(gn[:, None] > 0.0, 2.0 * (G @ nv)[:, None] * G / safe[:, None] - G, 0.0)
+    for i in range(n):
+        add_particle(pfield, Xm[i], Gm[i], P[SIGMA_INDEX, i], vol=P[VOL_INDEX, i],
+                     circulation=P[CIRCULATION_INDEX, i], C=P[C_INDEX, i].copy(), static=True)
+
+
+def Vvpm_on_Xs(pfield: ParticleField, Xs, *, static_particles_fun=None, dt: float = 0.0,
+               fsgm: float = 1.0, mirror: bool = False, mirror_X=None,
+               mirror_normal=None) -> np.ndarray:
+    """Velocity induced by the particle field at points Xs (m, 3), FLOWUnsteady's
+    Vvpm_on_Xs (src/FLOWUnsteady_simulation.jl:494-570) with its arguments:
+    static particles from static_particles_fun(pfield, t, dt) and, with mirror,
+    mirror images (mirror_particles) are appended for the evaluation and
+    removed afterwards; fsgm is applied and undone as the reference does it
+    (one matrix entry -- the first particle's sigma -- scaled once per static
+    particle already in the field).  The probes are not appended: the library
+    evaluates U at them from the field (vpmx_uj_targets), so only the m x N
+    target-source pairs are computed; the probes carry no circulation, so they
+    add nothing to each other.  The evaluation is direct also for an FMM
+    field."""
     Xs = np.ascontiguousarray(Xs, dtype=np.float64).reshape(-1, 3)
     m = Xs.shape[0]
     U = np.zeros((m, 3))
     J = np.zeros((m, 9))
     if m == 0:
         return U
-    ctx = _context(pfield.device)
-    L = N.lib()
-    ctx.check(L.vpmx_upload(ctx.ptr, pfield._ptr(), pfield.np, ctypes.byref(pfield.layout)))
-    P = pfield._uj_params(fmm=False)
-    dp = ctypes.POINTER(ctypes.c_double)
-    ctx.check(L.vpmx_uj_targets(ctx.ptr, Xs.ctypes.data_as(dp), m, U.ctypes.data_as(dp),
-                                J.ctypes.data_as(dp), ctypes.byref(P)))
+    org_np = pfield.np
+    nscaled = 0
+    if abs(fsgm) != 1:
+        for i in range(pfield.np):
+            if pfield.particles[STATIC_INDEX, i] > 0:
+                pfield.particles[SIGMA_INDEX, 0] *= fsgm
+                nscaled += 1
+    try:
+        if static_particles_fun is not None:
+            static_particles_fun(pfield, pfield.t, dt)
+        if mirror:
+            mirror_particles(pfield, mirror_X, mirror_normal)
+        ctx = _context(pfield.device)
+        L = N.lib()
+        ctx.check(L.vpmx_upload(ctx.ptr, pfield._ptr(), pfield.np, ctypes.byref(pfield.layout)))
+        P = pfield._uj_params(fmm=False)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ctx.check(L.vpmx_uj_targets(ctx.ptr, Xs.ctypes.data_as(dp), m, U.ctypes.data_as(dp),
+                                    J.ctypes.data_as(dp), ctypes.byref(P)))
+    finally:
+        for i in range(pfield.np - 1, org_np - 1, -1):
+            remove_particle(pfield, i)
+        for _ in range(nscaled):
+            pfield.particles[SIGMA_INDEX, 0] /= fsgm
     return U
 
 

@@ -69,8 +69,9 @@ extern "C" {
 /* SFS models (src/FLOWUnsteady_simulation.jl:39,130). */
 #define VPMX_SFS_NONE 0
 #define VPMX_SFS_CONSTANT 1   /* C = Cs with Estr, optional clipping          */
-#define VPMX_SFS_DYNAMIC 2    /* C from the dynamic procedure (test filter
-                                 alpha*sigma, Lagrangian average), clipped   */
+#define VPMX_SFS_DYNAMIC 2    /* C from the dynamic procedure (exact
+                                 core-size derivatives, test-filter level
+                                 alpha, Lagrangian average), clipped         */
 
 /* Viscous schemes: vpm.Inviscid, vpm.CoreSpreading, vpm.ParticleStrengthExchange
  * (src/FLOWUnsteady_simulation.jl:42). */
@@ -131,8 +132,14 @@ typedef struct {
     int32_t clip_backscatter; /* clip C to 0 where C Gamma.Estr < 0         */
     int32_t relax_kind;   /* VPMX_RELAX_*                                   */
     double rlxf;          /* relaxation factor                              */
-    /* dynamic SFS (sfs = VPMX_SFS_DYNAMIC): test filter width alpha*sigma,
-     * Lagrangian-average weight of the newest sample, bounds of C */
+    /* dynamic SFS (sfs = VPMX_SFS_DYNAMIC, docs/src/theory/rvpm.md:264-296):
+     * C = <Gamma.L>/<Gamma.m> with exact derivatives D = sigma d/dsigma of
+     * every core, Gamma.L = (3 alpha - 2)/sigma Gamma^T D[J] Gamma (the
+     * unresolved u - ubar taken at the test filter alpha sigma to first order
+     * in 1 - alpha) and Gamma.m = sigma^2/zeta(0) Gamma.D[E]; sfs_rlxf is the
+     * Lagrangian-average weight of the newest sample, [minC, maxC] the bounds
+     * of C.  With the FMM the derivatives are near-field sums (its far field
+     * uses the singular kernel, which has no core). */
     double alpha, sfs_rlxf, minC, maxC;
     /* viscous diffusion nu lap(omega) (vpm_viscous, src/FLOWUnsteady_simulation.jl:42,229-234):
      * VPMX_VISCOUS_PSE adds the particle-strength-exchange rate to dGamma/dt;
@@ -289,7 +296,8 @@ int vpmx_set_graph(vpmx_ctx *ctx, int32_t on);
 /* ---- instrumentation ----------------------------------------------------------- */
 /* Time every launch of the pairwise kernels with HIP events on the context's
  * stream.  which: 0 = UJ pair kernel, 1 = Estr pair kernel, 2 = fused update,
- * 3 = FMM device evaluation (P2M through P2P), 4 = graph replay of a whole step
+ * 3 = FMM device evaluation (P2M through P2P), 4 = graph replay of a whole step,
+ * 5 = the dynamic SFS's core-size derivatives D[U], D[J]
  * (the kernels inside a graph are not timed one by one).
  * Returns the summed milliseconds and the launch count since the last reset. */
 int vpmx_set_timing(vpmx_ctx *ctx, int32_t on);

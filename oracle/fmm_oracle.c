@@ -661,6 +661,68 @@ int orc_uj_fmm(int kernel, int p, int ncrit, double theta, double sigcut, int64_
     return 0;
 }
 
+void orc_pair_dj(int kernel, const double *xt, const double *xs, const double *gs, double sgm,
+                 double *DU, double *DJ);
+void orc_pair_estr(int kernel, int transposed, const double *xt, const double *tJ,
+                   const double *tDJ, const double *xs, const double *gs, double sg,
+                   const double *sJ, const double *sDJ, double *e, double *de);
+
+/* Sums over the near-field (P2P) lists of the tree and traversal orc_uj_fmm
+ * builds for the same particles, ncrit, theta and sigcut -- the part of a
+ * sum that the FMM evaluates pair by pair.  Outputs are overwritten.
+ *   what = 1: core-size derivatives D[U] (out1, n x 3) and D[J] (out2, n x 9)
+ *             of orc_pair_dj.  Their far field is zero: the multipole part of
+ *             the FMM uses the singular kernel, which has no core.
+ *   what = 2: Estr (out1, n x 3) of orc_pair_estr from J (n x 9), and with DJ
+ *             (n x 9) also D[E] (out2, n x 3).  The SFS model sums zeta, which
+ *             the FMM leaves to its near field as well.
+ * Each target sums its list's source leaves in list order, each leaf's
+ * particles in sorted order. */
+int orc_fmm_near(int kernel, int transposed, int what, int ncrit, double theta, double sigcut,
+                 int64_t n, const double *X, const double *G, const double *sig, const double *J,
+                 const double *DJ, double *out1, double *out2)
+{
+    if (ncrit < 1 || (what != 1 && what != 2) || (what == 2 && !J)) return 1;
+    const int w1 = 3, w2 = what == 1 ? 9 : 3;
+    memset(out1, 0, sizeof(double) * w1 * (size_t)n);
+    if (what == 1 || DJ) memset(out2, 0, sizeof(double) * w2 * (size_t)n);
+    if (n == 0) return 0;
+    orc_tree T;
+    tree_build(&T, n, X, sig, ncrit);
+    pairlist m2l = {0}, p2p = {0};
+    traverse(&T, theta, sigcut, &m2l, &p2p);
+    const int64_t nc = T.ncell;
+    int64_t *p2p_off = csr_offsets(&p2p, nc);
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int64_t c = 0; c < nc; c++) {
+        const orc_cell *A = &T.cells[c];
+        if (A->nchild) continue;
+        for (int64_t jt = A->begin; jt < A->end; jt++) {
+            const int64_t i = T.perm[jt];
+            double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int64_t e = p2p_off[c]; e < p2p_off[c + 1]; e++) {
+                const orc_cell *B = &T.cells[p2p.a[2 * e + 1]];
+                for (int64_t js = B->begin; js < B->end; js++) {
+                    const int64_t s = T.perm[js];
+                    if (what == 1)
+                        orc_pair_dj(kernel, X + 3 * i, X + 3 * s, G + 3 * s, sig[s], a, b);
+                    else
+                        orc_pair_estr(kernel, transposed, X + 3 * i, J + 9 * i,
+                                      DJ ? DJ + 9 * i : NULL, X + 3 * s, G + 3 * s, sig[s],
+                                      J + 9 * s, DJ ? DJ + 9 * s : NULL, a, b);
+                }
+            }
+            for (int k = 0; k < w1; k++) out1[w1 * i + k] = a[k];
+            if (what == 1 || DJ)
+                for (int k = 0; k < w2; k++) out2[w2 * i + k] = b[k];
+        }
+    }
+    free(p2p_off);
+    free(m2l.a); free(p2p.a);
+    tree_free(&T);
+    return 0;
+}
+
 /* Expose the derivative recurrence for tests: b_k(r), |k| <= P, graded order. */
 int orc_fmm_deriv(int P, const double r[3], double *b, int *mi_out)
 {

@@ -40,7 +40,9 @@ class _Params(ctypes.Structure):
                 ("alpha", ctypes.c_double), ("sfs_rlxf", ctypes.c_double),
                 ("minC", ctypes.c_double), ("maxC", ctypes.c_double),
                 ("viscous", ctypes.c_int), ("nu", ctypes.c_double), ("sgm0", ctypes.c_double),
-                ("beta", ctypes.c_double), ("itmax", ctypes.c_int), ("tol", ctypes.c_double)]
+                ("beta", ctypes.c_double), ("itmax", ctypes.c_int), ("tol", ctypes.c_double),
+                ("fmm", ctypes.c_int), ("p", ctypes.c_int), ("ncrit", ctypes.c_int),
+                ("theta", ctypes.c_double), ("sigcut", ctypes.c_double)]
 
 
 def build(force: bool = False) -> str:
@@ -84,6 +86,18 @@ def lib():
         L.orc_energy.argtypes = [ctypes.c_int64, _dp, _dp, _dp]
         L.orc_energy.restype = ctypes.c_double
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_dzeta.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.orc_dzeta.restype = ctypes.c_double
+        L.orc_kernel_dsig.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_dj_direct.argtypes = L.orc_uj_direct.argtypes
+        L.orc_estr_dyn_direct.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _dp, _dp,
+                                          _dp, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp,
+                                          _dp]
+        L.orc_dyn_terms.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                    _dp, _dp, _dp]
+        L.orc_fmm_near.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int64, _dp, _dp,
+                                   _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -120,6 +134,70 @@ def uj_direct(kernel, tX, sX, sG, ssig):
     U = np.zeros((nt, 3)); J = np.zeros((nt, 9))
     lib().orc_uj_direct(_kid(kernel), nt, _p(tX), ns, _p(sX), _p(sG), _p(ssig), _p(U), _p(J))
     return U, J
+
+
+def dzeta(kernel, rho):
+    """zeta'(rho)."""
+    return lib().orc_dzeta(_kid(kernel), float(rho))
+
+
+def kernel_dsig(kernel, rho):
+    """(h, h', zD) at rho: sigma d/dsigma of g(r/sigma) and of zeta(r/sigma)/sigma^3 (times
+    sigma^3), with h' = dh/drho."""
+    h, dh, zD = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib().orc_kernel_dsig(_kid(kernel), float(rho), ctypes.byref(h), ctypes.byref(dh),
+                          ctypes.byref(zD))
+    return h.value, dh.value, zD.value
+
+
+def dj_direct(kernel, tX, sX, sG, ssig):
+    """Core-size derivatives D[U] (nt,3), D[J] (nt,9), D = sigma d/dsigma of every core."""
+    tX = _c(tX, (-1, 3)); sX = _c(sX, (-1, 3)); sG = _c(sG, (-1, 3)); ssig = _c(ssig, (-1,))
+    nt, ns = tX.shape[0], sX.shape[0]
+    DU = np.zeros((nt, 3)); DJ = np.zeros((nt, 9))
+    lib().orc_dj_direct(_kid(kernel), nt, _p(tX), ns, _p(sX), _p(sG), _p(ssig), _p(DU), _p(DJ))
+    return DU, DJ
+
+
+def estr_dyn_direct(kernel, transposed, tX, tJ, tDJ, sX, sG, ssig, sJ, sDJ):
+    """Estr (nt,3) and its core-size derivative D[E] (nt,3)."""
+    tX = _c(tX, (-1, 3)); tJ = _c(tJ, (-1, 9)); tDJ = _c(tDJ, (-1, 9))
+    sX = _c(sX, (-1, 3)); sG = _c(sG, (-1, 3)); ssig = _c(ssig, (-1,))
+    sJ = _c(sJ, (-1, 9)); sDJ = _c(sDJ, (-1, 9))
+    nt = tX.shape[0]
+    E = np.zeros((nt, 3)); DE = np.zeros((nt, 3))
+    lib().orc_estr_dyn_direct(_kid(kernel), int(transposed), nt, _p(tX), _p(tJ), _p(tDJ),
+                              sX.shape[0], _p(sX), _p(sG), _p(ssig), _p(sJ), _p(sDJ), _p(E),
+                              _p(DE))
+    return E, DE
+
+
+def dyn_terms(alpha, zeta0_, sigma, G, DJ, DE):
+    """(num, den) of the dynamic coefficient of one particle."""
+    G = _c(G, (3,)); DJ = _c(DJ, (9,)); DE = _c(DE, (3,))
+    num, den = ctypes.c_double(), ctypes.c_double()
+    lib().orc_dyn_terms(float(alpha), float(zeta0_), float(sigma), _p(G), _p(DJ), _p(DE),
+                        ctypes.byref(num), ctypes.byref(den))
+    return num.value, den.value
+
+
+def fmm_near(kernel, transposed, what, X, G, sigma, J=None, DJ=None, ncrit=50, theta=0.4,
+             sigcut=0.0):
+    """Sums over the near-field lists of the oracle FMM's tree: what = 1 gives
+    (D[U], D[J]); what = 2 gives (E, D[E]) from J (and DJ; D[E] is None without it)."""
+    X = _c(X, (-1, 3)); G = _c(G, (-1, 3)); sigma = _c(sigma, (-1,))
+    n = X.shape[0]
+    Ja = _c(J, (-1, 9)) if J is not None else None
+    DJa = _c(DJ, (-1, 9)) if DJ is not None else None
+    o1 = np.zeros((n, 3))
+    o2 = np.zeros((n, 9 if what == 1 else 3))
+    rc = lib().orc_fmm_near(_kid(kernel), int(transposed), int(what), int(ncrit), float(theta),
+                            float(sigcut), n, _p(X), _p(G), _p(sigma),
+                            _p(Ja) if Ja is not None else None,
+                            _p(DJa) if DJa is not None else None, _p(o1), _p(o2))
+    if rc != 0:
+        raise ValueError("orc_fmm_near: bad arguments")
+    return o1, (o2 if (what == 1 or DJ is not None) else None)
 
 
 def _viscous_ready(L):
@@ -275,9 +353,13 @@ class Field:
 def make_params(kernel="gaussianerf", transposed=True, formulation="rVPM", Uinf=(0, 0, 0),
                 sfs=False, Cs=0.0, clip_backscatter=True, relax_kind=0, rlxf=0.3,
                 alpha=0.999, sfs_rlxf=0.005, minC=0.0, maxC=1.0, viscous=0, nu=0.0,
-                sgm0=0.0, beta=1.5, itmax=20, tol=1e-3):
+                sgm0=0.0, beta=1.5, itmax=20, tol=1e-3, fmm=False, p=8, ncrit=50, theta=0.4,
+                sigcut=0.0):
     """sfs: False/0 none, True/1 constant Cs, "dynamic"/2 dynamic procedure.
-    viscous: 0 inviscid, 1 / "corespreading", 2 / "pse"."""
+    viscous: 0 inviscid, 1 / "corespreading", 2 / "pse".
+    fmm: UJ by the oracle's CPU FMM (p, ncrit, theta, acceptance gap sigcut in
+    core radii; 0 = purely geometric) with Estr and the core-size derivatives
+    over its near-field lists."""
     P = _Params()
     P.kernel = _kid(kernel)
     P.transposed = int(transposed)
@@ -296,6 +378,8 @@ def make_params(kernel="gaussianerf", transposed=True, formulation="rVPM", Uinf=
     P.viscous = {"corespreading": 1, "pse": 2}.get(viscous, viscous)
     P.nu, P.sgm0, P.beta = float(nu), float(sgm0), float(beta)
     P.itmax, P.tol = int(itmax), float(tol)
+    P.fmm, P.p, P.ncrit = int(bool(fmm)), int(p), int(ncrit)
+    P.theta, P.sigcut = float(theta), float(sigcut)
     return P
 
 

@@ -52,6 +52,8 @@
 #include <omp.h>
 #endif
 
+#include "orc.h"
+
 /* 1/(4 pi) exactly as the reference pair loop writes it
  * (src/FLOWUnsteady_processing_force.jl:900-902). */
 #define ORC_CONST4 0.07957747154594767
@@ -102,6 +104,39 @@ void orc_kernel(int kernel, double rho, double *zeta, double *g, double *dg)
     if (dg) *dg = d;
 }
 
+/* zeta'(rho), the derivative of the closed forms above:
+ *   gaussianerf  -rho zeta,   gaussian  -3 rho^2 zeta,
+ *   winckelmans  -7 rho / (rho^2 + 1) zeta,   singular  0. */
+double orc_dzeta(int kernel, double rho)
+{
+    double z;
+    orc_kernel(kernel, rho, &z, NULL, NULL);
+    switch (kernel) {
+    case ORC_GAUSSIAN: return -3.0 * rho * rho * z;
+    case ORC_GAUSSIANERF: return -rho * z;
+    case ORC_WINCKELMANS: return -7.0 * rho / (rho * rho + 1.0) * z;
+    default: return 0.0;
+    }
+}
+
+/* Derivatives with respect to the core size, every core scaled together
+ * (sigma -> lambda sigma, D = d/dlambda at lambda = 1 = sigma d/dsigma), as the
+ * dynamic SFS procedure needs them (docs/src/theory/rvpm.md:276-290):
+ *   D g(r/sigma)              = h(rho)  = -rho g'(rho) = -4 pi rho^3 zeta(rho)
+ *   d h / d rho               = h'(rho) = -4 pi (3 rho^2 zeta + rho^3 zeta')
+ *   D [zeta(r/sigma)/sigma^3] = zD(rho) / sigma^3,  zD = -(3 zeta + rho zeta').
+ * U and J carry g only through g(r/sigma)/r^3, so D[U] and D[J] are the pair
+ * algebra of orc_pair_uj with (g, g') replaced by (h, h'). */
+void orc_kernel_dsig(int kernel, double rho, double *h, double *dh, double *zD)
+{
+    double z;
+    orc_kernel(kernel, rho, &z, NULL, NULL);
+    const double dz = orc_dzeta(kernel, rho);
+    if (h) *h = -4.0 * ORC_PI * rho * rho * rho * z;
+    if (dh) *dh = -4.0 * ORC_PI * (3.0 * rho * rho * z + rho * rho * rho * dz);
+    if (zD) *zD = -(3.0 * z + rho * dz);
+}
+
 double orc_zeta0(int kernel)
 {
     double z;
@@ -110,35 +145,29 @@ double orc_zeta0(int kernel)
     return z;
 }
 
-/* One source j acting on one target: accumulates U[3] and J[9].
- * Pair algebra of src/FLOWUnsteady_processing_force.jl:888-905 (dX = x_t - x_s,
- * skip r == 0, source sigma in g, 1/(4 pi) constant), extended to
- * J[k,l] = dU_k/dx_l by the product rule:
- *   J += (g'/(sigma r) - 3 g / r^2) * crss_k * dX_l   (radial part)
- *      + (-g/(4 pi r^3)) * eps_{klm} Gamma_m          (Kronecker part)
- * with crss = -(1/(4 pi r^3)) dX x Gamma. */
-void orc_pair_uj(int kernel, const double *xt, const double *xs, const double *gs,
-                           double sgm, double *U, double *J)
+/* Pair algebra shared by U, J and their sigma-derivatives: a radial function
+ * f(rho) in place of g with its derivative df, so the pair adds
+ *   U += -f/(4 pi r^3) dX x Gamma,
+ *   J += (df/(sigma r) - 3 f / r^2) * crss_k * dX_l   (radial part)
+ *      + (-f/(4 pi r^3)) * eps_{klm} Gamma_m          (Kronecker part)
+ * with crss = -(1/(4 pi r^3)) dX x Gamma.  r > 0. */
+static void pair_accum(double f, double df, double sgm, double dX1, double dX2, double dX3,
+                       double r2, const double *gs, double *U, double *J)
 {
-    double dX1 = xt[0] - xs[0], dX2 = xt[1] - xs[1], dX3 = xt[2] - xs[2];
-    double r2 = dX1 * dX1 + dX2 * dX2 + dX3 * dX3;
-    if (r2 == 0.0) return;
     double r = sqrt(r2);
-    double g, dg;
-    orc_kernel(kernel, r / sgm, NULL, &g, &dg);
     double r3 = r * r2;
     double c1 = -ORC_CONST4 / r3 * (dX2 * gs[2] - dX3 * gs[1]);
     double c2 = -ORC_CONST4 / r3 * (dX3 * gs[0] - dX1 * gs[2]);
     double c3 = -ORC_CONST4 / r3 * (dX1 * gs[1] - dX2 * gs[0]);
-    U[0] += g * c1;
-    U[1] += g * c2;
-    U[2] += g * c3;
-    double aux = dg / (sgm * r) - 3.0 * g / r2;
+    U[0] += f * c1;
+    U[1] += f * c2;
+    U[2] += f * c3;
+    double aux = df / (sgm * r) - 3.0 * f / r2;
     /* column l = derivative direction */
     J[0] += aux * c1 * dX1; J[1] += aux * c2 * dX1; J[2] += aux * c3 * dX1;
     J[3] += aux * c1 * dX2; J[4] += aux * c2 * dX2; J[5] += aux * c3 * dX2;
     J[6] += aux * c1 * dX3; J[7] += aux * c2 * dX3; J[8] += aux * c3 * dX3;
-    double a2 = -ORC_CONST4 * g / r3;
+    double a2 = -ORC_CONST4 * f / r3;
     /* eps_{klm} Gamma_m: (k,l) = (2,1):-G3 (3,1):+G2 (1,2):+G3 (3,2):-G1 (1,3):-G2 (2,3):+G1 */
     J[1] -= a2 * gs[2];
     J[2] += a2 * gs[1];
@@ -146,6 +175,35 @@ void orc_pair_uj(int kernel, const double *xt, const double *xs, const double *g
     J[5] -= a2 * gs[0];
     J[6] -= a2 * gs[1];
     J[7] += a2 * gs[0];
+}
+
+/* One source j acting on one target: accumulates U[3] and J[9].
+ * Pair algebra of src/FLOWUnsteady_processing_force.jl:888-905 (dX = x_t - x_s,
+ * skip r == 0, source sigma in g, 1/(4 pi) constant), extended to
+ * J[k,l] = dU_k/dx_l by the product rule (pair_accum with f = g). */
+void orc_pair_uj(int kernel, const double *xt, const double *xs, const double *gs,
+                           double sgm, double *U, double *J)
+{
+    double dX1 = xt[0] - xs[0], dX2 = xt[1] - xs[1], dX3 = xt[2] - xs[2];
+    double r2 = dX1 * dX1 + dX2 * dX2 + dX3 * dX3;
+    if (r2 == 0.0) return;
+    double g, dg;
+    orc_kernel(kernel, sqrt(r2) / sgm, NULL, &g, &dg);
+    pair_accum(g, dg, sgm, dX1, dX2, dX3, r2, gs, U, J);
+}
+
+/* The same pair's core-size derivatives D[U], D[J] (D = sigma d/dsigma with
+ * every core scaled together; pair_accum with f = h, df = h' of
+ * orc_kernel_dsig).  Coincident pairs skipped, as for U and J. */
+void orc_pair_dj(int kernel, const double *xt, const double *xs, const double *gs, double sgm,
+                 double *DU, double *DJ)
+{
+    double dX1 = xt[0] - xs[0], dX2 = xt[1] - xs[1], dX3 = xt[2] - xs[2];
+    double r2 = dX1 * dX1 + dX2 * dX2 + dX3 * dX3;
+    if (r2 == 0.0) return;
+    double h, dh;
+    orc_kernel_dsig(kernel, sqrt(r2) / sgm, &h, &dh, NULL);
+    pair_accum(h, dh, sgm, dX1, dX2, dX3, r2, gs, DU, DJ);
 }
 
 /* UJ_direct: U and J at every target from every source (accumulating; the
@@ -162,6 +220,21 @@ void orc_uj_direct(int kernel, int64_t nt, const double *tX, int64_t ns, const d
             orc_pair_uj(kernel, tX + 3 * i, sX + 3 * j, sG + 3 * j, ssig[j], u, jj);
         for (int k = 0; k < 3; k++) U[3 * i + k] += u[k];
         for (int k = 0; k < 9; k++) J[9 * i + k] += jj[k];
+    }
+}
+
+/* D[U] and D[J] (core-size derivatives, orc_pair_dj) at every target from
+ * every source, accumulating like orc_uj_direct. */
+void orc_dj_direct(int kernel, int64_t nt, const double *tX, int64_t ns, const double *sX,
+                   const double *sG, const double *ssig, double *DU, double *DJ)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nt; i++) {
+        double u[3] = {0, 0, 0}, jj[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int64_t j = 0; j < ns; j++)
+            orc_pair_dj(kernel, tX + 3 * i, sX + 3 * j, sG + 3 * j, ssig[j], u, jj);
+        for (int k = 0; k < 3; k++) DU[3 * i + k] += u[k];
+        for (int k = 0; k < 9; k++) DJ[9 * i + k] += jj[k];
     }
 }
 
@@ -219,6 +292,35 @@ void orc_stretching(int transposed, const double *J, const double *G, double *S)
     }
 }
 
+/* One source j of the SFS model of vortex stretching at one target
+ * (docs/src/theory/rvpm.md:251-262):
+ *   e  += zeta_sigma_j(x_i - x_j) S(J_i - J_j, Gamma_j),
+ * and, when the core-size derivatives DJ are given (tDJ and sDJ not NULL),
+ *   de += D[zeta_sigma_j] S(J_i - J_j, Gamma_j) + zeta_sigma_j S(DJ_i - DJ_j, Gamma_j),
+ * the derivative D = sigma d/dsigma of e (orc_kernel_dsig's zD).  S is the
+ * stretching of orc_stretching (transposed or classic). */
+void orc_pair_estr(int kernel, int transposed, const double *xt, const double *tJ,
+                   const double *tDJ, const double *xs, const double *gs, double sg,
+                   const double *sJ, const double *sDJ, double *e, double *de)
+{
+    double d1 = xt[0] - xs[0], d2 = xt[1] - xs[1], d3 = xt[2] - xs[2];
+    double r = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+    double z;
+    orc_kernel(kernel, r / sg, &z, NULL, NULL);
+    double is3 = 1.0 / (sg * sg * sg);
+    double dJ[9], S[3];
+    for (int k = 0; k < 9; k++) dJ[k] = tJ[k] - sJ[k];
+    orc_stretching(transposed, dJ, gs, S);
+    for (int k = 0; k < 3; k++) e[k] += z * is3 * S[k];
+    if (tDJ && sDJ) {
+        double zD, dS[3];
+        orc_kernel_dsig(kernel, r / sg, NULL, NULL, &zD);
+        for (int k = 0; k < 9; k++) dJ[k] = tDJ[k] - sDJ[k];
+        orc_stretching(transposed, dJ, gs, dS);
+        for (int k = 0; k < 3; k++) de[k] += zD * is3 * S[k] + z * is3 * dS[k];
+    }
+}
+
 /* Estr_direct: SFS model of vortex stretching (docs/src/theory/rvpm.md:251-262)
  *   E(x_i) = sum_j zeta_sigma_j(x_i - x_j) (Gamma_j . grad)(U(x_i) - U(x_j)),
  * transposed or classic as above.  Accumulates into E. */
@@ -229,19 +331,30 @@ void orc_estr_direct(int kernel, int transposed, int64_t nt, const double *tX, c
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t i = 0; i < nt; i++) {
         double e[3] = {0, 0, 0};
-        for (int64_t j = 0; j < ns; j++) {
-            double d1 = tX[3 * i] - sX[3 * j], d2 = tX[3 * i + 1] - sX[3 * j + 1],
-                   d3 = tX[3 * i + 2] - sX[3 * j + 2];
-            double r = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
-            double sg = ssig[j], z;
-            orc_kernel(kernel, r / sg, &z, NULL, NULL);
-            double zs = z / (sg * sg * sg);
-            double dJ[9], S[3];
-            for (int k = 0; k < 9; k++) dJ[k] = tJ[9 * i + k] - sJ[9 * j + k];
-            orc_stretching(transposed, dJ, sG + 3 * j, S);
-            for (int k = 0; k < 3; k++) e[k] += zs * S[k];
-        }
+        for (int64_t j = 0; j < ns; j++)
+            orc_pair_estr(kernel, transposed, tX + 3 * i, tJ + 9 * i, NULL, sX + 3 * j,
+                          sG + 3 * j, ssig[j], sJ + 9 * j, NULL, e, NULL);
         for (int k = 0; k < 3; k++) E[3 * i + k] += e[k];
+    }
+}
+
+/* Estr and its core-size derivative D[E] from J and D[J] (orc_pair_estr),
+ * accumulating into E and DE. */
+void orc_estr_dyn_direct(int kernel, int transposed, int64_t nt, const double *tX,
+                         const double *tJ, const double *tDJ, int64_t ns, const double *sX,
+                         const double *sG, const double *ssig, const double *sJ,
+                         const double *sDJ, double *E, double *DE)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nt; i++) {
+        double e[3] = {0, 0, 0}, de[3] = {0, 0, 0};
+        for (int64_t j = 0; j < ns; j++)
+            orc_pair_estr(kernel, transposed, tX + 3 * i, tJ + 9 * i, tDJ + 9 * i, sX + 3 * j,
+                          sG + 3 * j, ssig[j], sJ + 9 * j, sDJ + 9 * j, e, de);
+        for (int k = 0; k < 3; k++) {
+            E[3 * i + k] += e[k];
+            DE[3 * i + k] += de[k];
+        }
     }
 }
 
@@ -318,37 +431,6 @@ void orc_relax(int kind, double rlxf, const double *J, double *G)
 }
 
 /* Particle field state for the integrators: AoS per particle. */
-typedef struct {
-    int64_t np;
-    double *X, *G, *sigma, *U, *J, *E, *M;    /* M: 7 registers per particle */
-    double *C;                                /* 3 per particle: C, <num>, <den> */
-    const uint8_t *is_static;                  /* may be NULL: no static particles */
-    const double *vol;                         /* may be NULL: PSE needs it */
-} orc_field;
-
-typedef struct {
-    int kernel, transposed;
-    double f, g;               /* formulation: rVPM f=0 g=1/5, cVPM f=g=0 */
-    double Uinf[3];
-    int sfs;                   /* 0 none, 1 constant Cs, 2 dynamic; Estr_direct */
-    double Cs;
-    int clip_backscatter;
-    int relax_kind;            /* 0 none, 1 pedrizzetti, 2 corrected */
-    double rlxf;
-    double alpha, sfs_rlxf, minC, maxC;   /* dynamic procedure */
-    int viscous;               /* 0 inviscid, 1 core spreading, 2 PSE */
-    double nu, sgm0, beta;     /* viscosity; core spreading reset size and trigger */
-    int itmax;                 /* core spreading: CG iterations */
-    double tol;                /* core spreading: CG relative residual */
-} orc_params;
-
-/* viscous_oracle.c */
-void orc_pse(int kernel, double nu, int64_t n, const double *X, const double *G,
-             const double *sig, const double *vol, int64_t nt, const int64_t *tidx, double *dG);
-void orc_zeta_sum(int kernel, int64_t nt, const double *tX, int64_t ns, const double *sX,
-                  const double *sG, const double *ssig, double s0, double *w);
-int orc_rbf_cg(int kernel, int64_t n, const double *X, double s0, const double *w, double *G,
-               int itmax, double tol, double *res);
 
 /* Core spreading's spatial adaptation (viscous_oracle.c): when a free particle's
  * core exceeds beta sgm0, every free particle's core is reset to sgm0 and its
@@ -395,16 +477,48 @@ int orc_core_spreading_reset(const orc_params *P, orc_field *F, int *iters)
 
 static void field_uj(const orc_params *P, orc_field *F)
 {
+    if (P->fmm) {   /* overwrites U and J */
+        orc_uj_fmm(P->kernel, P->p, P->ncrit, P->theta, P->sigcut, F->np, F->X, F->G, F->sigma,
+                   F->U, F->J, NULL, 0, NULL);
+        return;
+    }
     memset(F->U, 0, sizeof(double) * 3 * F->np);
     memset(F->J, 0, sizeof(double) * 9 * F->np);
     orc_uj_direct(P->kernel, F->np, F->X, F->np, F->X, F->G, F->sigma, F->U, F->J);
 }
 
-static void field_estr(const orc_params *P, orc_field *F)
+/* Estr (and, with DJ, its core-size derivative DE) with the UJ's scheme. */
+static void field_estr(const orc_params *P, orc_field *F, const double *DJ, double *DE)
 {
+    if (P->fmm) {
+        orc_fmm_near(P->kernel, P->transposed, 2, P->ncrit, P->theta, P->sigcut, F->np, F->X,
+                     F->G, F->sigma, F->J, DJ, F->E, DE);
+        return;
+    }
     memset(F->E, 0, sizeof(double) * 3 * F->np);
-    orc_estr_direct(P->kernel, P->transposed, F->np, F->X, F->J, F->np, F->X, F->G, F->sigma,
-                    F->J, F->E);
+    if (DJ) {
+        memset(DE, 0, sizeof(double) * 3 * F->np);
+        orc_estr_dyn_direct(P->kernel, P->transposed, F->np, F->X, F->J, DJ, F->np, F->X, F->G,
+                            F->sigma, F->J, DJ, F->E, DE);
+    } else {
+        orc_estr_direct(P->kernel, P->transposed, F->np, F->X, F->J, F->np, F->X, F->G, F->sigma,
+                        F->J, F->E);
+    }
+}
+
+/* Core-size derivatives D[U], D[J] with the UJ's scheme.  The FMM's far field
+ * (singular kernel, multipoles) does not depend on sigma, so with the FMM
+ * they are sums over its near-field lists only. */
+static void field_dj(const orc_params *P, const orc_field *F, double *DU, double *DJ)
+{
+    if (P->fmm) {
+        orc_fmm_near(P->kernel, P->transposed, 1, P->ncrit, P->theta, P->sigcut, F->np, F->X,
+                     F->G, F->sigma, NULL, NULL, DU, DJ);
+        return;
+    }
+    memset(DU, 0, sizeof(double) * 3 * F->np);
+    memset(DJ, 0, sizeof(double) * 9 * F->np);
+    orc_dj_direct(P->kernel, F->np, F->X, F->np, F->X, F->G, F->sigma, DU, DJ);
 }
 
 static int clipped(const orc_params *P, const orc_field *F, int64_t i, double c)
@@ -413,58 +527,62 @@ static int clipped(const orc_params *P, const orc_field *F, int64_t i, double c)
     return P->clip_backscatter && c * (g[0] * e[0] + g[1] * e[1] + g[2] * e[2]) < 0.0;
 }
 
+/* Numerator and denominator of the dynamic coefficient of one particle
+ * (docs/src/theory/rvpm.md:264-290), C_d = <Gamma.L> / <Gamma.m> with
+ *   m = sigma^3/zeta0 dE/dsigma,
+ *   L = (3/sigma) (Gamma.grad)(u - ubar) + (Gamma.grad) dubar/dsigma.
+ * The sigma-derivatives are exact (orc_kernel_dsig): dubar/dsigma = D[ubar]/sigma
+ * and dE/dsigma = D[E]/sigma with D = sigma d/dsigma over every core at once.
+ * The unfiltered velocity u is not resolved; the procedure's test filter
+ * alpha sigma (alpha < 1, a finer filter) stands in for it, as the third level
+ * of a pseudo-three-level procedure, to first order in 1 - alpha:
+ *   u - ubar ~ ubar(alpha sigma) - ubar(sigma) ~ (alpha - 1) D[ubar],
+ * so L = (3 alpha - 2)/sigma (Gamma.grad) D[ubar] and
+ *   num = (3 alpha - 2)/sigma  Gamma^T D[J] Gamma   (the same for J or J^T),
+ *   den = sigma^2/zeta0  Gamma . D[E]. */
+void orc_dyn_terms(double alpha, double zeta0, double sigma, const double *G, const double *DJ,
+                   const double *DE, double *num, double *den)
+{
+    double q = 0.0;
+    for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++) q += G[k] * DJ[k + 3 * l] * G[l];
+    *num = (3.0 * alpha - 2.0) / sigma * q;
+    *den = sigma * sigma / zeta0 * (G[0] * DE[0] + G[1] * DE[1] + G[2] * DE[2]);
+}
+
 /* One stage's UJ and SFS coefficient.
- * Constant model: E = Estr_direct, C = Cs, then backscatter clipping C = 0
- * where C Gamma.E < 0 (docs/src/theory/rvpm.md:296).
- * Dynamic model (rvpm.md:264-296, C_d = <Gamma.L>/<Gamma.m>): the sigma
- * derivatives of the resolved stretching (L) and of the SFS model (m) are
- * taken as the differences between a test filter alpha*sigma and the domain
- * filter sigma, so per particle
- *   num = Gamma . (S(alpha sigma) - S(sigma)),
- *   den = Gamma . (E(alpha sigma) - E(sigma)) sigma^3 / zeta0,
- * averaged along the particle's trajectory as a <- w a + (1 - w) a_old (the
- * first sample is taken as is), C = <num>/<den> bounded to [minC, maxC], then
- * backscatter clipping.  This restates the documented procedure; FLOWVPM's
- * own implementation is not in the reference tree. */
+ * Constant model: E = Estr, C = Cs, then backscatter clipping C = 0 where
+ * C Gamma.E < 0 (docs/src/theory/rvpm.md:296).
+ * Dynamic model: num and den of orc_dyn_terms, each averaged along the
+ * particle's trajectory as a <- w a + (1 - w) a_old (the first sample is
+ * taken as is), C = <num>/<den> bounded to [minC, maxC], then backscatter
+ * clipping.  FLOWVPM's own implementation is not in the reference tree; this
+ * restates the documented procedure with both terms of L. */
 static void field_stage(const orc_params *P, orc_field *F)
 {
     const int64_t n = F->np;
-    double *D = NULL;
-    if (P->sfs == 2) {
-        D = malloc(sizeof(double) * 6 * (n > 0 ? n : 1));
-        double *sig0 = malloc(sizeof(double) * (n > 0 ? n : 1));
-        for (int64_t i = 0; i < n; i++) {
-            sig0[i] = F->sigma[i];
-            F->sigma[i] = P->alpha * sig0[i];
-        }
-        field_uj(P, F);
-        field_estr(P, F);
-        for (int64_t i = 0; i < n; i++) {
-            orc_stretching(P->transposed, F->J + 9 * i, F->G + 3 * i, D + 6 * i);
-            for (int k = 0; k < 3; k++) D[6 * i + 3 + k] = F->E[3 * i + k];
-            F->sigma[i] = sig0[i];
-        }
-        free(sig0);
-    }
     field_uj(P, F);
     if (!P->sfs) return;
-    field_estr(P, F);
-    const double zeta0 = orc_zeta0(P->kernel);
-    for (int64_t i = 0; i < n; i++) {
-        if (P->sfs == 1) {
+    if (P->sfs == 1) {
+        field_estr(P, F, NULL, NULL);
+        for (int64_t i = 0; i < n; i++) {
             double c = P->Cs;
             if (clipped(P, F, i, c)) c = 0.0;
             F->C[3 * i] = c;
-            continue;
         }
+        return;
+    }
+    const size_t m = n > 0 ? (size_t)n : 1;
+    double *DU = malloc(sizeof(double) * 3 * m), *DJ = malloc(sizeof(double) * 9 * m),
+           *DE = malloc(sizeof(double) * 3 * m);
+    field_dj(P, F, DU, DJ);
+    field_estr(P, F, DJ, DE);
+    const double zeta0 = orc_zeta0(P->kernel);
+    for (int64_t i = 0; i < n; i++) {
         if (F->is_static && F->is_static[i]) continue;
-        double S[3];
-        const double *g = F->G + 3 * i, *e = F->E + 3 * i, *d = D + 6 * i;
-        orc_stretching(P->transposed, F->J + 9 * i, g, S);
-        double num = g[0] * (d[0] - S[0]) + g[1] * (d[1] - S[1]) + g[2] * (d[2] - S[2]);
-        double sg = F->sigma[i];
-        double den = (g[0] * (d[3] - e[0]) + g[1] * (d[4] - e[1]) + g[2] * (d[5] - e[2])) *
-                     (sg * sg * sg / zeta0);
+        double num, den;
+        orc_dyn_terms(P->alpha, zeta0, F->sigma[i], F->G + 3 * i, DJ + 9 * i, DE + 3 * i, &num,
+                      &den);
         double *C = F->C + 3 * i;
         double na = num, da = den;
         if (C[1] != 0.0 || C[2] != 0.0) {
@@ -478,7 +596,7 @@ static void field_stage(const orc_params *P, orc_field *F)
         if (clipped(P, F, i, c)) c = 0.0;
         C[0] = c;
     }
-    free(D);
+    free(DU); free(DJ); free(DE);
 }
 
 static void field_relax(const orc_params *P, orc_field *F)

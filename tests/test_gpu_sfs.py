@@ -18,6 +18,9 @@ def blob(n=3000, seed=4):
 
 @pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
 def test_dynamic_sfs_rk3_vs_oracle(kernel):
+    """Direct FP64 RK3 with the dynamic procedure and Pedrizzetti relaxation
+    against the oracle: state, coefficient and its averaged numerator and
+    denominator per particle."""
     X, G, s = blob()
     sfs = vpm.DynamicSFS(alpha=0.999, rlxf=0.005, minC=0.0, maxC=1.0)
     pf = make_field(X, G, s, kernel=kernel, SFS=sfs, relaxation=vpm.pedrizzetti)
@@ -27,16 +30,55 @@ def test_dynamic_sfs_rk3_vs_oracle(kernel):
     for i in range(1, 3):
         vpm.nextstep(pf, 0.002, relax=True)
         O.rk3_step(P, F, 0.002, True)
-    # the alpha = 0.999 test-filter differences amplify FP64 rounding (~1e3),
-    # so the state agrees to 1e-11 rather than 1e-12
-    for get, ref in ((vpm.get_X, F.X), (vpm.get_Gamma, F.G)):
-        assert normwise(get(pf), ref)[0] < 1e-11
+    for get, ref in ((vpm.get_X, F.X), (vpm.get_Gamma, F.G), (vpm.get_sigma, F.sigma)):
+        assert normwise(get(pf), ref)[0] < 1e-12
     C = pf.particles[vpm.C_INDEX, :pf.np].T
-    # the alpha = 0.999 differences cancel ~3 digits: compare normwise at 1e-9
-    for k in range(3):
-        scale = np.abs(F.C[:, k]).max()
-        assert np.abs(C[:, k] - F.C[:, k]).max() <= 1e-9 * scale, k
+    assert np.abs(C[:, 0] - F.C[:, 0]).max() <= 1e-12
+    for k in (1, 2):
+        assert np.abs(C[:, k] - F.C[:, k]).max() <= 1e-12 * np.abs(F.C[:, k]).max(), k
     assert np.any(C[:, 0] > 0)
+
+
+def _blob_c4(n, seed=4):
+    """The C4 blob's distribution at n particles with the 2M blob's particle
+    density (std scaled by (n / 2^21)^(1/3)): swirl Gamma, lognormal sigma."""
+    std = 0.5 * (n / float(1 << 21)) ** (1.0 / 3.0)
+    X, G, s, _ = workloads.gaussian_blob(n=n, seed=seed, std=std, swirl=0.01 * 0.5 / std)
+    return X, G, s
+
+
+C4_FMM = dict(p=8, ncrit=50, theta=0.4)   # FLOWUnsteady's default geometric acceptance
+
+
+def test_fmm_dynamic_sfs_rk3_vs_oracle():
+    """The full rVPM step of the rotor-hover high-fidelity setting on a 20k
+    blob: FMM p = 8 (geometric acceptance), dynamic SFS, Pedrizzetti
+    relaxation, RK3 -- FP64 GPU against the oracle's CPU FMM step (same tree,
+    lists and near-field sums), per particle."""
+    X, G, s = _blob_c4(20000)
+    pf = make_field(X, G, s, SFS=vpm.DynamicSFS(), relaxation=vpm.pedrizzetti,
+                    fmm=vpm.FMM(**C4_FMM))
+    pf.UJ = vpm.UJ_fmm_gpu
+    P = O.make_params("gaussianerf", True, "rVPM", sfs="dynamic", relax_kind=1, rlxf=0.3,
+                      fmm=True, sigcut=0.0, **C4_FMM)
+    F = O.Field(X, G, s)
+    dt = 1e-3
+    for i in range(2):
+        vpm.nextstep(pf, dt, relax=True)
+        O.rk3_step(P, F, dt, True)
+        print("step", i, flush=True)
+    st = vpm._context(0).fmm_stats()
+    print("fmm stats", st)
+    for get, ref in ((vpm.get_X, F.X), (vpm.get_Gamma, F.G), (vpm.get_sigma, F.sigma)):
+        e = normwise(get(pf), ref)
+        print(get.__name__, e)
+        assert e[0] < 1e-12
+    C = pf.particles[vpm.C_INDEX, :pf.np].T
+    print("C range", F.C[:, 0].min(), F.C[:, 0].max(), "clipped", np.mean(F.C[:, 0] == 0))
+    assert np.abs(C[:, 0] - F.C[:, 0]).max() <= 1e-11
+    for k in (1, 2):
+        assert np.abs(C[:, k] - F.C[:, k]).max() <= 1e-12 * np.abs(F.C[:, k]).max(), k
+    assert np.any((C[:, 0] > 0) & (C[:, 0] < 1))
 
 
 def test_estr_fmm_vs_direct():

@@ -272,6 +272,46 @@ def test_targets_match_appended_probes():
     assert normwise(Ut, Up)[0] < 1e-13
 
 
+def test_targets_with_static_particles_and_mirror():
+    """Vvpm_on_Xs with static_particles_fun and mirror images (the arguments
+    of simulation.jl:494-534) against the reference's steps written out:
+    statics and images appended, zero-strength probes appended, one UJ over
+    everything, U read at the probes; and against the oracle.  The field is
+    left as it was."""
+    X, G, s, _ = workloads.vortex_ring(n_stations=20, n_cross=40)
+    rng = np.random.default_rng(9)
+    P = rng.uniform(-1.5, 1.5, (48, 3))
+    SX = rng.uniform(-0.5, 0.5, (30, 3)) + [0.0, 0.0, 0.6]
+    SG = rng.normal(0.0, 1e-2, (30, 3))
+
+    def statics(pf, t, dt):
+        for k in range(len(SX)):
+            vpm.add_particle(pf, SX[k], SG[k], 0.05, vol=1e-4, static=True)
+
+    X0, nrm = np.array([0.0, 0.0, -0.4]), np.array([0.0, 0.0, 1.0])
+    extra = 2 * (len(X) + len(SX)) + len(P)
+    pf = make_field(X, G, s, extra=extra)
+    before = pf.particles[:, :pf.np].copy()
+    Ut = vpm.Vvpm_on_Xs(pf, P, static_particles_fun=statics, mirror=True, mirror_X=X0,
+                        mirror_normal=nrm, fsgm=0.5)
+    assert pf.np == len(X) and np.array_equal(pf.particles[:, :pf.np], before)
+    # the reference's way
+    ref = make_field(X, G, s, extra=extra)
+    statics(ref, 0.0, 0.0)
+    vpm.mirror_particles(ref, X0, nrm)
+    nsrc = ref.np
+    src = (vpm.get_X(ref).copy(), vpm.get_Gamma(ref).copy(), vpm.get_sigma(ref).copy())
+    for p in P:
+        vpm.add_particle(ref, p, np.zeros(3), 1e-6, vol=0.0)
+    vpm.UJ_gpu(ref)
+    Up = vpm.get_U(ref)[nsrc:]
+    assert normwise(Ut, Up)[0] < 1e-13
+    Ur, _ = O.uj_direct("gaussianerf", P, *src)
+    assert normwise(Ut, Ur)[0] < 1e-12
+    # images of the ring below the plane z = -0.4 change the probe velocities
+    assert normwise(Ut, vpm.Vvpm_on_Xs(pf, P))[0] > 1e-3
+
+
 @pytest.mark.parametrize("kernel", ["gaussianerf", "gaussian"])
 @pytest.mark.parametrize("n", [1 << 18, 60000])
 def test_mixed_far_tiles_and_split_sources(kernel, n):

@@ -1,12 +1,24 @@
-"""Properties of the oracle's SFS coefficient procedures.
+"""Properties and pins of the oracle's SFS coefficient procedures.
 
-The dynamic procedure restates docs/src/theory/rvpm.md:264-296 with the
-sigma-derivatives taken by a test filter alpha*sigma; FLOWVPM's own
-implementation is not in the reference tree, so its exact output is parity
-unpinned.  What is checked here is what the documented model fixes: the
-coefficient is a ratio of two quantities cubic in Gamma (so invariant when
-every Gamma is scaled), it respects its bounds, and backscatter clipping zeroes
-it wherever C Gamma.E < 0.
+The dynamic procedure restates docs/src/theory/rvpm.md:264-296 with both
+terms of L and analytic core-size derivatives (D = sigma d/dsigma of every
+core at once, per pair through h = -rho g'(rho)).  FLOWVPM's own
+implementation is not in the reference tree.  What pins the oracle here:
+  - the kernel derivatives h, h', zD against central differences of the
+    closed forms (orc_kernel, itself pinned by quadrature);
+  - D[U], D[J] against central differences in sigma of orc_uj_direct (pinned
+    by the golden vectors), and D[U], D[J] of a lattice-sampled Gaussian
+    vortex column against the closed form of its smoothed field, whose core
+    has variance L^2 + sigma^2;
+  - D[E] against central differences in sigma of orc_estr_direct fed with
+    J(sigma);
+  - the coefficient's numerator and denominator against the documented
+    formula evaluated term by term with an actual test-filter evaluation
+    (u - ubar = ubar(alpha sigma) - ubar(sigma)) and finite-difference
+    sigma-derivatives, which a dropped term of L fails;
+  - the near-field sums of the oracle FMM against the direct sums;
+plus what the model fixes: a ratio of two quantities cubic in Gamma, its
+bounds, backscatter clipping and the Lagrangian average.
 """
 import numpy as np
 import pytest
@@ -47,9 +59,8 @@ def test_dynamic_scale_invariance():
     F2 = O.Field(X, 3.0 * G, s)
     O.euler_step(P, F1, 0.0)
     O.euler_step(P, F2, 0.0)
-    # the alpha = 0.999 differences cancel ~3 digits, hence 1e-7 rather than 1e-13
-    np.testing.assert_allclose(F2.C[:, 0], F1.C[:, 0], rtol=1e-7, atol=1e-12)
-    np.testing.assert_allclose(F2.C[:, 1], 27.0 * F1.C[:, 1], rtol=1e-7)
+    np.testing.assert_allclose(F2.C[:, 0], F1.C[:, 0], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(F2.C[:, 1], 27.0 * F1.C[:, 1], rtol=1e-12)
 
 
 def test_dynamic_lagrangian_average():
@@ -77,3 +88,173 @@ def test_constant_model_clipping():
     ge = np.sum(F.G * F.E, axis=1)
     assert set(np.unique(C)) <= {0.0, 0.2}
     np.testing.assert_array_equal(C == 0.0, 0.2 * ge < 0.0)
+
+
+SMOOTH = ["gaussianerf", "winckelmans", "gaussian"]
+
+
+@pytest.mark.parametrize("kernel", SMOOTH)
+def test_kernel_dsig_vs_finite_differences(kernel):
+    """h = sigma d/dsigma g(r/sigma) = d/dlam g(rho/lam) at lam = 1, h' = dh/drho,
+    zD = d/dlam [zeta(rho/lam)/lam^3]: central differences of the closed forms."""
+    eps = 1e-5
+    for rho in (0.05, 0.3, 0.9, 1.7, 2.6, 4.0):
+        h, dh, zD = O.kernel_dsig(kernel, rho)
+        g = lambda lam: O.kernel(kernel, rho / lam)[1]
+        z = lambda lam: O.kernel(kernel, rho / lam)[0] / lam ** 3
+        h_fd = (g(1 + eps) - g(1 - eps)) / (2 * eps)
+        zD_fd = (z(1 + eps) - z(1 - eps)) / (2 * eps)
+        hr = lambda r: O.kernel_dsig(kernel, r)[0]
+        dh_fd = (hr(rho + eps) - hr(rho - eps)) / (2 * eps)
+        sc = max(abs(h), 1e-3)
+        assert abs(h - h_fd) < 1e-8 * sc + 1e-12, (rho, h, h_fd)
+        assert abs(dh - dh_fd) < 1e-7 * max(abs(dh), 1e-3), (rho, dh, dh_fd)
+        assert abs(zD - zD_fd) < 1e-8 * max(abs(zD), 1e-3), (rho, zD, zD_fd)
+
+
+def _scaled_uj(kernel, X, G, s, lam):
+    return O.uj_direct(kernel, X, X, G, s * lam)
+
+
+@pytest.mark.parametrize("kernel", SMOOTH)
+def test_dj_direct_vs_sigma_differences(kernel):
+    """D[U], D[J] equal d/dlam of U, J with every core scaled by lam."""
+    X, G, s = blob(300, seed=11)
+    DU, DJ = O.dj_direct(kernel, X, X, G, s)
+    eps = 1e-4
+    Up, Jp = _scaled_uj(kernel, X, G, s, 1 + eps)
+    Um, Jm = _scaled_uj(kernel, X, G, s, 1 - eps)
+    DU_fd, DJ_fd = (Up - Um) / (2 * eps), (Jp - Jm) / (2 * eps)
+    for a, b in ((DU, DU_fd), (DJ, DJ_fd)):
+        err = np.abs(a - b).max() / np.abs(b).max()
+        assert err < 1e-6, err
+    # a plausible slip -- h taken as +rho g' -- fails the same check
+    assert np.abs(-DJ - DJ_fd).max() / np.abs(DJ_fd).max() > 1.0
+
+
+def test_dj_lattice_gaussian_column_closed_form():
+    """Particles sampling a Gaussian vortex column omega_z = G0/(2 pi L^2)
+    exp(-r^2/(2 L^2)) on a lattice; with the gaussian-erf kernel the smoothed
+    column has variance s^2 = L^2 + sigma^2, so u_theta = G0/(2 pi r)(1 -
+    exp(-r^2/(2 s^2))) and
+      D[u_theta] = sigma d/dsigma u_theta = -G0 sigma^2 r exp(-r^2/(2 s^2)) / (2 pi s^4).
+    The column's finite length changes U but not D[U], whose pair kernel is
+    the core zeta_sigma itself (exponentially local)."""
+    h, L, sig, G0 = 0.08, 0.2, 0.1, 1.3
+    ax = np.arange(-18, 19) * h
+    az = (np.arange(-20, 20) + 0.5) * h
+    x, y, z = np.meshgrid(ax, ax, az, indexing="ij")
+    X = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    w = G0 / (2 * np.pi * L * L) * np.exp(-(X[:, 0] ** 2 + X[:, 1] ** 2) / (2 * L * L))
+    G = np.zeros_like(X)
+    G[:, 2] = w * h ** 3
+    s = np.full(len(X), sig)
+    r = np.array([0.5 * L, L, 1.7 * L, 2.5 * L])
+    T = np.stack([r, np.zeros_like(r), np.zeros_like(r)], axis=1)
+    DU, DJ = O.dj_direct("gaussianerf", T, X, G, s)
+    s2 = L * L + sig * sig
+    f = -G0 * sig * sig * r * np.exp(-r * r / (2 * s2)) / (2 * np.pi * s2 * s2)
+    df = f / r - f * r / s2                     # d f / dr
+    scale = np.abs(f).max()
+    # the lattice sum of the Gaussian integrand converges spectrally: 1.7e-10 at h = 0.08
+    assert np.abs(DU[:, 1] - f).max() < 1e-8 * scale          # e_theta = e_y on the x axis
+    assert np.abs(DU[:, 0]).max() < 1e-10 * scale and np.abs(DU[:, 2]).max() < 1e-10 * scale
+    assert np.abs(DJ[:, 1 + 3 * 0] - df).max() < 1e-8 * np.abs(df).max()   # d(Du_y)/dx
+    assert np.abs(DJ[:, 0 + 3 * 1] + f / r).max() < 1e-8 * np.abs(f / r).max()   # d(Du_x)/dy
+
+
+@pytest.mark.parametrize("kernel", SMOOTH)
+@pytest.mark.parametrize("transposed", [True, False])
+def test_estr_dyn_vs_sigma_differences(kernel, transposed):
+    """D[E] equals d/dlam of Estr evaluated with every core scaled by lam and
+    the J of that scaling; E itself equals orc_estr_direct."""
+    X, G, s = blob(250, seed=12)
+    _, J = O.uj_direct(kernel, X, X, G, s)
+    _, DJ = O.dj_direct(kernel, X, X, G, s)
+    E, DE = O.estr_dyn_direct(kernel, transposed, X, J, DJ, X, G, s, J, DJ)
+    E0 = O.estr_direct(kernel, transposed, X, J, X, G, s, J)
+    np.testing.assert_allclose(E, E0, rtol=1e-13, atol=1e-13 * np.abs(E0).max())
+    eps = 1e-4
+    Es = []
+    for lam in (1 + eps, 1 - eps):
+        _, Jl = O.uj_direct(kernel, X, X, G, s * lam)
+        Es.append(O.estr_direct(kernel, transposed, X, Jl, X, G, s * lam, Jl))
+    DE_fd = (Es[0] - Es[1]) / (2 * eps)
+    err = np.abs(DE - DE_fd).max() / np.abs(DE_fd).max()
+    assert err < 1e-6, err
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_dyn_terms_follow_documented_L_and_m(kernel):
+    """num = Gamma.L and den = Gamma.m of docs/src/theory/rvpm.md:276-290,
+    evaluated term by term: u - ubar from an actual test-filter evaluation
+    ubar(alpha sigma) - ubar(sigma) and the sigma-derivatives by central
+    differences.  The oracle's first-order form agrees to O((1 - alpha)^2)
+    (the error falls ~4x when 1 - alpha halves); dropping L's first term
+    would be off by 3 (1 - alpha)."""
+    X, G, s = blob(250, seed=13)
+    _, J = O.uj_direct(kernel, X, X, G, s)
+    _, DJ = O.dj_direct(kernel, X, X, G, s)
+    _, DE = O.estr_dyn_direct(kernel, True, X, J, DJ, X, G, s, J, DJ)
+    z0 = O.zeta0(kernel)
+    eps = 1e-4
+    _, Jp = O.uj_direct(kernel, X, X, G, s * (1 + eps))
+    _, Jm = O.uj_direct(kernel, X, X, G, s * (1 - eps))
+    Ep = O.estr_direct(kernel, True, X, Jp, X, G, s * (1 + eps), Jp)
+    Em = O.estr_direct(kernel, True, X, Jm, X, G, s * (1 - eps), Jm)
+    errs = []
+    for alpha in (0.98, 0.99):
+        _, Ja = O.uj_direct(kernel, X, X, G, s * alpha)
+        num_o, den_o, num_d, den_d = [], [], [], []
+        for i in range(len(X)):
+            n1, d1 = O.dyn_terms(alpha, z0, s[i], G[i], DJ[i], DE[i])
+            num_o.append(n1); den_o.append(d1)
+            Gm = G[i]
+            JGG = lambda Jr: Gm @ Jr.reshape(3, 3, order="F") @ Gm      # Gamma^T J Gamma
+            dudsig = (JGG(Jp[i]) - JGG(Jm[i])) / (2 * eps * s[i])
+            num_d.append(3.0 / s[i] * (JGG(Ja[i]) - JGG(J[i])) + dudsig)
+            dEdsig = (Ep[i] - Em[i]) / (2 * eps * s[i])
+            den_d.append(s[i] ** 3 / z0 * (Gm @ dEdsig))
+        num_o, den_o, num_d, den_d = map(np.array, (num_o, den_o, num_d, den_d))
+        en = np.abs(num_o - num_d).max() / np.abs(num_d).max()
+        ed = np.abs(den_o - den_d).max() / np.abs(den_d).max()
+        assert en < 0.1 * 3 * (1 - alpha), (alpha, en)
+        assert ed < 1e-6, ed
+        # the one-term numerator (first term of L dropped) fails
+        one = num_o / (3 * alpha - 2)
+        assert np.abs(one - num_d).max() / np.abs(num_d).max() > 0.5 * 3 * (1 - alpha)
+        errs.append(en)
+    assert errs[1] < errs[0] / 3.0, errs
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_fmm_near_sums_vs_direct(kernel):
+    """With the core-aware acceptance (gap of sigcut core radii) every pair
+    the near lists leave out is beyond the kernel's reach, so the near-field
+    sums equal the direct ones; theta = 0 makes every pair near."""
+    X, G, s = blob(1500, seed=14)
+    _, J = O.uj_direct(kernel, X, X, G, s)
+    DU, DJ = O.dj_direct(kernel, X, X, G, s)
+    E, DE = O.estr_dyn_direct(kernel, True, X, J, DJ, X, G, s, J, DJ)
+    for theta, sigcut, tol in ((0.0, 0.0, 1e-13), (0.5, 9.0 if kernel == "gaussianerf" else 60.0, 1e-6)):
+        a, b = O.fmm_near(kernel, True, 1, X, G, s, ncrit=20, theta=theta, sigcut=sigcut)
+        c, d = O.fmm_near(kernel, True, 2, X, G, s, J=J, DJ=DJ, ncrit=20, theta=theta,
+                          sigcut=sigcut)
+        for got, ref in ((a, DU), (b, DJ), (c, E), (d, DE)):
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            assert err < tol, (theta, sigcut, err)
+
+
+def test_fmm_step_with_theta_zero_is_the_direct_step():
+    """The oracle's FMM step (UJ_fmm, Estr and core-size derivatives over the
+    near lists) with theta = 0 has no far field: it is the direct step."""
+    X, G, s = blob(400, seed=15)
+    kw = dict(sfs="dynamic", relax_kind=1, rlxf=0.3, minC=0.0, maxC=1.0)
+    Pd = O.make_params("gaussianerf", True, "rVPM", **kw)
+    Pf = O.make_params("gaussianerf", True, "rVPM", fmm=True, p=4, ncrit=30, theta=0.0, **kw)
+    Fd, Ff = O.Field(X, G, s), O.Field(X, G, s)
+    for _ in range(2):
+        O.rk3_step(Pd, Fd, 2e-3, True)
+        O.rk3_step(Pf, Ff, 2e-3, True)
+    for a, b in ((Ff.X, Fd.X), (Ff.G, Fd.G), (Ff.C, Fd.C)):
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
