@@ -389,7 +389,8 @@ def full_step_run(n, steps, log, precisions=("mixed", "fp64"), every=10):
             res["dt_rule"] = "0.05 median(sigma) / max|U| of the initial field"
         series = [(0,) + diag()]
         ctx.set_timing(True)
-        kinds = (("fmm_device_ms", N.T_FMM), ("estr_ms", N.T_ESTR), ("update_ms", N.T_UPDATE))
+        kinds = (("fmm_device_ms", N.T_FMM), ("dsig_ms", N.T_DSIG), ("estr_ms", N.T_ESTR),
+                 ("update_ms", N.T_UPDATE))
         split = {name: 0.0 for name, _ in kinds}
         wall = 0.0
         for k in range(1, steps + 1):
@@ -407,7 +408,8 @@ def full_step_run(n, steps, log, precisions=("mixed", "fp64"), every=10):
         ens0, en0 = series[0][1], series[0][2]
         res[prec] = {"ms_per_step": 1e3 * wall / steps, "per_step_split": split,
                      "host_tree_ms_per_uj": st["ms_tree"], "traverse_ms_per_uj": st["ms_traverse"],
-                     "uj_per_step": 7, "estr_per_step": 6,
+                     "uj_per_step": 4, "core_size_derivative_passes_per_step": 3,
+                     "estr_per_step": 3,
                      "enstrophy": [[k, e] for k, e, _ in series],
                      "energy": [[k, e] for k, _, e in series],
                      "enstrophy_drift": (series[-1][1] - ens0) / abs(ens0),
