@@ -78,6 +78,10 @@ SIGNATURES = {
     "vpmx_version": (ctypes.c_char_p, []),
     "vpmx_uj": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64, ctypes.POINTER(Layout),
                                ctypes.POINTER(UJParams)]),
+    "vpmx_uj_device_noalloc": (ctypes.c_int, [_ctxp, ctypes.c_int64, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.POINTER(UJParams),
+                                              ctypes.c_void_p]),
     "vpmx_rk3_step": (ctypes.c_int, [_ctxp, _dp, ctypes.c_int64, ctypes.POINTER(Layout),
                                      ctypes.POINTER(UJParams), ctypes.POINTER(StepParams),
                                      ctypes.c_double, ctypes.c_int32]),

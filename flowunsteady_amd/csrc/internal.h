@@ -152,6 +152,13 @@ namespace vpmx {
 hipError_t launch_pack_f64(hipStream_t st, const DeviceField &f, Scratch &s);
 hipError_t launch_uj_f64(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
                          int reset);
+// FP64 direct UJ on caller-owned SoA device arrays of stride n, with its
+// scratch (source records, split partials) in a caller-provided block of at
+// least uj_f64_dev_scratch_bytes(n) bytes, 256-byte aligned.
+size_t uj_f64_dev_scratch_bytes(int64_t n);
+hipError_t launch_uj_f64_dev(hipStream_t st, int kernel, int64_t n, const double *X,
+                             const double *G, const double *sigma, double *U, double *J,
+                             int reset, void *scratch);
 hipError_t launch_sort_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, Scratch &s);
 hipError_t launch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
                            int reset);

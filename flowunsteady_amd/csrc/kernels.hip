@@ -177,6 +177,35 @@ hipError_t launch_uj_f64(hipStream_t st, int kernel, const DeviceField &f, const
     return uj_f64_any(st, kernel, s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset, s.kpart);
 }
 
+// Caller-owned device arrays (stride n) and a caller-provided scratch block:
+// the source records (2n double4) first, then the split partials when the
+// launch is split (12 doubles per target per segment).
+static size_t uj_f64_dev_part_offset(int64_t n)
+{
+    return ((size_t)n * 2 * sizeof(double4) + 255) / 256 * 256;
+}
+
+size_t uj_f64_dev_scratch_bytes(int64_t n)
+{
+    const Split sp = split_for((n + UJ_BLOCK - 1) / UJ_BLOCK, n);
+    const size_t part = sp.nseg > 1 ? (size_t)sp.nseg * 12 * (size_t)n * sizeof(double) : 0;
+    return uj_f64_dev_part_offset(n) + part;
+}
+
+hipError_t launch_uj_f64_dev(hipStream_t st, int kernel, int64_t n, const double *X,
+                             const double *G, const double *sigma, double *U, double *J,
+                             int reset, void *scratch)
+{
+    if (n == 0) return hipSuccess;
+    double4 *src = static_cast<double4 *>(scratch);
+    double *part = reinterpret_cast<double *>(static_cast<char *>(scratch) +
+                                              uj_f64_dev_part_offset(n));
+    pack_f64_kernel<<<grid_for(n, EW_BLOCK), EW_BLOCK, 0, st>>>(X, G, sigma, n, n, src);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return uj_f64_any(st, kernel, src, n, X, n, n, U, J, reset, part);
+}
+
 // Core-size derivatives D[U], D[J] of every particle into DU (3 x cap) and DJ
 // (9 x cap), FP64, from the source records of the UJ that just ran.
 hipError_t launch_dsig_f64(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,

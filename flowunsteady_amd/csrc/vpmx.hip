@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -20,7 +21,19 @@
 
 using namespace vpmx;
 
+// Scratch of vpmx_uj_device_noalloc: a fixed block in every device's global
+// memory, held until the process ends, so that entry point allocates nothing.
+// Host code never passes `scratch` itself (on the host it names a stand-in,
+// not device memory): vpmx_create resolves each device's own address once,
+// with that device current, into g_noalloc_ptr.
+#define VPMX_NOALLOC_SCRATCH_BYTES (48u << 20)
+static __device__ __align__(256) unsigned char scratch[VPMX_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
 
 thread_local std::string g_noctx_err;
 
@@ -1248,6 +1261,18 @@ int vpmx_create(vpmx_ctx **out, int32_t device)
                                               " out of range (" + std::to_string(n) + " devices)");
     e = hipSetDevice(device);
     if (e != hipSuccess) return fail(nullptr, VPMX_ENODEV, hipGetErrorString(e));
+    if (device < NOALLOC_MAX_DEVICES) {
+        // once per device, with it current: its own address of `scratch`
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (!g_noalloc_ptr[device]) {
+            void *ptr = nullptr;
+            e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr)
+                return fail(nullptr, VPMX_EHIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                                    hipGetErrorString(e));
+            g_noalloc_ptr[device] = ptr;
+        }
+    }
     vpmx_ctx *c = new (std::nothrow) vpmx_ctx();
     if (!c) return fail(nullptr, VPMX_ENOMEM, "out of host memory");
     c->device = device;
@@ -1297,6 +1322,35 @@ int vpmx_uj(vpmx_ctx *c, double *particles, int64_t np, const vpmx_layout *L,
     if (P->sfs) VPMX_TRY(enqueue_estr(c, P));
     VPMX_TRY(sync(c));
     return download(c, particles, L, DL_UJ | (P->sfs ? DL_SFS : 0));
+}
+
+int vpmx_uj_device_noalloc(vpmx_ctx *c, int64_t n, const double *X, const double *Gamma,
+                           const double *sigma, double *U, double *J, const vpmx_uj_params *P,
+                           void *stream)
+{
+    VPMX_TRY(entry(c));
+    VPMX_TRY(check_uj_params(c, P));
+    if (n < 0) return fail(c, VPMX_EINVAL, "n is negative");
+    if (n > VPMX_NOALLOC_MAX_N)
+        return fail(c, VPMX_EINVAL, "n = " + std::to_string(n) + " exceeds the no-allocation " +
+                                        "limit of " + std::to_string(VPMX_NOALLOC_MAX_N) +
+                                        " particles: use vpmx_uj");
+    if (P->precision != VPMX_FP64 || P->fmm || P->sfs)
+        return fail(c, VPMX_EUNSUPPORTED, "vpmx_uj_device_noalloc runs the FP64 direct UJ only "
+                                          "(precision 0, fmm 0, sfs 0)");
+    if (n == 0) return VPMX_OK;
+    if (!X || !Gamma || !sigma || !U || !J)
+        return fail(c, VPMX_EINVAL, "a device array is NULL");
+    void *ptr = c->device < NOALLOC_MAX_DEVICES ? g_noalloc_ptr[c->device] : nullptr;
+    if (!ptr)
+        return fail(c, VPMX_EUNSUPPORTED, "no scratch block on device " +
+                                              std::to_string(c->device));
+    if (uj_f64_dev_scratch_bytes(n) > VPMX_NOALLOC_SCRATCH_BYTES)
+        return fail(c, VPMX_EINVAL, "n = " + std::to_string(n) + " needs more scratch than the " +
+                                        "fixed block holds");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    VPMX_HIP(c, launch_uj_f64_dev(st, P->kernel, n, X, Gamma, sigma, U, J, P->reset, ptr));
+    return VPMX_OK;
 }
 
 static int step_host(vpmx_ctx *c, double *particles, int64_t np, const vpmx_layout *L,

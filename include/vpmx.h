@@ -172,6 +172,24 @@ const char *vpmx_version(void);
 int vpmx_uj(vpmx_ctx *ctx, double *particles, int64_t np, const vpmx_layout *layout,
             const vpmx_uj_params *params);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the FP64 direct UJ of vpmx_uj (same kernels, same
+ * semantics, params->precision = VPMX_FP64, fmm = 0, sfs = 0) on n particles
+ * already in device memory, SoA of stride n: X, Gamma (3 x n), sigma (n) in,
+ * U (3 x n), J (9 x n; J[(k + 3l) n + i] = dU_k/dx_l of particle i) out,
+ * accumulated when params->reset = 0.  It calls no allocator: its scratch is a
+ * fixed 48 MiB block of every device's memory, set up by vpmx_create, so
+ * n <= VPMX_NOALLOC_MAX_N (larger n fails with VPMX_EINVAL).  Asynchronous:
+ * enqueued on `stream` (a hipStream_t; NULL: the context's stream), results
+ * valid once that stream is synchronized.  Every call on one device shares the
+ * one block, which keeps the last call's contents: calls that overlap on one
+ * GPU (other streams, other contexts, other threads) race. */
+#define VPMX_NOALLOC_MAX_N 131072
+int vpmx_uj_device_noalloc(vpmx_ctx *ctx, int64_t n, const double *X, const double *Gamma,
+                           const double *sigma, double *U, double *J,
+                           const vpmx_uj_params *params, void *stream);
+
 /* Replaces FLOWVPM's rungekutta3(pfield, dt; relax) as selected by
  * vpm_integration (src/FLOWUnsteady_simulation.jl:40,131,242) and called
  * through vpm.nextstep (src/FLOWUnsteady_simulation.jl:358): three low-storage
