@@ -170,6 +170,33 @@ def batched_section(ctx, log, nfields=64, seed=5, reps=2, steps=2):
     diff = max(normwise(vpm.get_U(pfs[k]), u)[0] for k, u in seq_U.items())
     log(f"batched C5: one launch {bat_s:.3f} s ({pairs / bat_s:.3e} interactions/s), "
         f"max U diff vs sequential {diff:.2e}")
+    # every field's mixed U, J against its FP64 UJ (the FP64 batch of the same fields)
+    ref = []
+    for X, G, s_, v in cases:
+        pf = vpm.ParticleField(len(X), precision="fp64")
+        vpm.add_particles(pf, X, G, s_, vol=v)
+        ref.append(pf)
+    P64 = ref[0]._uj_params(fmm=False)
+    rptrs = N.pointer_array([pf.particles for pf in ref])
+    ctx.check(L.vpmx_batch_upload(ctx.ptr, len(ref), rptrs, nps, lay))
+    ctx.check(L.vpmx_uj_resident(ctx.ptr, ctypes.byref(P64)))
+    ctx.check(L.vpmx_batch_download(ctx.ptr, len(ref), rptrs, nps, lay))
+    per_field = []
+    for a_, b_ in zip(pfs, ref):
+        if b_.np < 2:
+            continue
+        eu = normwise(vpm.get_U(a_), vpm.get_U(b_))
+        ej = normwise(vpm.get_J(a_), vpm.get_J(b_))
+        per_field.append((eu[0], ej[0], eu[1], ej[1]))
+    per_field = np.array(per_field)
+    field_err = {"U": {"max_over_fields": float(per_field[:, 0].max()),
+                       "median_over_fields": float(np.median(per_field[:, 0])),
+                       "rms_max_over_fields": float(per_field[:, 2].max())},
+                 "J": {"max_over_fields": float(per_field[:, 1].max()),
+                       "median_over_fields": float(np.median(per_field[:, 1])),
+                       "rms_max_over_fields": float(per_field[:, 3].max())},
+                 "norm": "per field: max_i |d_i| / max_j |ref_j| and rms ratio; then over fields"}
+    log("batched C5 mixed vs fp64 per field:", field_err)
     # batched RK3 steps (3 UJ + fused updates), eager and replayed from a graph
     S = pfs[0]._step_params()
     step_ms = {}
@@ -212,11 +239,52 @@ def batched_section(ctx, log, nfields=64, seed=5, reps=2, steps=2):
             "batched": {"s": bat_s, "value": pairs / bat_s, "unit": "interactions/s"},
             "speedup_batched_vs_sequential": seq_s / bat_s,
             "max_U_diff_vs_sequential": diff,
+            "rel_err_mixed_vs_fp64_per_field": field_err,
             "schedule": {"kind": names[sched], "block": block, "grid": grid},
             "autotune_ms": [{"kind": names[a], "block": b, "ms": m} for a, b, m in tune],
             "rk3_step_ms": step_ms,
             "rk3_step_interactions_per_s": {k: 3 * pairs / (v / 1e3) for k, v in step_ms.items()},
             "ring8k_fp64_rk3_relax_step_ms": small}
+
+
+def ring_section(ctx, log, cpu=True, steps=20, dt=0.01):
+    """The 8,000-particle vortex ring (R = 1, a = 0.1, 40 stations x 200 per
+    cross-section on a Vogel spiral, sigma = 0.08, total circulation 1):
+    gaussian-erf, rVPM, transposed, Pedrizzetti relaxation (rlxf 0.3) every
+    step, FP64 RK3, `steps` steps of dt.  Time per step, ring speed against the
+    thin-core estimate, |Gamma| sigma^2 drift, and (cpu) the oracle's same
+    steps per particle."""
+    from flowunsteady_amd import vpm, workloads
+    X, G, s, v = workloads.vortex_ring()
+    pf = vpm.ParticleField(len(X), relaxation=vpm.pedrizzetti, device=0)
+    vpm.add_particles(pf, X, G, s, vol=v)
+    z0 = X[:, 2].mean()
+    inv0 = np.linalg.norm(G, axis=1) * s ** 2
+    vpm.nextstep(pf, dt, relax=True)                  # warm-up step, not timed
+    t0 = time.perf_counter()
+    for _ in range(steps - 1):
+        vpm.nextstep(pf, dt, relax=True)
+    ms = 1e3 * (time.perf_counter() - t0) / (steps - 1)
+    Xg, Gg, sg = vpm.get_X(pf), vpm.get_Gamma(pf), vpm.get_sigma(pf)
+    speed = (Xg[:, 2].mean() - z0) / (steps * dt)
+    kelvin = 1.0 / (4 * np.pi) * (np.log(8 / 0.1) - 0.25)
+    drift = float(np.abs(np.linalg.norm(Gg, axis=1) * sg ** 2 / inv0 - 1.0).max())
+    res = {"workload": "vortex ring N=8000 (R=1, a=0.1, sigma=0.08), gaussian-erf, rVPM, "
+                       "transposed, Pedrizzetti rlxf=0.3 every step, FP64 RK3, "
+                       f"{steps} steps dt={dt}",
+           "ms_per_step": ms, "ms_per_step_note": "host-matrix steps (upload, 4 UJ, download)",
+           "ring_speed": speed, "thin_core_speed": kelvin,
+           "abs_Gamma_sigma2_max_rel_drift": drift}
+    if cpu:
+        from oracle import oracle as O
+        P = O.make_params("gaussianerf", True, "rVPM", relax_kind=1, rlxf=0.3)
+        F = O.Field(X, G, s)
+        for _ in range(steps):
+            O.rk3_step(P, F, dt, True)
+        res["fp64_vs_oracle"] = {"X_max": normwise(Xg, F.X)[0], "Gamma_max": normwise(Gg, F.G)[0],
+                                 "sigma_max": normwise(sg[:, None], F.sigma[:, None])[0]}
+    log("ring C1:", res)
+    return res
 
 
 def fmm_section(ctx, n, orders, log, cpu=True, nref=65536):
@@ -427,10 +495,12 @@ def full_step_run(n, steps, log, precisions=("mixed", "fp64"), every=10):
     return res
 
 
-def traffic_probe(n, copy_bytes=1 << 30):
-    """One launch of the pair kernel on the bench workload and a plain device
-    copy of `copy_bytes`, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs: the
-    copy's counter readings per byte scale the pair kernel's readings to bytes."""
+def traffic_probe(n, copy_bytes=1 << 30, precision="mixed"):
+    """Launches of the pair kernel on the bench workload (mixed: a UJ and a
+    substep; fp64: one UJ) and a plain device copy of `copy_bytes`, for
+    rocprofv3 --pmc runs: FETCH_SIZE / WRITE_SIZE (the copy's counter readings
+    per byte scale the pair kernel's readings to bytes) and the SQ counters of
+    the pair kernel itself."""
     import torch
     from flowunsteady_amd import _native as N
     from flowunsteady_amd import workloads
@@ -442,9 +512,13 @@ def traffic_probe(n, copy_bytes=1 << 30):
     ctx = N.Context(0)
     fld = Field(ctx, X, G, s, v)
     fld.upload()
-    fld.uj(N.GAUSSIANERF, N.MIXED)
-    fld.substep(N.GAUSSIANERF, N.MIXED, 1e-4)
-    print(json.dumps({"traffic_probe": {"N": n, "copy_bytes": copy_bytes}}), flush=True)
+    if precision == "fp64":
+        fld.uj(N.GAUSSIANERF, N.FP64)
+    else:
+        fld.uj(N.GAUSSIANERF, N.MIXED)
+        fld.substep(N.GAUSSIANERF, N.MIXED, 1e-4)
+    print(json.dumps({"traffic_probe": {"N": n, "copy_bytes": copy_bytes,
+                                        "precision": precision}}), flush=True)
 
 
 def read_traffic(path=os.path.join(ROOT, "profiles", "uj_mixed_cube1m_traffic.md")):
@@ -474,9 +548,14 @@ def main():
                     help="only run the full rVPM step on the 2M Gaussian blob for STEPS steps "
                          "(mixed and FP64) and print its JSON")
     ap.add_argument("--full-step-n", type=int, default=1 << 21)
+    ap.add_argument("--c4-steps", type=int, default=3,
+                    help="steps of the full rVPM step (2M blob, mixed and FP64) in the default "
+                         "run (0: skip)")
+    ap.add_argument("--no-ring", action="store_true", help="skip the 8k vortex ring section")
     ap.add_argument("--full-step-precisions", type=str, default="mixed,fp64")
     ap.add_argument("--traffic-probe", action="store_true",
-                    help="only run one pair-kernel launch and a reference copy (for PMC runs)")
+                    help="only run the pair kernel and a reference copy (for PMC runs)")
+    ap.add_argument("--probe-precision", choices=("mixed", "fp64"), default="mixed")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "vpmx runs on one GPU only: the particle field is not "
@@ -484,7 +563,7 @@ def main():
         sys.exit(2)
 
     if args.traffic_probe:
-        traffic_probe(args.n)
+        traffic_probe(args.n, precision=args.probe_precision)
         return
     if args.full_step:
         log = lambda *a: print(*a, file=sys.stderr, flush=True)
@@ -600,6 +679,17 @@ def main():
             out["batched"] = batched_section(ctx, log)
         except Exception as exc:
             out["batched"] = {"error": repr(exc)}
+    if not args.no_ring:
+        try:
+            out["ring"] = ring_section(ctx, log, cpu=not args.no_cpu)
+        except Exception as exc:
+            out["ring"] = {"error": repr(exc)}
+    if args.c4_steps > 0:
+        try:
+            out["full_step"] = full_step_run(args.full_step_n, args.c4_steps, log,
+                                             every=args.c4_steps)
+        except Exception as exc:
+            out["full_step"] = {"error": repr(exc)}
     print(json.dumps(out), flush=True)
 
 

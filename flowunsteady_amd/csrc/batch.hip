@@ -108,9 +108,10 @@ __global__ __launch_bounds__(BLK) void uj_mixed_items_kernel(const BatchItem *__
 {
     __shared__ float4 tpos[BLK];
     __shared__ float4 tgam[BLK];
+    __shared__ float4 tlo[BLK];
     const BatchItem it = items[blockIdx.x];
     uj_mixed_block<K, BLK>(spos, it.t0, it.t1, perm, spos, sgam, it.s0, it.s1, ldo, U, J, reset,
-                           tpos, tgam, tbnd + 2 * it.tb, it.s0);
+                           tpos, tgam, tlo, tbnd + 2 * it.tb, it.s0);
 }
 
 // Persistent grid: each workgroup takes the next item from the queue head until
@@ -128,6 +129,7 @@ __global__ __launch_bounds__(BLK) void uj_mixed_queue_kernel(const BatchItem *__
 {
     __shared__ float4 tpos[BLK];
     __shared__ float4 tgam[BLK];
+    __shared__ float4 tlo[BLK];
     __shared__ int32_t cur;
     for (;;) {
         if (threadIdx.x == 0) cur = atomicAdd(head, 1);
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(BLK) void uj_mixed_queue_kernel(const BatchItem *__
         if (k >= nitems) break;
         const BatchItem it = items[k];
         uj_mixed_block<K, BLK>(spos, it.t0, it.t1, perm, spos, sgam, it.s0, it.s1, ldo, U, J,
-                               reset, tpos, tgam, tbnd + 2 * it.tb, it.s0);
+                               reset, tpos, tgam, tlo, tbnd + 2 * it.tb, it.s0);
     }
 }
 

@@ -401,16 +401,78 @@ __device__ __forceinline__ void mixed_pair_d(float dx, float dy, float dz, float
 // The core-size derivative of the same pair (D[U], D[J] in the U, J slots of
 // the accumulator).  Coincident pairs contribute nothing.
 template <int K>
+__device__ __forceinline__ void mixed_pair_dsig_d(float dx, float dy, float dz, float p0,
+                                                  const float4 b, MixedAcc &acc);
+template <int K>
 __device__ __forceinline__ void mixed_pair_dsig(const float4 a, const float4 b, float xi,
                                                 float yi, float zi, MixedAcc &acc)
 {
     const float dx = xi - a.x, dy = yi - a.y, dz = zi - a.z;
+    mixed_pair_dsig_d<K>(dx, dy, dz, a.w, b, acc);
+}
+
+// The same with (hi, lo) offsets of both ends: separation = hi difference +
+// lo difference (these pairs all lie within a few cores).
+template <int K>
+__device__ __forceinline__ void mixed_pair_dsig_hl(const float4 a, const float4 l, const float4 b,
+                                                   float xi, float yi, float zi, float xl,
+                                                   float yl, float zl, MixedAcc &acc)
+{
+    const float dx = (xi - a.x) + (xl - l.x), dy = (yi - a.y) + (yl - l.y),
+                dz = (zi - a.z) + (zl - l.z);
+    mixed_pair_dsig_d<K>(dx, dy, dz, a.w, b, acc);
+}
+
+template <int K>
+__device__ __forceinline__ void mixed_pair_dsig_d(float dx, float dy, float dz, float p0,
+                                                  const float4 b, MixedAcc &acc)
+{
+    const float4 a = make_float4(0.f, 0.f, 0.f, p0);
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const bool self = !(r2 > 0.f);
     float A, B;
     radial_dsig_f32<K>(r2, a.w, b.w, A, B);
     A = self ? 0.f : A;
     B = self ? 0.f : B;
+    mixed_accumulate(dx, dy, dz, A, B, b, acc);
+}
+
+// One pair with (hi, lo) FP32 offsets of both ends from the block origin:
+// hi = (float)(x - o), lo = (float)((x - o) - hi).  While every lane of the
+// wave has rho^2 >= 49 by the hi separation, the pair takes the far branch on
+// the hi separation (its relative rounding is small against r >= 7 sigma);
+// otherwise the whole wave takes the near branch on the separation hi
+// difference + lo difference, which keeps close pairs at FP32 relative
+// accuracy however far the block origin lies.  The lo record `alo` is read
+// only there.  p1 = 1/sigma^2 for every kernel.  Coincident pairs (zero
+// separation) contribute nothing.
+template <int K>
+__device__ __forceinline__ void mixed_pair_hl(const float4 a, const float4 *alo, const float4 b,
+                                              float xi, float yi, float zi, float xl, float yl,
+                                              float zl, MixedAcc &acc)
+{
+    float dx = xi - a.x, dy = yi - a.y, dz = zi - a.z;
+    float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    float A, B;
+    if (__any(r2 * b.w < RHO2_FAR)) {
+        const float4 l = *alo;
+        dx += xl - l.x;
+        dy += yl - l.y;
+        dz += zl - l.z;
+        r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const bool self = !(r2 > 0.f);
+        const float rinv = self ? 0.f : frsq(r2);
+        radial_near_f32<K>(r2, rinv, a.w, b.w, A, B);
+        A = self ? 0.f : A;
+        B = self ? 0.f : B;
+    } else if constexpr (K == WINCKELMANS) {
+        radial_near_f32<K>(r2, 0.f, a.w, b.w, A, B);   // no far-field form: full evaluation
+    } else {
+        const float rinv = frsq(r2);                   // r2 > 0: rho^2 >= 49
+        const float ri2 = rinv * rinv;
+        A = ri2 * rinv;
+        B = -3.0f * A * ri2;
+    }
     mixed_accumulate(dx, dy, dz, A, B, b, acc);
 }
 

@@ -50,7 +50,8 @@ struct Scratch {
     float4 *sgam = nullptr;          // Morton-sorted (Gx, Gy, Gz, p1) FP32
     double4 *tbnd = nullptr;         // per 256-slot tile of the sorted field:
                                      // (centre, radius), (largest sigma, -, -, -)
-    // split source ranges of small launches: partial sums (nseg x 14 x kcap)
+    // split source ranges of small launches: partial sums (up to 15 per slot
+    // per segment; kcap doubles)
     int64_t kcap = 0;
     double *kpart = nullptr;
     uint32_t *kin = nullptr, *kout = nullptr;
@@ -159,6 +160,12 @@ size_t uj_f64_dev_scratch_bytes(int64_t n);
 hipError_t launch_uj_f64_dev(hipStream_t st, int kernel, int64_t n, const double *X,
                              const double *G, const double *sigma, double *U, double *J,
                              int reset, void *scratch);
+// FP64 direct UJ of a single field on its Morton order (launch_sort_mixed's
+// perm): sorted source records into s.src64 and tile bounds into s.tbnd, then
+// the pair kernel with the far-field branch, results stored through perm.
+hipError_t launch_pack_f64_sorted(hipStream_t st, const DeviceField &f, Scratch &s);
+hipError_t launch_uj_f64_sorted(hipStream_t st, int kernel, const DeviceField &f,
+                                const Scratch &s, int reset);
 hipError_t launch_sort_pack_mixed(hipStream_t st, int kernel, const DeviceField &f, Scratch &s);
 hipError_t launch_uj_mixed(hipStream_t st, int kernel, const DeviceField &f, const Scratch &s,
                            int reset);

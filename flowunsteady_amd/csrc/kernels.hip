@@ -5,7 +5,8 @@
 //                      J = grad U); LDS-tiled sources, one target per lane.
 //   uj_mixed_kernel    all-pairs U and J with FP32 pair math on Morton-sorted
 //                      targets, coordinates relative to the block's first
-//                      target, FP32 tile partials folded into FP64 per tile.
+//                      target as FP32 (hi, lo) pairs (lo used within 7
+//                      cores), FP32 tile partials folded into FP64 per tile.
 //   estr_f64_kernel    SFS stretching Estr (docs/src/theory/rvpm.md:251-262).
 //   update_kernel      fused low-storage RK / Euler substep of the rVPM
 //                      equations (rvpm.md:107-178,196-236,364) in FP64.
@@ -177,6 +178,169 @@ hipError_t launch_uj_f64(hipStream_t st, int kernel, const DeviceField &f, const
     return uj_f64_any(st, kernel, s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset, s.kpart);
 }
 
+// ---- FP64 on the Morton-sorted field (single fields) -----------------------
+
+// Sorted source records: slot s holds particle perm[s] as
+// (x, y, z, Gx) (Gy, Gz, sigma, 1/sigma).
+__global__ __launch_bounds__(EW_BLOCK) void pack_f64_sorted_kernel(const double *__restrict__ X,
+                                                                   const double *__restrict__ G,
+                                                                   const double *__restrict__ sg,
+                                                                   const int32_t *__restrict__ perm,
+                                                                   int64_t n, int64_t cap,
+                                                                   double4 *__restrict__ out)
+{
+    const int64_t s = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = perm[s];
+    const double sig = sg[i];
+    out[2 * s] = make_double4(X[i], X[cap + i], X[2 * cap + i], G[i]);
+    out[2 * s + 1] = make_double4(G[cap + i], G[2 * cap + i], sig, 1.0 / sig);
+}
+
+// Bounds of each 256-slot tile of the sorted records: bbox centre, half
+// diagonal, largest sigma.  One workgroup per tile.
+__global__ __launch_bounds__(UJ_BLOCK) void tile_bounds_f64_kernel(const double4 *__restrict__ src,
+                                                                   int64_t n,
+                                                                   double4 *__restrict__ tb)
+{
+    __shared__ double red[7][UJ_BLOCK];
+    const int t = threadIdx.x;
+    const int64_t s = (int64_t)blockIdx.x * UJ_BLOCK + t;
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    if (s < n) {
+        const double4 p = src[2 * s];
+        v[0] = v[3] = p.x;
+        v[1] = v[4] = p.y;
+        v[2] = v[5] = p.z;
+        v[6] = src[2 * s + 1].z;
+    }
+    for (int k = 0; k < 7; ++k) red[k][t] = v[k];
+    __syncthreads();
+    for (int w = UJ_BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int k = 0; k < 3; ++k) red[k][t] = fmin(red[k][t], red[k][t + w]);
+            for (int k = 3; k < 7; ++k) red[k][t] = fmax(red[k][t], red[k][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double hx = 0.5 * (red[3][0] - red[0][0]), hy = 0.5 * (red[4][0] - red[1][0]),
+                     hz = 0.5 * (red[5][0] - red[2][0]);
+        tb[2 * blockIdx.x] = make_double4(red[0][0] + hx, red[1][0] + hy, red[2][0] + hz,
+                                          sqrt(hx * hx + hy * hy + hz * hz));
+        tb[2 * blockIdx.x + 1] = make_double4(red[6][0], 0.0, 0.0, 0.0);
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(UJ_BLOCK) void uj_f64s_kernel(const double4 *__restrict__ src,
+                                                           int64_t n,
+                                                           const double4 *__restrict__ tb,
+                                                           const int32_t *__restrict__ perm,
+                                                           int64_t cap, double *__restrict__ U,
+                                                           double *__restrict__ J, int reset)
+{
+    __shared__ double4 tile[2 * UJ_BLOCK];
+    const int64_t t0 = (int64_t)blockIdx.x * UJ_BLOCK;
+    double d[15];
+    uj_f64s_sums<K, UJ_BLOCK>(src, 0, n, t0, n, tb, tile, d);
+    const int64_t s = t0 + threadIdx.x;
+    if (s < n) uj_f64s_store(d, perm[s], cap, U, J, reset);
+}
+
+// Split source range: workgroup (block b, segment g) writes its 15 sums to
+// part[(g 15 + k) pld + slot].
+template <int K>
+__global__ __launch_bounds__(UJ_BLOCK) void uj_f64s_split_kernel(const double4 *__restrict__ src,
+                                                                 int64_t n,
+                                                                 const double4 *__restrict__ tb,
+                                                                 int64_t seglen, int nseg,
+                                                                 double *__restrict__ part,
+                                                                 int64_t pld)
+{
+    __shared__ double4 tile[2 * UJ_BLOCK];
+    const int g = (int)(blockIdx.x % nseg);
+    const int64_t t0 = (int64_t)(blockIdx.x / nseg) * UJ_BLOCK;
+    const int64_t s0 = (int64_t)g * seglen;
+    const int64_t s1 = s0 + seglen < n ? s0 + seglen : n;
+    double d[15];
+    uj_f64s_sums<K, UJ_BLOCK>(src, s0, s1, t0, n, tb, tile, d);
+    const int64_t s = t0 + threadIdx.x;
+    if (s < n)
+        for (int k = 0; k < 15; ++k) part[((int64_t)g * 15 + k) * pld + s] = d[k];
+}
+
+// Segments of each slot added in segment order (bitwise repeatable), stored
+// through perm.
+__global__ __launch_bounds__(EW_BLOCK) void uj_f64s_split_reduce_kernel(const double *__restrict__ part,
+                                                                        int64_t pld, int nseg,
+                                                                        int64_t n,
+                                                                        const int32_t *__restrict__ perm,
+                                                                        int64_t ldo,
+                                                                        double *__restrict__ U,
+                                                                        double *__restrict__ J,
+                                                                        int reset)
+{
+    const int64_t s = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    double d[15];
+    for (int k = 0; k < 15; ++k) {
+        double v = 0.0;
+        for (int g = 0; g < nseg; ++g) v += part[((int64_t)g * 15 + k) * pld + s];
+        d[k] = v;
+    }
+    uj_f64s_store(d, perm[s], ldo, U, J, reset);
+}
+
+hipError_t launch_pack_f64_sorted(hipStream_t st, const DeviceField &f, Scratch &s)
+{
+    const int64_t n = f.np;
+    if (n == 0) return hipSuccess;
+    pack_f64_sorted_kernel<<<grid_for(n, EW_BLOCK), EW_BLOCK, 0, st>>>(f.X, f.G, f.sigma, s.perm, n,
+                                                                       f.cap, s.src64);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    tile_bounds_f64_kernel<<<grid_for(n, UJ_BLOCK), UJ_BLOCK, 0, st>>>(s.src64, n, s.tbnd);
+    return hipGetLastError();
+}
+
+template <int K>
+static hipError_t uj_f64s_go(hipStream_t st, const DeviceField &f, const Scratch &s, int reset)
+{
+    const int64_t n = f.np;
+    const int64_t nb = (n + UJ_BLOCK - 1) / UJ_BLOCK;
+    const Split sp = split_for(nb, n);
+    if (sp.nseg == 1 || !s.kpart || (int64_t)sp.nseg * 15 * n > s.kcap) {
+        uj_f64s_kernel<K><<<(unsigned)nb, UJ_BLOCK, 0, st>>>(s.src64, n, s.tbnd, s.perm, f.cap,
+                                                             f.U, f.J, reset);
+        return hipGetLastError();
+    }
+    uj_f64s_split_kernel<K><<<(unsigned)(nb * sp.nseg), UJ_BLOCK, 0, st>>>(
+        s.src64, n, s.tbnd, sp.seglen, sp.nseg, s.kpart, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    uj_f64s_split_reduce_kernel<<<grid_for(n, EW_BLOCK), EW_BLOCK, 0, st>>>(
+        s.kpart, n, sp.nseg, n, s.perm, f.cap, f.U, f.J, reset);
+    return hipGetLastError();
+}
+
+hipError_t launch_uj_f64_sorted(hipStream_t st, int kernel, const DeviceField &f,
+                                const Scratch &s, int reset)
+{
+    if (f.np == 0) return hipSuccess;
+    switch (kernel) {
+    case SINGULAR: return uj_f64s_go<SINGULAR>(st, f, s, reset);
+    case GAUSSIAN: return uj_f64s_go<GAUSSIAN>(st, f, s, reset);
+    case GAUSSIANERF: return uj_f64s_go<GAUSSIANERF>(st, f, s, reset);
+    case WINCKELMANS:
+        // no far-field form (1 - g decays as rho^-4): the unsorted-target
+        // kernel over the sorted sources keeps its higher occupancy
+        return uj_f64_go<WINCKELMANS>(st, s.src64, f.np, f.X, f.np, f.cap, f.U, f.J, reset,
+                                      s.kpart);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 // Caller-owned device arrays (stride n) and a caller-provided scratch block:
 // the source records (2n double4) first, then the split partials when the
 // launch is split (12 doubles per target per segment).
@@ -279,7 +443,7 @@ __global__ __launch_bounds__(EW_BLOCK) void morton_kernel(const double *__restri
 }
 
 // Per-source constants of the FP32 radial functions: gaussian(-erf)
-// p0 = 1/sigma, p1 = 1/sigma^2; winckelmans p0 = sigma^2.
+// p0 = 1/sigma; winckelmans p0 = sigma^2; p1 = 1/sigma^2 for every kernel.
 template <int K>
 __global__ __launch_bounds__(EW_BLOCK) void pack_mixed_kernel(const double *__restrict__ X,
                                                               const double *__restrict__ G,
@@ -293,10 +457,10 @@ __global__ __launch_bounds__(EW_BLOCK) void pack_mixed_kernel(const double *__re
     if (s >= n) return;
     const int64_t i = perm[s];
     const double sig = sg[i];
-    double p0 = 0.0, p1 = 0.0;
+    double p0 = 0.0;
+    const double p1 = 1.0 / (sig * sig);
     if constexpr (K == GAUSSIANERF || K == GAUSSIAN) {
         p0 = 1.0 / sig;
-        p1 = p0 * p0;
     } else if constexpr (K == WINCKELMANS) {
         p0 = sig * sig;
     }
@@ -317,9 +481,10 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_kernel(const double4 *__res
 {
     __shared__ float4 tpos[UJ_BLOCK];
     __shared__ float4 tgam[UJ_BLOCK];
+    __shared__ float4 tlo[UJ_BLOCK];
     const int64_t t0 = (int64_t)blockIdx.x * UJ_BLOCK;
     uj_mixed_block<K, UJ_BLOCK, DS>(tgt, t0, nt, perm, spos, sgam, 0, n, cap, U, J, reset, tpos,
-                                    tgam, tb);
+                                    tgam, tlo, tb);
 }
 
 // Split source range of the mixed kernel: workgroup (block b, segment g) writes
@@ -337,12 +502,13 @@ __global__ __launch_bounds__(UJ_BLOCK) void uj_mixed_split_kernel(const double4 
 {
     __shared__ float4 tpos[UJ_BLOCK];
     __shared__ float4 tgam[UJ_BLOCK];
+    __shared__ float4 tlo[UJ_BLOCK];
     const int g = (int)(blockIdx.x % nseg);
     const int64_t t0 = (int64_t)(blockIdx.x / nseg) * UJ_BLOCK;
     const int64_t s0 = (int64_t)g * seglen;
     const int64_t s1 = s0 + seglen < n ? s0 + seglen : n;
     double d[14];
-    uj_mixed_sums<K, UJ_BLOCK, DS>(tgt, t0, nt, spos, sgam, s0, s1, tb, 0, tpos, tgam, d);
+    uj_mixed_sums<K, UJ_BLOCK, DS>(tgt, t0, nt, spos, sgam, s0, s1, tb, 0, tpos, tgam, tlo, d);
     const int64_t s = t0 + threadIdx.x;
     if (s < nt)
         for (int k = 0; k < 14; ++k) part[((int64_t)g * 14 + k) * pld + s] = d[k];

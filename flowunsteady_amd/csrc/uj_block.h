@@ -81,6 +81,154 @@ __device__ __forceinline__ void uj_f64_sums(const double4 *__restrict__ src, int
     d[9] = j6; d[10] = j7; d[11] = j8;
 }
 
+// ---------------------------------------------------------------------------
+// FP64 on the Morton-sorted field.  Sources and targets are slots of the
+// sorted records (x, y, z, Gx) (Gy, Gz, sigma, 1/sigma) (pack_f64_sorted);
+// results go through perm.  The 15 sums are kept without the -1/(4 pi)
+// factor: d = (u 0..2, w = sum g r^-3 Gamma 0..2, radial J 0..8 (k + 3 l)).
+//
+// Far field: for the gaussian kernels 1 - g(rho) < 3e-19 and rho g'(rho)
+// < 2e-17 at rho >= 9.5, so g = 1 and g' = 0 to FP64 rounding there and the
+// pair is the singular one: no erf / exp / sqrt / division.  A source tile
+// whose gap to the target block is >= 9.5 sigma_max (> 0 for the singular
+// kernel) runs f64_pair_far for every pair; within other tiles a wave whose
+// every lane has rho^2 >= 90.25 for the current source takes it too.
+constexpr double RHO2_FAR_F64 = 90.25;
+constexpr double RHO_FAR_F64 = 9.5;
+
+__device__ __forceinline__ void f64_pair_far(const double4 a, const double4 b, double xi,
+                                             double yi, double zi, double d[15])
+{
+    const double dX1 = xi - a.x, dX2 = yi - a.y, dX3 = zi - a.z;
+    const double r2 = dX1 * dX1 + dX2 * dX2 + dX3 * dX3;
+    const double rinv = rsqrt(r2);
+    const double ri2 = rinv * rinv, r3i = ri2 * rinv;
+    const double G1 = a.w, G2 = b.x, G3 = b.y;
+    const double c1 = r3i * (dX2 * G3 - dX3 * G2);
+    const double c2 = r3i * (dX3 * G1 - dX1 * G3);
+    const double c3 = r3i * (dX1 * G2 - dX2 * G1);
+    d[0] += c1; d[1] += c2; d[2] += c3;
+    d[3] += r3i * G1; d[4] += r3i * G2; d[5] += r3i * G3;
+    const double aux = -3.0 * ri2;
+    const double e1 = aux * c1, e2 = aux * c2, e3 = aux * c3;
+    d[6] += e1 * dX1; d[7] += e2 * dX1; d[8] += e3 * dX1;
+    d[9] += e1 * dX2; d[10] += e2 * dX2; d[11] += e3 * dX2;
+    d[12] += e1 * dX3; d[13] += e2 * dX3; d[14] += e3 * dX3;
+}
+
+// Full evaluation (reference pair algebra, src/FLOWUnsteady_processing_force.jl:888-905
+// extended to J); coincident pairs skipped.
+template <int K>
+__device__ __forceinline__ void f64_pair_full(const double4 a, const double4 b, double xi,
+                                              double yi, double zi, double d[15])
+{
+    const double dX1 = xi - a.x, dX2 = yi - a.y, dX3 = zi - a.z;
+    const double r2 = dX1 * dX1 + dX2 * dX2 + dX3 * dX3;
+    if (!(r2 > 0.0)) return;   // the reference skips r == 0
+    const double r = sqrt(r2);
+    double g, dg;
+    g_dg_f64<K>(r * b.w, g, dg);
+    const double rinv = 1.0 / r;
+    const double ri2 = rinv * rinv, r3i = ri2 * rinv;
+    const double G1 = a.w, G2 = b.x, G3 = b.y;
+    const double c1 = r3i * (dX2 * G3 - dX3 * G2);
+    const double c2 = r3i * (dX3 * G1 - dX1 * G3);
+    const double c3 = r3i * (dX1 * G2 - dX2 * G1);
+    d[0] += g * c1; d[1] += g * c2; d[2] += g * c3;
+    const double gr = g * r3i;
+    d[3] += gr * G1; d[4] += gr * G2; d[5] += gr * G3;
+    const double aux = dg * b.w * rinv - 3.0 * g * ri2;
+    const double e1 = aux * c1, e2 = aux * c2, e3 = aux * c3;
+    d[6] += e1 * dX1; d[7] += e2 * dX1; d[8] += e3 * dX1;
+    d[9] += e1 * dX2; d[10] += e2 * dX2; d[11] += e3 * dX2;
+    d[12] += e1 * dX3; d[13] += e2 * dX3; d[14] += e3 * dX3;
+}
+
+// One target block (slots [t0, t1), t0 a multiple of BLK) against source
+// slots [s0, s1) (s0 a multiple of BLK).  tb: bounds of each BLK-slot tile
+// (centre, radius), (largest sigma, -, -, -).  `tile` holds 2 BLK records.
+template <int K, int BLK>
+__device__ __forceinline__ void uj_f64s_sums(const double4 *__restrict__ src, int64_t s0,
+                                             int64_t s1, int64_t t0, int64_t t1,
+                                             const double4 *__restrict__ tb, double4 *tile,
+                                             double d[15])
+{
+    const int t = threadIdx.x;
+    const int64_t s = t0 + t;
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    if (s < t1) {
+        const double4 p = src[2 * s];
+        xi = p.x;
+        yi = p.y;
+        zi = p.z;
+    }
+    const double4 T = tb[2 * (t0 / BLK)];
+    for (int k = 0; k < 15; ++k) d[k] = 0.0;
+    for (int64_t base = s0; base < s1; base += BLK) {
+        __syncthreads();
+        const int64_t js = base + t;
+        if (js < s1) {
+            tile[2 * t] = src[2 * js];
+            tile[2 * t + 1] = src[2 * js + 1];
+        }
+        __syncthreads();
+        const int cnt = (int)((s1 - base) < BLK ? (s1 - base) : BLK);
+        bool far = false;
+        if constexpr (K == GAUSSIANERF || K == GAUSSIAN || K == SINGULAR) {
+            if (cnt == BLK) {
+                const int64_t ts = base / BLK;
+                const double4 S = tb[2 * ts];
+                const double ex = S.x - T.x, ey = S.y - T.y, ez = S.z - T.z;
+                const double gap = sqrt(ex * ex + ey * ey + ez * ez) - S.w - T.w;
+                if constexpr (K == SINGULAR)
+                    far = gap > 0.0;
+                else
+                    far = gap >= RHO_FAR_F64 * tb[2 * ts + 1].x;
+            }
+        }
+        if (far) {
+#pragma unroll 2
+            for (int jj = 0; jj < BLK; ++jj)
+                f64_pair_far(tile[2 * jj], tile[2 * jj + 1], xi, yi, zi, d);
+        } else {
+            for (int jj = 0; jj < cnt; ++jj) {
+                const double4 a = tile[2 * jj], b = tile[2 * jj + 1];
+                bool lane_far = false;
+                if constexpr (K == GAUSSIANERF || K == GAUSSIAN) {
+                    const double dX1 = xi - a.x, dX2 = yi - a.y, dX3 = zi - a.z;
+                    const double rho2 = (dX1 * dX1 + dX2 * dX2 + dX3 * dX3) * (b.w * b.w);
+                    lane_far = rho2 >= RHO2_FAR_F64;
+                }
+                if (__all(lane_far))
+                    f64_pair_far(a, b, xi, yi, zi, d);
+                else
+                    f64_pair_full<K>(a, b, xi, yi, zi, d);
+            }
+        }
+    }
+}
+
+// U and J of particle i (stride ldo) from the 15 sums of uj_f64s_sums.
+__device__ __forceinline__ void uj_f64s_store(const double d[15], int64_t i, int64_t ldo,
+                                              double *__restrict__ U, double *__restrict__ J,
+                                              int reset)
+{
+    const double k = -INV4PI;
+    double v[12];
+    v[0] = k * d[0]; v[1] = k * d[1]; v[2] = k * d[2];
+    v[3] = k * d[6];
+    v[4] = k * (d[7] - d[5]);
+    v[5] = k * (d[8] + d[4]);
+    v[6] = k * (d[9] + d[5]);
+    v[7] = k * d[10];
+    v[8] = k * (d[11] - d[3]);
+    v[9] = k * (d[12] - d[4]);
+    v[10] = k * (d[13] + d[3]);
+    v[11] = k * d[14];
+    for (int q = 0; q < 3; ++q) U[q * ldo + i] = reset ? v[q] : U[q * ldo + i] + v[q];
+    for (int q = 0; q < 9; ++q) J[q * ldo + i] = reset ? v[3 + q] : J[q * ldo + i] + v[3 + q];
+}
+
 // U and J of target i (stride ldo) from the 12 FP64 sums.
 __device__ __forceinline__ void uj_f64_store(const double d[12], int64_t i, int64_t ldo,
                                              double *__restrict__ U, double *__restrict__ J,
@@ -126,18 +274,22 @@ __device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, i
                                               const float4 *__restrict__ sgam, int64_t s0,
                                               int64_t s1, const double4 *__restrict__ tb,
                                               int64_t torg, float4 *tpos, float4 *tgam,
-                                              double d[14])
+                                              float4 *tlo, double d[14])
 {
     const int t = threadIdx.x;
     const int64_t s = t0 + t;
     const bool active = s < t1;
     const double4 o = tgt[t0];
-    float xi = 0.f, yi = 0.f, zi = 0.f;
+    float xi = 0.f, yi = 0.f, zi = 0.f, xl = 0.f, yl = 0.f, zl = 0.f;
     if (active) {
         const double4 p = tgt[s];
-        xi = (float)(p.x - o.x);
-        yi = (float)(p.y - o.y);
-        zi = (float)(p.z - o.z);
+        const double ox = p.x - o.x, oy = p.y - o.y, oz = p.z - o.z;
+        xi = (float)ox;
+        yi = (float)oy;
+        zi = (float)oz;
+        xl = (float)(ox - (double)xi);
+        yl = (float)(oy - (double)yi);
+        zl = (float)(oz - (double)zi);
     }
     double4 T = make_double4(0.0, 0.0, 0.0, 0.0);
     if (tb) T = tb[2 * ((t0 - torg) / BLK)];
@@ -148,8 +300,11 @@ __device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, i
         const int64_t js = base + t;
         if (js < s1) {
             const double4 p = spos[js];
-            tpos[t] = make_float4((float)(p.x - o.x), (float)(p.y - o.y), (float)(p.z - o.z),
-                                  (float)p.w);
+            const double ox = p.x - o.x, oy = p.y - o.y, oz = p.z - o.z;
+            const float hx = (float)ox, hy = (float)oy, hz = (float)oz;
+            tpos[t] = make_float4(hx, hy, hz, (float)p.w);
+            tlo[t] = make_float4((float)(ox - (double)hx), (float)(oy - (double)hy),
+                                 (float)(oz - (double)hz), 0.f);
             tgam[t] = sgam[js];
         }
         __syncthreads();
@@ -173,10 +328,12 @@ __device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, i
                 if (cnt == BLK) {
 #pragma unroll 4
                     for (int jj = 0; jj < BLK; ++jj)
-                        mixed_pair_dsig<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
+                        mixed_pair_dsig_hl<K>(tpos[jj], tlo[jj], tgam[jj], xi, yi, zi, xl, yl,
+                                              zl, acc);
                 } else {
                     for (int jj = 0; jj < cnt; ++jj)
-                        mixed_pair_dsig<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
+                        mixed_pair_dsig_hl<K>(tpos[jj], tlo[jj], tgam[jj], xi, yi, zi, xl, yl,
+                                              zl, acc);
                 }
             }
         } else if (far) {
@@ -186,10 +343,10 @@ __device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, i
         } else if (cnt == BLK) {
 #pragma unroll 4
             for (int jj = 0; jj < BLK; ++jj)
-                mixed_pair<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
+                mixed_pair_hl<K>(tpos[jj], &tlo[jj], tgam[jj], xi, yi, zi, xl, yl, zl, acc);
         } else {
             for (int jj = 0; jj < cnt; ++jj)
-                mixed_pair<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
+                mixed_pair_hl<K>(tpos[jj], &tlo[jj], tgam[jj], xi, yi, zi, xl, yl, zl, acc);
         }
         d[0] += acc.U0; d[1] += acc.U1; d[2] += acc.U2;
         d[3] += acc.W0; d[4] += acc.W1; d[5] += acc.W2;
@@ -230,11 +387,12 @@ __device__ __forceinline__ void uj_mixed_block(const double4 *__restrict__ tgt, 
                                                int64_t s1, int64_t ldo,
                                                double *__restrict__ U, double *__restrict__ J,
                                                int reset, float4 *tpos, float4 *tgam,
+                                               float4 *tlo,
                                                const double4 *__restrict__ tb = nullptr,
                                                int64_t torg = 0)
 {
     double d[14];
-    uj_mixed_sums<K, BLK, DS>(tgt, t0, t1, spos, sgam, s0, s1, tb, torg, tpos, tgam, d);
+    uj_mixed_sums<K, BLK, DS>(tgt, t0, t1, spos, sgam, s0, s1, tb, torg, tpos, tgam, tlo, d);
     const int64_t s = t0 + threadIdx.x;
     if (s < t1) uj_mixed_store(d, perm[s], ldo, U, J, reset);
 }

@@ -207,7 +207,7 @@ int ensure_capacity(vpmx_ctx *c, int64_t np)
     VPMX_TRY(dalloc(c, &s.sgam, cap));
     VPMX_TRY(dalloc(c, &s.tbnd, 2 * ((cap + 255) / 256)));
     if (!s.kpart) {
-        s.kcap = 14 * 256 * 1536;   // the most a split launch needs (see split_for)
+        s.kcap = 15 * 256 * 1536;   // the most a split launch needs (see split_for)
         VPMX_TRY(dalloc(c, &s.kpart, s.kcap));
     }
     VPMX_TRY(dalloc(c, &s.kin, cap));
@@ -940,13 +940,15 @@ int enqueue_uj(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool sort = true
     }
     if (B.nfields && (B.items_block != B.block || B.items_off != B.off))
         return fail(c, VPMX_EINVAL, "batch work list not prepared");
-    if (P->precision == VPMX_FP64) {
+    if (P->precision == VPMX_FP64 && B.nfields) {
         VPMX_HIP(c, launch_pack_f64(c->stream, f, c->s));
         TimedScope ts(c, T_UJ);
-        if (B.nfields)
-            VPMX_HIP(c, launch_batch_uj_f64(c->stream, P->kernel, f, c->s, B, reset));
-        else
-            VPMX_HIP(c, launch_uj_f64(c->stream, P->kernel, f, c->s, reset));
+        VPMX_HIP(c, launch_batch_uj_f64(c->stream, P->kernel, f, c->s, B, reset));
+    } else if (P->precision == VPMX_FP64) {
+        if (sort) VPMX_TRY(enqueue_sort(c));
+        VPMX_HIP(c, launch_pack_f64_sorted(c->stream, f, c->s));
+        TimedScope ts(c, T_UJ);
+        VPMX_HIP(c, launch_uj_f64_sorted(c->stream, P->kernel, f, c->s, reset));
     } else {
         if (sort) VPMX_TRY(enqueue_sort(c));
         VPMX_HIP(c, launch_pack_mixed(c->stream, P->kernel, f, c->s));
@@ -1186,7 +1188,7 @@ int run_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S,
         if (!isfinite(c->h_small[k])) return fail(c, VPMX_EINVAL, "freestream is not finite");
     VPMX_HIP(c, hipMemcpyAsync(B.uinf, c->h_small, (size_t)3 * nf * sizeof(double),
                                hipMemcpyHostToDevice, c->stream));
-    if (!U->fmm && U->precision == VPMX_MIXED) VPMX_TRY(enqueue_sort(c));
+    if (!U->fmm && (U->precision == VPMX_MIXED || !B.nfields)) VPMX_TRY(enqueue_sort(c));
     if (S->viscous == VPMX_VISCOUS_CORESPREADING) {   // host-driven adaptation after the step
         VPMX_TRY(enqueue_step(c, U, S, dt, relax, rk3));
         return core_spreading_reset(c, U->kernel, S, false, nullptr, nullptr, nullptr);

@@ -102,12 +102,23 @@ kernel_id(k) = k === vpm.gaussianerf ? 2 : k === vpm.winckelmans ? 3 :
 
 prop(x, name, default) = hasproperty(x, name) ? getproperty(x, name) : default
 
-# vpm.FMM(; p, ncrit, theta, nonzero_sigma) (src/FLOWUnsteady_simulation.jl:43)
+# vpm.FMM(; p, ncrit, theta, nonzero_sigma, ε_tol) (src/FLOWUnsteady_simulation.jl:43)
 sigma_cut(fmm) = prop(fmm, :nonzero_sigma, false) ? -1.0 : 0.0
+
+# Expansion order: p, or with ε_tol set the smallest q <= 12 whose leading
+# truncation term theta^(q+1) is at most ε_tol (as flowunsteady_amd.vpm.FMM.order).
+function fmm_order(fmm)
+    tol = prop(fmm, :ε_tol, nothing)
+    tol === nothing && return Int32(fmm.p)
+    for q in 0:12
+        fmm.theta^(q + 1) <= tol && return Int32(q)
+    end
+    return Int32(12)
+end
 
 uj_params(pf, fmm; reset=true, sfs=false, transposed=pf.transposed) =
     UJParams(kernel_id(pf.kernel), precision[], transposed, reset, sfs, fmm,
-             pf.fmm.p, pf.fmm.ncrit, pf.fmm.theta, sigma_cut(pf.fmm))
+             fmm_order(pf.fmm), pf.fmm.ncrit, pf.fmm.theta, sigma_cut(pf.fmm))
 
 # SFS models (src/FLOWUnsteady_simulation.jl:39,130): none, a constant
 # coefficient, or the dynamic procedure (vpm.DynamicSFS(Estr, pseudo3level_positive;

@@ -1,6 +1,7 @@
 """Host logic of the probe evaluation (no GPU): the method of images of
 Vvpm_on_Xs (src/FLOWUnsteady_simulation.jl:518-532)."""
 import numpy as np
+import pytest
 
 from flowunsteady_amd import vpm
 from helpers import make_field
@@ -29,3 +30,21 @@ def test_mirror_particles_formula():
         np.testing.assert_allclose(Gm[i], ref, atol=1e-15)
     assert np.all(pf.particles[vpm.STATIC_INDEX, 20:40] == 1.0)
     np.testing.assert_array_equal(vpm.get_sigma(pf)[20:], s)
+
+
+def test_fmm_eps_tol_picks_the_order():
+    """vpm.FMM(eps_tol=...) (the ε_tol of src/FLOWUnsteady_simulation.jl:43):
+    the smallest order q whose leading truncation term theta^(q+1) meets the
+    tolerance; None keeps p; the choice reaches the UJ parameters."""
+    from flowunsteady_amd import vpm
+    assert vpm.FMM(p=7).order() == 7
+    assert vpm.FMM(theta=0.5, eps_tol=0.5 ** 9).order() == 8
+    assert vpm.FMM(theta=0.5, eps_tol=0.5 ** 9 * 1.01).order() == 8
+    assert vpm.FMM(theta=0.5, eps_tol=0.5 ** 9 * 0.99).order() == 9
+    assert vpm.FMM(theta=0.4, eps_tol=1e-30).order() == vpm.FMM.MAX_P
+    assert vpm.FMM(theta=0.4, eps_tol=1.0).order() == 0
+    pf = vpm.ParticleField(4, fmm=vpm.FMM(theta=0.4, eps_tol=1e-4))
+    q = pf._uj_params(fmm=True).p
+    assert 0.4 ** (q + 1) <= 1e-4 < 0.4 ** q
+    with pytest.raises(ValueError):
+        vpm.FMM(eps_tol=-1.0).order()
