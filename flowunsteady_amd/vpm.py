@@ -669,8 +669,9 @@ def monitor_enstrophy(pfield: ParticleField) -> float:
 def save(pfield: ParticleField, file_name: str, *, path: str = "", add_num: bool = True,
          num=None, overwrite_time=None) -> str:
     """Write the field for visualisation and restart (FLOWVPM's `vpm.save`,
-    called at src/FLOWUnsteady_simulation.jl:438): XDMF + raw binary + a JSON
-    sidecar (pfield_io module); returns the XDMF file name."""
+    called at src/FLOWUnsteady_simulation.jl:438): an HDF5 file of FLOWVPM's
+    datasets and its XDMF descriptor (pfield_io module); returns the XDMF
+    file name."""
     from . import pfield_io
     return pfield_io.save(pfield, file_name, path=path, add_num=add_num, num=num,
                           overwrite_time=overwrite_time)
