@@ -72,6 +72,18 @@ __device__ __forceinline__ double zeta_f64(double rho)
     }
 }
 
+// eta(rho) = -zeta'(rho)/rho, the PSE kernel (second moment 1 for every
+// smooth kernel).
+template <int K>
+__device__ __forceinline__ double eta_f64(double rho)
+{
+    const double z = zeta_f64<K>(rho);
+    if constexpr (K == GAUSSIANERF) return z;
+    else if constexpr (K == GAUSSIAN) return 3.0 * rho * z;
+    else if constexpr (K == WINCKELMANS) return 7.0 * z / (rho * rho + 1.0);
+    else return 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // Core-size derivatives for the dynamic SFS procedure (docs/src/theory/rvpm.md:276-290),
 // every core scaled together (D = sigma d/dsigma):

@@ -209,6 +209,16 @@ struct FmmState {
 hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int transposed,
                            int precision, const DeviceField &f, Scratch &s,
                            const double *DJ = nullptr, double *DE = nullptr);
+// Viscous sums over the near-field lists of the last FMM evaluation (FP64,
+// outputs overwritten, natural particle order): the vorticity
+// w = sum_j G_j zeta_s(x - x_j) (s = sigma_j, or s0 > 0 for every source;
+// sources in `mask` without strength; G and w of strides gld and wld) and
+// the PSE rate into f.V.  `rec` holds 2 records per particle.
+hipError_t launch_fmm_zeta(hipStream_t st, const FmmState &S, int kernel, const DeviceField &f,
+                           const Scratch &s, const double *G, int64_t gld, double s0,
+                           const uint8_t *mask, double4 *rec, double *w, int64_t wld);
+hipError_t launch_fmm_pse(hipStream_t st, const FmmState &S, int kernel, const DeviceField &f,
+                          const Scratch &s, double nu, double4 *rec);
 // Core-size derivatives D[U] (3 x cap) and D[J] (9 x cap) over the near-field
 // lists of the last FMM evaluation (overwritten; the far field has none).
 hipError_t launch_fmm_dsig(hipStream_t st, const FmmState &S, int kernel, int precision,

@@ -1849,6 +1849,161 @@ hipError_t launch_fmm_dsig(hipStream_t st, const FmmState &S, int kernel, int pr
 }
 
 // ---------------------------------------------------------------------------
+// Viscous sums over the near-field work items (FP64)
+// ---------------------------------------------------------------------------
+// The FMM leaves zeta (and PSE's eta = -zeta'/rho, both decaying within a few
+// cores for the gaussian kernels) to its near field, as FLOWVPM's zeta_fmm
+// does (examples/rotorhover/rotorhover.jl:170): these run over the work items
+// of the last FMM evaluation's P2P lists, on records in its sorted order.
+//   PSE = false: w_i = sum_j Gamma_j zeta(r / s_j) / s_j^3 (coincident pairs
+//                included); records (x, y, z, 1/s) (Gx, Gy, Gz, 1/s^3), masked
+//                sources with no strength.
+//   PSE = true:  V_i = sum_j (2 nu / s^2) (vol_i Gamma_j - vol_j Gamma_i)
+//                eta(r / s) / s^3, s^2 = (sigma_i^2 + sigma_j^2) / 2, j != i;
+//                records (x, y, z, sigma^2) (Gx, Gy, Gz, vol).
+template <bool PSE>
+__global__ __launch_bounds__(P2P_BLOCK) void pack_visc_kernel(
+    const double *__restrict__ X, const double *__restrict__ G, int64_t gld,
+    const double *__restrict__ sg, double s0, const double *__restrict__ vol,
+    const uint8_t *__restrict__ mask, const int32_t *__restrict__ perm, int64_t n, int64_t cap,
+    double4 *__restrict__ rec)
+{
+    const int64_t s = (int64_t)blockIdx.x * P2P_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = perm[s];
+    const double x = X[i], y = X[cap + i], z = X[2 * cap + i];
+    if constexpr (PSE) {
+        const double sig = sg[i];
+        rec[2 * s] = make_double4(x, y, z, sig * sig);
+        rec[2 * s + 1] = make_double4(G[i], G[gld + i], G[2 * gld + i], vol[i]);
+    } else {
+        const double is = 1.0 / (s0 > 0.0 ? s0 : sg[i]);
+        const bool off = mask && mask[i];
+        rec[2 * s] = make_double4(x, y, z, is);
+        rec[2 * s + 1] = make_double4(off ? 0.0 : G[i], off ? 0.0 : G[gld + i],
+                                      off ? 0.0 : G[2 * gld + i], is * is * is);
+    }
+}
+
+template <int KK, bool PSE>
+__global__ __launch_bounds__(P2P_BLOCK) void near_visc_kernel(
+    const P2PChunk *__restrict__ chunks, const FmmCellDev *__restrict__ cells,
+    const int32_t *__restrict__ src, const double4 *__restrict__ rec,
+    const int32_t *__restrict__ perm, double nu, int64_t ld, double *__restrict__ out,
+    double *__restrict__ part, int64_t pstride)
+{
+    __shared__ double4 tile[2 * P2P_BLOCK];
+    const P2PChunk ch = chunks[blockIdx.x];
+    const FmmCellDev A = cells[ch.c];
+    for (int tb = A.begin; tb < A.end; tb += P2P_BLOCK) {
+        const int it = tb + threadIdx.x;
+        const bool active = it < A.end;
+        double4 a = make_double4(0.0, 0.0, 0.0, 1.0), b = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (active) {
+            a = rec[2 * it];
+            b = rec[2 * it + 1];
+        }
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        for (int64_t e = ch.e0; e < ch.e1; e++) {
+            const FmmCellDev B = cells[src[e]];
+            for (int sb = B.begin; sb < B.end; sb += P2P_BLOCK) {
+                __syncthreads();
+                const int js = sb + threadIdx.x;
+                if (js < B.end) {
+                    tile[2 * threadIdx.x] = rec[2 * js];
+                    tile[2 * threadIdx.x + 1] = rec[2 * js + 1];
+                }
+                __syncthreads();
+                const int cnt = min(P2P_BLOCK, B.end - sb);
+                for (int jj = 0; jj < cnt; ++jj) {
+                    const double4 p = tile[2 * jj], q = tile[2 * jj + 1];
+                    const double d0 = a.x - p.x, d1 = a.y - p.y, d2 = a.z - p.z;
+                    const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                    if constexpr (PSE) {
+                        if (sb + jj == it) continue;
+                        const double s2 = 0.5 * (a.w + p.w);
+                        const double sq = sqrt(s2);
+                        const double w = 2.0 * nu / s2 * eta_f64<KK>(r / sq) / (s2 * sq);
+                        v0 += w * (b.w * q.x - q.w * b.x);
+                        v1 += w * (b.w * q.y - q.w * b.y);
+                        v2 += w * (b.w * q.z - q.w * b.z);
+                    } else {
+                        const double z = zeta_f64<KK>(r * p.w) * q.w;
+                        v0 += z * q.x;
+                        v1 += z * q.y;
+                        v2 += z * q.z;
+                    }
+                }
+            }
+        }
+        if (active) {
+            if (ch.part < 0) {
+                const int64_t i = perm[it];
+                out[i] = v0;
+                out[ld + i] = v1;
+                out[2 * ld + i] = v2;
+            } else {
+                const int64_t o = ch.part + (it - A.begin);
+                part[o] = v0;
+                part[pstride + o] = v1;
+                part[2 * pstride + o] = v2;
+            }
+        }
+    }
+}
+
+template <bool PSE>
+static hipError_t near_visc_go(hipStream_t st, const FmmState &S, int kernel,
+                               const DeviceField &f, const Scratch &s, const double *G,
+                               int64_t gld, double s0, const uint8_t *mask, double nu,
+                               double4 *rec, double *out, int64_t ld)
+{
+    const int64_t n = f.np;
+    if (n == 0) return hipSuccess;
+    pack_visc_kernel<PSE><<<gridn(n, P2P_BLOCK), P2P_BLOCK, 0, st>>>(
+        f.X, G, gld, f.sigma, s0, f.vol, mask, s.perm, n, f.cap, rec);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t nleaf = (int64_t)S.plan.leaves.size();
+    if (nleaf == 0 || S.nchunk == 0) return hipSuccess;
+    const unsigned g = (unsigned)S.nchunk;
+    switch (kernel) {
+    case GAUSSIAN:
+        near_visc_kernel<GAUSSIAN, PSE><<<g, P2P_BLOCK, 0, st>>>(
+            S.chunks, S.cells, S.p2p_src, rec, s.perm, nu, ld, out, S.part, S.nslot);
+        break;
+    case GAUSSIANERF:
+        near_visc_kernel<GAUSSIANERF, PSE><<<g, P2P_BLOCK, 0, st>>>(
+            S.chunks, S.cells, S.p2p_src, rec, s.perm, nu, ld, out, S.part, S.nslot);
+        break;
+    case WINCKELMANS:
+        near_visc_kernel<WINCKELMANS, PSE><<<g, P2P_BLOCK, 0, st>>>(
+            S.chunks, S.cells, S.p2p_src, rec, s.perm, nu, ld, out, S.part, S.nslot);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // leaves cut into several items: their partials added in item order
+    return launch_part_reduce<2>(st, S.leaves, nleaf, S.chunk_off, S.part_base, S.cells, S.part,
+                                 S.nslot, s.perm, ld, nullptr, nullptr, out);
+}
+
+hipError_t launch_fmm_zeta(hipStream_t st, const FmmState &S, int kernel, const DeviceField &f,
+                           const Scratch &s, const double *G, int64_t gld, double s0,
+                           const uint8_t *mask, double4 *rec, double *w, int64_t wld)
+{
+    return near_visc_go<false>(st, S, kernel, f, s, G, gld, s0, mask, 0.0, rec, w, wld);
+}
+
+hipError_t launch_fmm_pse(hipStream_t st, const FmmState &S, int kernel, const DeviceField &f,
+                          const Scratch &s, double nu, double4 *rec)
+{
+    return near_visc_go<true>(st, S, kernel, f, s, f.G, f.cap, 0.0, nullptr, nu, rec, f.V, f.cap);
+}
+
+// ---------------------------------------------------------------------------
 // Device dual tree traversal
 // ---------------------------------------------------------------------------
 // Level-synchronous: every frontier pair (target a, source b) is classified by

@@ -26,16 +26,6 @@ constexpr int VRED = 512;    // partial-sum blocks of the reductions
 
 inline unsigned vgrid(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-template <int K>
-__device__ __forceinline__ double eta_f64(double rho)
-{
-    const double z = zeta_f64<K>(rho);
-    if constexpr (K == GAUSSIANERF) return z;
-    else if constexpr (K == GAUSSIAN) return 3.0 * rho * z;
-    else if constexpr (K == WINCKELMANS) return 7.0 * z / (rho * rho + 1.0);
-    else return 0.0;
-}
-
 // Source records for PSE: (x, y, z, sigma^2) (Gx, Gy, Gz, vol).
 __global__ __launch_bounds__(VB) void pack_pse_kernel(const double *__restrict__ X,
                                                       const double *__restrict__ G,

@@ -733,7 +733,9 @@ int fmm_reserve_n(vpmx_ctx *c, FmmState &S, int64_t n)
     return VPMX_OK;
 }
 
-int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
+// One FMM UJ evaluation; lists_only: the tree, the traversal and the
+// near-field work items only (for the viscous sums), no evaluation.
+int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = false)
 {
     if (!c->fmm) c->fmm = new (std::nothrow) FmmState();
     if (!c->fmm) return fail(c, VPMX_ENOMEM, "out of host memory");
@@ -851,6 +853,17 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset)
     VPMX_HIP(c, fmm_p2p_chunks(st, S, (int64_t)pl.leaves.size()));
     const int64_t nm2l = dr.nm2l, np2p = dr.np2p, pairs = dr.p2p_pairs;
     const double t3 = now_ms();
+    if (lists_only) {
+        S.stats[0] = t1 - t0;
+        S.stats[1] = t3 - t2;
+        S.stats[2] = t2 - t1;
+        S.stats[3] = (double)nc;
+        S.stats[4] = (double)pl.leaves.size();
+        S.stats[5] = (double)nm2l;
+        S.stats[6] = (double)np2p;
+        S.stats[7] = (double)pairs;
+        return VPMX_OK;
+    }
 
     FmmDev d;
     d.P = P->p;
@@ -1030,8 +1043,13 @@ int enqueue_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S
             VPMX_TRY(enqueue_estr(c, U));
             VPMX_HIP(c, launch_sfs_coeff(c->stream, f, S->Cs, S->clip_backscatter));
         }
-        if (S->viscous == VPMX_VISCOUS_PSE)
-            VPMX_HIP(c, launch_pse(c->stream, U->kernel, f, S->nu, c->s.vrec));
+        if (S->viscous == VPMX_VISCOUS_PSE) {
+            if (U->fmm)   // over the near lists of the stage's FMM evaluation
+                VPMX_HIP(c, launch_fmm_pse(c->stream, *c->fmm, U->kernel, f, c->s, S->nu,
+                                           c->s.vrec));
+            else
+                VPMX_HIP(c, launch_pse(c->stream, U->kernel, f, S->nu, c->s.vrec));
+        }
         TimedScope ts(c, T_UPDATE);
         VPMX_HIP(c, launch_update_dev(c->stream, f, U->transposed, sfs, S->f, S->g, zeta0,
                                       c->b.uinf, fid, rk3 ? RK3_A[s] : 0.0,
@@ -1049,12 +1067,13 @@ int enqueue_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S
 // strengths re-fitted by conjugate gradients to the free particles' vorticity
 // at the free particles (three independent systems, one per component).
 // Host-driven: every iteration reads its three dot products back.
-int core_spreading_reset(vpmx_ctx *c, int kernel, const vpmx_step_params *S, bool force,
-                         int *did, int *iters, double res[3])
+int core_spreading_reset(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S,
+                         bool force, int *did, int *iters, double res[3])
 {
     DeviceField &f = c->f;
     Scratch &s = c->s;
     hipStream_t st = c->stream;
+    const int kernel = U->kernel;
     if (did) *did = 0;
     if (iters) *iters = 0;
     if (res) res[0] = res[1] = res[2] = 0.0;
@@ -1069,10 +1088,19 @@ int core_spreading_reset(vpmx_ctx *c, int kernel, const vpmx_step_params *S, boo
     }
     const int64_t n = f.np, ld = f.cap;
     const uint8_t *mask = f.any_static ? f.is_static : nullptr;
+    // With the FMM every vorticity sum runs over the near lists of one tree
+    // built here at the current positions and cores (zeta_fmm); directly
+    // otherwise (zeta_direct).
+    if (U->fmm) VPMX_TRY(run_fmm(c, U, 1, true));
+    auto zeta = [&](const double *G, int64_t gld, double s0, double *w) -> hipError_t {
+        if (U->fmm)
+            return launch_fmm_zeta(st, *c->fmm, kernel, f, s, G, gld, s0, mask, s.vrec, w, ld);
+        return launch_zeta_sum(st, kernel, f, G, gld, s0, true, s.vrec, w, ld);
+    };
     // right-hand side: the free particles' vorticity with their current cores
-    VPMX_HIP(c, launch_zeta_sum(st, kernel, f, f.G, f.cap, 0.0, true, s.vrec, s.cw, ld));
+    VPMX_HIP(c, zeta(f.G, f.cap, 0.0, s.cw));
     VPMX_HIP(c, launch_set_sigma(st, f, S->sgm0));
-    VPMX_HIP(c, launch_zeta_sum(st, kernel, f, f.G, f.cap, S->sgm0, true, s.vrec, s.cAp, ld));
+    VPMX_HIP(c, zeta(f.G, f.cap, S->sgm0, s.cAp));
     VPMX_HIP(c, launch_cg_init(st, s.cw, s.cAp, mask, n, ld, s.cr, s.cp));
     double h[6];
     VPMX_HIP(c, launch_cg_dot(st, s.cr, s.cr, mask, n, ld, s.cpart, s.cdev));
@@ -1084,7 +1112,7 @@ int core_spreading_reset(vpmx_ctx *c, int kernel, const vpmx_step_params *S, boo
     for (int k = 0; k < 3; ++k) done[k] = !(sqrt(rr[k]) > S->tol * sqrt(ww[k]));
     int it = 0;
     while (it < S->itmax && !(done[0] && done[1] && done[2])) {
-        VPMX_HIP(c, launch_zeta_sum(st, kernel, f, s.cp, ld, S->sgm0, true, s.vrec, s.cAp, ld));
+        VPMX_HIP(c, zeta(s.cp, ld, S->sgm0, s.cAp));
         double pAp[3];
         VPMX_HIP(c, launch_cg_dot(st, s.cp, s.cAp, mask, n, ld, s.cpart, s.cdev));
         VPMX_HIP(c, hipMemcpyAsync(pAp, s.cdev, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1191,7 +1219,7 @@ int run_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S,
     if (!U->fmm && (U->precision == VPMX_MIXED || !B.nfields)) VPMX_TRY(enqueue_sort(c));
     if (S->viscous == VPMX_VISCOUS_CORESPREADING) {   // host-driven adaptation after the step
         VPMX_TRY(enqueue_step(c, U, S, dt, relax, rk3));
-        return core_spreading_reset(c, U->kernel, S, false, nullptr, nullptr, nullptr);
+        return core_spreading_reset(c, U, S, false, nullptr, nullptr, nullptr);
     }
     if (!c->g.on || U->fmm) return enqueue_step(c, U, S, dt, relax, rk3);
 
@@ -1917,7 +1945,7 @@ int vpmx_core_spreading_reset(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_s
         return fail(c, VPMX_EUNSUPPORTED, "viscous schemes do not run on batched fields");
     int d = 0, it = 0;
     double r3[3];
-    VPMX_TRY(core_spreading_reset(c, U->kernel, S, force != 0, &d, &it, r3));
+    VPMX_TRY(core_spreading_reset(c, U, S, force != 0, &d, &it, r3));
     if (did) *did = d;
     if (iters) *iters = it;
     if (res)

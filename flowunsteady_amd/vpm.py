@@ -226,6 +226,7 @@ class ViscousScheme:
     beta: float = 1.5
     itmax: int = 20
     tol: float = 1e-3
+    zeta: object = None
 
 
 def Inviscid() -> ViscousScheme:
@@ -238,16 +239,37 @@ def ParticleStrengthExchange(nu: float) -> ViscousScheme:
     return ViscousScheme("ParticleStrengthExchange", N.VISCOUS_PSE, float(nu))
 
 
+@dataclass(frozen=True)
+class ZetaScheme:
+    """How core spreading evaluates the vorticity sum_j Gamma_j zeta_s(x - x_j)
+    at the particles: all pairs (zeta_direct) or the near field of an FMM tree
+    with the field's FMM settings (zeta_fmm; the far field of zeta is
+    neglected, as the reference's zeta_fmm leaves it to the FMM's near field)."""
+    name: str
+    fmm: bool
+
+
+zeta_direct = ZetaScheme("zeta_direct", False)
+zeta_fmm = ZetaScheme("zeta_fmm", True)
+
+
 def CoreSpreading(nu: float, sgm0: float, zeta=None, *, beta: float = 1.5, itmax: int = 20,
                   tol: float = 1e-3) -> ViscousScheme:
     """Core spreading (cores grow as d sigma/dt = nu/sigma) with radial-basis
     function re-fitting when a core exceeds beta * sgm0 (conjugate gradients,
-    itmax iterations, relative residual tol).  `zeta` (vpm.zeta_fmm in the
-    reference) selects the vorticity evaluation there; here the sums are
-    direct.  The reference's examples pass beta=100.0, whose meaning in FLOWVPM
-    is not available here; this trigger is beta * sgm0."""
+    itmax iterations, relative residual tol), as
+    vpm.CoreSpreading(nu, sgm0, zeta; beta, itmax, tol)
+    (examples/rotorhover/rotorhover.jl:170).  `zeta` (zeta_direct or
+    zeta_fmm) selects the vorticity sums of a standalone
+    rbf_conjugategradient; inside a time step they follow the field's UJ
+    scheme (UJ_fmm_gpu: the near lists of an FMM tree built at the step's
+    final positions; UJ_gpu: all pairs).  The reference's examples pass
+    beta=100.0, whose meaning in FLOWVPM is not available here; this trigger
+    is beta * sgm0."""
+    if zeta is not None and not isinstance(zeta, ZetaScheme):
+        raise ValueError("zeta must be vpm.zeta_direct or vpm.zeta_fmm")
     return ViscousScheme("CoreSpreading", N.VISCOUS_CORESPREADING, float(nu), float(sgm0),
-                         float(beta), int(itmax), float(tol))
+                         float(beta), int(itmax), float(tol), zeta)
 
 
 def isinviscid(v: ViscousScheme) -> bool:
@@ -644,7 +666,9 @@ def rbf_conjugategradient(pfield: ParticleField, *, force: bool = True):
     ctx.check(L.vpmx_upload(ctx.ptr, pfield._ptr(), pfield.np, ctypes.byref(pfield.layout)))
     did, it = ctypes.c_int32(), ctypes.c_int32()
     res = (ctypes.c_double * 3)()
-    ctx.check(L.vpmx_core_spreading_reset(ctx.ptr, ctypes.byref(pfield._uj_params(fmm=False)),
+    z = pfield.viscous.zeta
+    fmm = z.fmm if z is not None else pfield.UJ is UJ_fmm_gpu
+    ctx.check(L.vpmx_core_spreading_reset(ctx.ptr, ctypes.byref(pfield._uj_params(fmm=fmm)),
                                           ctypes.byref(pfield._step_params()), int(force),
                                           ctypes.byref(did), ctypes.byref(it), res))
     ctx.check(L.vpmx_download(ctx.ptr, pfield._ptr(), pfield.np, ctypes.byref(pfield.layout)))

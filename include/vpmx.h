@@ -250,8 +250,12 @@ int vpmx_ffv(vpmx_ctx *ctx, const double *bound, int64_t nb, const vpmx_layout *
  * (non-static) core is set to sgm0 and the free strengths are re-fitted so
  * that sum_j Gamma'_j zeta_sgm0(x_i - x_j) equals the free particles' vorticity
  * sum_j Gamma_j zeta_sigma_j(x_i - x_j) at every free particle, by conjugate
- * gradients from the current strengths (step->itmax, step->tol).  Direct
- * FP64 sums; the kernel must be gaussian-erf.  *did (optional) = 1 when the
+ * gradients from the current strengths (step->itmax, step->tol).  FP64 sums:
+ * all pairs when uj->fmm = 0 (vpm.zeta_direct); with uj->fmm = 1 the near
+ * field of an FMM tree built at the current positions with uj's ncrit, theta
+ * and sigma_cut (vpm.zeta_fmm, the setting of examples/rotorhover/rotorhover.jl:170),
+ * whose far field of zeta is neglected.  The same near lists carry the PSE
+ * sums of an FMM time step.  The kernel must be gaussian-erf.  *did (optional) = 1 when the
  * reset ran, *iters the CG iterations, res[3] (optional) the final relative
  * residual per component. */
 int vpmx_core_spreading_reset(vpmx_ctx *ctx, const vpmx_uj_params *uj,

@@ -723,6 +723,110 @@ int orc_fmm_near(int kernel, int transposed, int what, int ncrit, double theta, 
     return 0;
 }
 
+/* ---- viscous sums over the near-field lists ------------------------------------
+ * The FMM leaves the vorticity kernel zeta (and PSE's eta = -zeta'/rho) to its
+ * near field, as its rbf mode does for FLOWVPM's zeta_fmm
+ * (examples/rotorhover/rotorhover.jl:170): a plan is the tree and P2P lists
+ * orc_uj_fmm builds for the same particles, sigmas, ncrit, theta and sigcut,
+ * and each sum runs over a target leaf's list in list order, each source
+ * leaf's particles in sorted order. */
+struct orc_near_plan {
+    orc_tree T;
+    pairlist m2l, p2p;
+    int64_t *p2p_off;
+};
+
+struct orc_near_plan *orc_near_plan_build(int ncrit, double theta, double sigcut, int64_t n,
+                                          const double *X, const double *sig)
+{
+    if (ncrit < 1 || n < 0) return NULL;
+    struct orc_near_plan *pl = calloc(1, sizeof(*pl));
+    tree_build(&pl->T, n, X, sig, ncrit);
+    traverse(&pl->T, theta, sigcut, &pl->m2l, &pl->p2p);
+    pl->p2p_off = csr_offsets(&pl->p2p, pl->T.ncell);
+    return pl;
+}
+
+void orc_near_plan_free(struct orc_near_plan *pl)
+{
+    if (!pl) return;
+    free(pl->p2p_off);
+    free(pl->m2l.a);
+    free(pl->p2p.a);
+    tree_free(&pl->T);
+    free(pl);
+}
+
+double orc_eta(int kernel, double rho);
+void orc_kernel(int kernel, double rho, double *zeta, double *g, double *dg);
+
+/* w_i = sum_j Gamma_j zeta(r_ij / s_j) / s_j^3 over the near lists (s_j =
+ * ssig[j], or s0 for every source when s0 > 0; masked sources have no
+ * strength; coincident points included), w (n x 3) overwritten. */
+void orc_near_zeta(const struct orc_near_plan *pl, int kernel, const double *X, const double *G,
+                   const double *ssig, double s0, const uint8_t *mask, double *w)
+{
+    const orc_tree *T = &pl->T;
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int64_t c = 0; c < T->ncell; c++) {
+        const orc_cell *A = &T->cells[c];
+        if (A->nchild) continue;
+        for (int64_t jt = A->begin; jt < A->end; jt++) {
+            const int64_t i = T->perm[jt];
+            double acc[3] = {0, 0, 0};
+            for (int64_t e = pl->p2p_off[c]; e < pl->p2p_off[c + 1]; e++) {
+                const orc_cell *B = &T->cells[pl->p2p.a[2 * e + 1]];
+                for (int64_t js = B->begin; js < B->end; js++) {
+                    const int64_t j = T->perm[js];
+                    if (mask && mask[j]) continue;
+                    const double d0 = X[3 * i] - X[3 * j], d1 = X[3 * i + 1] - X[3 * j + 1],
+                                 d2 = X[3 * i + 2] - X[3 * j + 2];
+                    const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                    const double sj = s0 > 0.0 ? s0 : ssig[j];
+                    double z;
+                    orc_kernel(kernel, r / sj, &z, NULL, NULL);
+                    z /= sj * sj * sj;
+                    for (int k = 0; k < 3; k++) acc[k] += z * G[3 * j + k];
+                }
+            }
+            for (int k = 0; k < 3; k++) w[3 * i + k] = acc[k];
+        }
+    }
+}
+
+/* PSE dGamma/dt (orc_pse's pair terms) over the near lists, V (n x 3)
+ * overwritten. */
+void orc_near_pse(const struct orc_near_plan *pl, int kernel, double nu, const double *X,
+                  const double *G, const double *sig, const double *vol, double *V)
+{
+    const orc_tree *T = &pl->T;
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int64_t c = 0; c < T->ncell; c++) {
+        const orc_cell *A = &T->cells[c];
+        if (A->nchild) continue;
+        for (int64_t jt = A->begin; jt < A->end; jt++) {
+            const int64_t i = T->perm[jt];
+            double acc[3] = {0, 0, 0};
+            for (int64_t e = pl->p2p_off[c]; e < pl->p2p_off[c + 1]; e++) {
+                const orc_cell *B = &T->cells[pl->p2p.a[2 * e + 1]];
+                for (int64_t js = B->begin; js < B->end; js++) {
+                    const int64_t j = T->perm[js];
+                    if (j == i) continue;
+                    const double d0 = X[3 * i] - X[3 * j], d1 = X[3 * i + 1] - X[3 * j + 1],
+                                 d2 = X[3 * i + 2] - X[3 * j + 2];
+                    const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                    const double s2 = 0.5 * (sig[i] * sig[i] + sig[j] * sig[j]);
+                    const double s = sqrt(s2);
+                    const double wgt = 2.0 * nu / s2 * orc_eta(kernel, r / s) / (s2 * s);
+                    for (int k = 0; k < 3; k++)
+                        acc[k] += wgt * (vol[i] * G[3 * j + k] - vol[j] * G[3 * i + k]);
+                }
+            }
+            for (int k = 0; k < 3; k++) V[3 * i + k] = acc[k];
+        }
+    }
+}
+
 /* Expose the derivative recurrence for tests: b_k(r), |k| <= P, graded order. */
 int orc_fmm_deriv(int P, const double r[3], double *b, int *mi_out)
 {

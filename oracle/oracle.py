@@ -214,8 +214,55 @@ def _viscous_ready(L):
         L.orc_core_spreading_reset.restype = ctypes.c_int
         L.orc_core_spreading_reset.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(_Field),
                                                ctypes.POINTER(ctypes.c_int)]
+        L.orc_near_plan_build.restype = ctypes.c_void_p
+        L.orc_near_plan_build.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int64, _dp, _dp]
+        L.orc_near_plan_free.argtypes = [ctypes.c_void_p]
+        L.orc_near_zeta.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
+                                    ctypes.POINTER(ctypes.c_uint8), _dp]
+        L.orc_near_pse.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp,
+                                   _dp, _dp]
         L._viscous_ready = True
     return L
+
+
+def _near_plan(L, X, sigma, ncrit, theta, sigcut):
+    pl = L.orc_near_plan_build(int(ncrit), float(theta), float(sigcut), X.shape[0], _p(X),
+                               _p(sigma))
+    if not pl:
+        raise ValueError("orc_near_plan_build: bad arguments")
+    return pl
+
+
+def near_zeta(kernel, X, G, sigma, s0=0.0, mask=None, ncrit=50, theta=0.4, sigcut=0.0):
+    """The vorticity sums of zeta_sum (targets = sources = the particles) over
+    the near-field lists of the oracle FMM's tree of (X, sigma)."""
+    L = _viscous_ready(lib())
+    X = _c(X, (-1, 3)); G = _c(G, (-1, 3)); sigma = _c(sigma, (-1,))
+    m = None
+    if mask is not None:
+        mk = np.ascontiguousarray(mask, dtype=np.uint8)
+        m = mk.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    w = np.zeros((X.shape[0], 3))
+    pl = _near_plan(L, X, sigma, ncrit, theta, sigcut)
+    try:
+        L.orc_near_zeta(pl, _kid(kernel), _p(X), _p(G), _p(sigma), float(s0), m, _p(w))
+    finally:
+        L.orc_near_plan_free(pl)
+    return w
+
+
+def near_pse(kernel, nu, X, G, sigma, vol, ncrit=50, theta=0.4, sigcut=0.0):
+    """The PSE rate of pse() over the near-field lists of the oracle FMM's tree."""
+    L = _viscous_ready(lib())
+    X = _c(X, (-1, 3)); G = _c(G, (-1, 3)); sigma = _c(sigma, (-1,)); vol = _c(vol, (-1,))
+    V = np.zeros((X.shape[0], 3))
+    pl = _near_plan(L, X, sigma, ncrit, theta, sigcut)
+    try:
+        L.orc_near_pse(pl, _kid(kernel), float(nu), _p(X), _p(G), _p(sigma), _p(vol), _p(V))
+    finally:
+        L.orc_near_plan_free(pl)
+    return V
 
 
 def eta(kernel, rho):

@@ -101,6 +101,14 @@ int orc_num_threads(void);
 
 /* fmm_oracle.c */
 int orc_fmm_deriv(int P, const double r[3], double *b, int *mi_out);
+struct orc_near_plan;
+struct orc_near_plan *orc_near_plan_build(int ncrit, double theta, double sigcut, int64_t n,
+                                          const double *X, const double *sig);
+void orc_near_plan_free(struct orc_near_plan *pl);
+void orc_near_zeta(const struct orc_near_plan *pl, int kernel, const double *X, const double *G,
+                   const double *ssig, double s0, const uint8_t *mask, double *w);
+void orc_near_pse(const struct orc_near_plan *pl, int kernel, double nu, const double *X,
+                  const double *G, const double *sig, const double *vol, double *V);
 
 /* viscous_oracle.c */
 double orc_eta(int kernel, double rho);

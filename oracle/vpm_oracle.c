@@ -437,6 +437,66 @@ void orc_relax(int kind, double rlxf, const double *J, double *G)
  * strength re-fitted so that the free particles' vorticity at the free
  * particles is kept (static particles are left as they are and drop out of
  * both sides).  Returns 1 when a reset happened. */
+/* The same with the FMM (P->fmm): every vorticity sum runs over the near
+ * lists of one tree of all particles (statics included, with no strength)
+ * built at the current positions and cores; conjugate gradients on the free
+ * entries (static ones drop out of the products and keep their strength). */
+static int core_spreading_reset_fmm(const orc_params *P, orc_field *F)
+{
+    const int64_t n = F->np;
+    const uint8_t *mask = F->is_static;
+    struct orc_near_plan *pl = orc_near_plan_build(P->ncrit, P->theta, P->sigcut, n, F->X,
+                                                   F->sigma);
+    double *w = malloc(sizeof(double) * 3 * n), *r = malloc(sizeof(double) * 3 * n),
+           *p = malloc(sizeof(double) * 3 * n), *Ap = malloc(sizeof(double) * 3 * n);
+    orc_near_zeta(pl, P->kernel, F->X, F->G, F->sigma, 0.0, mask, w);
+    for (int64_t i = 0; i < n; i++)
+        if (!(mask && mask[i])) F->sigma[i] = P->sgm0;
+    orc_near_zeta(pl, P->kernel, F->X, F->G, NULL, P->sgm0, mask, Ap);
+    double rr[3] = {0, 0, 0}, wn[3] = {0, 0, 0};
+    for (int64_t i = 0; i < n; i++) {
+        const int off = mask && mask[i];
+        for (int k = 0; k < 3; k++) {
+            r[3 * i + k] = off ? 0.0 : w[3 * i + k] - Ap[3 * i + k];
+            p[3 * i + k] = r[3 * i + k];
+            if (!off) {
+                rr[k] += r[3 * i + k] * r[3 * i + k];
+                wn[k] += w[3 * i + k] * w[3 * i + k];
+            }
+        }
+    }
+    int it = 0, done[3];
+    for (int k = 0; k < 3; k++) done[k] = !(sqrt(rr[k]) > P->tol * sqrt(wn[k]));
+    while (it < P->itmax && !(done[0] && done[1] && done[2])) {
+        orc_near_zeta(pl, P->kernel, F->X, p, NULL, P->sgm0, mask, Ap);
+        double pAp[3] = {0, 0, 0}, rr1[3] = {0, 0, 0}, a[3];
+        for (int64_t i = 0; i < n; i++)
+            if (!(mask && mask[i]))
+                for (int k = 0; k < 3; k++) pAp[k] += p[3 * i + k] * Ap[3 * i + k];
+        for (int k = 0; k < 3; k++) a[k] = done[k] || pAp[k] == 0.0 ? 0.0 : rr[k] / pAp[k];
+        for (int64_t i = 0; i < n; i++) {
+            if (mask && mask[i]) continue;
+            for (int k = 0; k < 3; k++) {
+                F->G[3 * i + k] += a[k] * p[3 * i + k];
+                r[3 * i + k] -= a[k] * Ap[3 * i + k];
+                rr1[k] += r[3 * i + k] * r[3 * i + k];
+            }
+        }
+        for (int k = 0; k < 3; k++) {
+            const double b = done[k] || rr[k] == 0.0 ? 0.0 : rr1[k] / rr[k];
+            if (!done[k])
+                for (int64_t i = 0; i < n; i++)
+                    if (!(mask && mask[i])) p[3 * i + k] = r[3 * i + k] + b * p[3 * i + k];
+            rr[k] = rr1[k];
+            done[k] = done[k] || !(sqrt(rr[k]) > P->tol * sqrt(wn[k]));
+        }
+        it++;
+    }
+    free(w); free(r); free(p); free(Ap);
+    orc_near_plan_free(pl);
+    return it;
+}
+
 int orc_core_spreading_reset(const orc_params *P, orc_field *F, int *iters)
 {
     const int64_t n = F->np;
@@ -449,6 +509,11 @@ int orc_core_spreading_reset(const orc_params *P, orc_field *F, int *iters)
     }
     if (iters) *iters = 0;
     if (nf == 0 || !(smax > P->beta * P->sgm0)) return 0;
+    if (P->fmm) {
+        const int it = core_spreading_reset_fmm(P, F);
+        if (iters) *iters = it;
+        return 1;
+    }
     double *X = malloc(sizeof(double) * 3 * nf), *G = malloc(sizeof(double) * 3 * nf),
            *s = malloc(sizeof(double) * nf), *w = malloc(sizeof(double) * 3 * nf);
     int64_t *idx = malloc(sizeof(int64_t) * nf);
@@ -619,6 +684,13 @@ static double *viscous_stage(const orc_params *P, const orc_field *F)
 {
     if (P->viscous != 2) return NULL;
     double *V = malloc(sizeof(double) * 3 * (F->np > 0 ? F->np : 1));
+    if (P->fmm) {   /* over the near lists of the stage's FMM tree */
+        struct orc_near_plan *pl = orc_near_plan_build(P->ncrit, P->theta, P->sigcut, F->np,
+                                                       F->X, F->sigma);
+        orc_near_pse(pl, P->kernel, P->nu, F->X, F->G, F->sigma, F->vol, V);
+        orc_near_plan_free(pl);
+        return V;
+    }
     orc_pse(P->kernel, P->nu, F->np, F->X, F->G, F->sigma, F->vol, F->np, NULL, V);
     return V;
 }

@@ -121,3 +121,69 @@ def test_viscous_errors():
     with pytest.raises(N.VPMXError) as e:
         vpm.nextstep_batch([a, b], 0.01)
     assert e.value.code == N.EUNSUPPORTED
+
+
+FMM_VISC = dict(p=8, ncrit=50, theta=0.4, nonzero_sigma=True)   # core-aware acceptance
+
+
+@pytest.mark.parametrize("kernel,sigcut", [("gaussianerf", 7.0), ("gaussian", 3.0)])
+def test_pse_fmm_rk3_vs_oracle(kernel, sigcut):
+    """PSE in an FMM time step runs over the near lists of each stage's FMM
+    evaluation: GPU FP64 against the oracle's FMM step with the same tree and
+    lists (PSE over its near lists), per particle."""
+    X, G, s, v = blob(4000, seed=23)
+    pf = make_field(X, G, s, vol=v, kernel=kernel, viscous=vpm.ParticleStrengthExchange(0.02),
+                    relaxation=vpm.pedrizzetti, fmm=vpm.FMM(**FMM_VISC))
+    pf.UJ = vpm.UJ_fmm_gpu
+    P = O.make_params(kernel, viscous="pse", nu=0.02, relax_kind=1, rlxf=0.3, fmm=True, p=8,
+                      ncrit=50, theta=0.4, sigcut=sigcut)
+    F = O.Field(X, G, s, vol=v)
+    for i in range(2):
+        vpm.nextstep(pf, 0.002, relax=True)
+        O.rk3_step(P, F, 0.002, True)
+    for get, ref in ((vpm.get_X, F.X), (vpm.get_Gamma, F.G)):
+        e = normwise(get(pf), ref)
+        print(kernel, get.__name__, e)
+        assert e[0] < 1e-12
+
+
+def test_core_spreading_fmm_reset_vs_oracle():
+    """A step with the FMM whose core growth crosses beta * sgm0: the re-fit's
+    vorticity sums run over the near lists of a tree built at the step's
+    final positions (zeta_fmm); against the oracle's same procedure, static
+    particles untouched."""
+    X, G, st = lattice_field()
+    s = np.full(len(X), 0.0555)
+    cs = vpm.CoreSpreading(0.05, sgm0=0.04, zeta=vpm.zeta_fmm, beta=1.4, itmax=400, tol=1e-10)
+    fmm = vpm.FMM(p=8, ncrit=20, theta=0.4, nonzero_sigma=True)
+    pf = make_field(X, G, s, static=st, viscous=cs, fmm=fmm)
+    pf.UJ = vpm.UJ_fmm_gpu
+    vpm.nextstep(pf, 0.01)
+    P = O.make_params("gaussianerf", viscous="corespreading", nu=0.05, sgm0=0.04, beta=1.4,
+                      itmax=400, tol=1e-10, fmm=True, p=8, ncrit=20, theta=0.4, sigcut=7.0)
+    F = O.Field(X, G, s, is_static=st.astype(np.uint8))
+    O.rk3_step(P, F, 0.01)
+    free = st == 0
+    assert np.all(vpm.get_sigma(pf)[free] == 0.04)
+    np.testing.assert_array_equal(vpm.get_Gamma(pf)[~free], G[~free])
+    eg = normwise(vpm.get_Gamma(pf), F.G)
+    print("fmm reset Gamma vs oracle", eg)
+    assert eg[0] < 1e-7
+
+
+def test_rbf_conjugategradient_zeta_fmm_matches_zeta_direct():
+    """rbf_conjugategradient with zeta_fmm (near lists at the core-aware
+    acceptance) and with zeta_direct fit the same strengths: the far pairs the
+    FMM leaves out are beyond 7 cores."""
+    X, G, st = lattice_field(nstatic=0)
+    s = np.full(len(X), 0.06)
+    out = {}
+    for z in (vpm.zeta_direct, vpm.zeta_fmm):
+        pf = make_field(X, G, s, fmm=vpm.FMM(p=4, ncrit=16, theta=0.4, nonzero_sigma=True),
+                        viscous=vpm.CoreSpreading(0.0, sgm0=0.04, zeta=z, itmax=500, tol=1e-11))
+        did, it, res = vpm.rbf_conjugategradient(pf)
+        assert did and max(res) <= 1e-11
+        out[z.name] = vpm.get_Gamma(pf).copy()
+    e = normwise(out["zeta_fmm"], out["zeta_direct"])
+    print("zeta_fmm vs zeta_direct Gamma", e)
+    assert e[0] < 1e-8

@@ -6,6 +6,8 @@ import numpy as np
 import pytest
 from scipy import integrate
 
+from flowunsteady_amd import workloads
+from helpers import normwise
 from oracle import oracle as O
 
 KERNELS = ["gaussian", "gaussianerf", "winckelmans"]
@@ -155,3 +157,44 @@ def test_pse_enters_the_step_as_an_additive_rate():
     np.testing.assert_allclose(Fv.G - Fi.G, 1e-3 * O.pse("gaussianerf", 0.05, X, G, s, vol),
                                rtol=1e-9, atol=1e-18)
     np.testing.assert_array_equal(Fv.X, Fi.X)
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "gaussian"])
+def test_near_list_viscous_sums(kernel):
+    """The FMM-driven viscous sums (zeta_fmm, PSE over the near lists):
+    with theta = 0 every pair is near and they are the direct sums; at the
+    core-aware acceptance (gap >= 7 source cores, equal cores) the neglected
+    far pairs lie beyond 7 cores (zeta(7)/zeta(0) = e^-24.5 for gaussian-erf),
+    so they match the direct sums to that tail."""
+    X, G, s, v = workloads.gaussian_blob(n=2500, seed=7, std=0.3)
+    s = np.full(len(X), 0.02)
+    vol = s ** 3
+    mask = (np.arange(len(X)) % 7 == 0).astype(np.uint8)
+    Gm = np.where(mask[:, None] == 1, 0.0, G)
+    wd = O.zeta_sum(kernel, X, X, Gm, s)
+    pd = O.pse(kernel, 0.01, X, G, s, vol)
+    w0 = O.near_zeta(kernel, X, G, s, mask=mask, theta=0.0)
+    p0 = O.near_pse(kernel, 0.01, X, G, s, vol, theta=0.0)
+    assert normwise(w0, wd)[0] < 1e-13 and normwise(p0, pd)[0] < 1e-13
+    w7 = O.near_zeta(kernel, X, G, s, mask=mask, sigcut=7.0)
+    p7 = O.near_pse(kernel, 0.01, X, G, s, vol, sigcut=7.0)
+    assert normwise(w7, wd)[0] < 1e-9 and normwise(p7, pd)[0] < 1e-9
+    # with s0 the uniform core replaces every source core
+    ws = O.near_zeta(kernel, X, G, s * 3, s0=0.02, mask=mask, theta=0.0)
+    assert normwise(ws, wd)[0] < 1e-13
+
+
+def test_core_spreading_reset_with_the_fmm_near_lists():
+    """The oracle's core-spreading reset over the near lists (zeta_fmm) with
+    theta = 0 is the direct reset, to the CG's rounding."""
+    X, G, s, v = workloads.gaussian_blob(n=600, seed=9, std=0.2)
+    s = np.full(len(X), 0.05)
+    st = (np.arange(len(X)) % 9 == 0).astype(np.uint8)
+    kw = dict(viscous="corespreading", nu=0.0, sgm0=0.03, beta=1.2, itmax=400, tol=1e-13)
+    Pd = O.make_params("gaussianerf", **kw)
+    Pf = O.make_params("gaussianerf", fmm=True, theta=0.0, ncrit=16, **kw)
+    Fd, Ff = O.Field(X, G, s, is_static=st), O.Field(X, G, s, is_static=st)
+    rd, rf = O.core_spreading_reset(Pd, Fd), O.core_spreading_reset(Pf, Ff)
+    assert rd[0] and rf[0] and rd[1] == rf[1] < 400     # both converge (100 iterations)
+    assert normwise(Ff.G, Fd.G)[0] < 1e-11              # measured 8.6e-14
+    np.testing.assert_array_equal(Ff.sigma, Fd.sigma)
