@@ -326,8 +326,8 @@ def fmm_section(ctx, n, orders, log, cpu=True, nref=65536):
         res["orders"][str(p)] = {
             "ms_per_uj": 1e3 * wall,
             "direct_equivalent_interactions_per_s": float(n) * n / wall,
-            "device_ms": st["ms_eval"], "host_tree_ms": st["ms_tree"],
-            "host_traverse_ms": st["ms_traverse"], "list_upload_ms": st["ms_upload"],
+            "device_ms": st["ms_eval"], "tree_build_ms": st["ms_tree"],
+            "traverse_ms": st["ms_traverse"], "host_to_device_tree_ms": st["ms_upload"],
             "cells": int(st["cells"]), "leaves": int(st["leaves"]),
             "m2l_pairs": int(st["m2l_pairs"]), "p2p_pairs": int(st["p2p_pairs"]),
             "p2p_pairs_per_device_s": st["p2p_pairs"] / dev_s if dev_s > 0 else None,
@@ -475,7 +475,7 @@ def full_step_run(n, steps, log, precisions=("mixed", "fp64"), every=10):
         st = ctx.fmm_stats()
         ens0, en0 = series[0][1], series[0][2]
         res[prec] = {"ms_per_step": 1e3 * wall / steps, "per_step_split": split,
-                     "host_tree_ms_per_uj": st["ms_tree"], "traverse_ms_per_uj": st["ms_traverse"],
+                     "tree_build_ms_per_uj": st["ms_tree"], "traverse_ms_per_uj": st["ms_traverse"],
                      "uj_per_step": 4, "core_size_derivative_passes_per_step": 3,
                      "estr_per_step": 3,
                      "enstrophy": [[k, e] for k, e, _ in series],

@@ -85,6 +85,22 @@ struct DttBufs {
     size_t tmp_bytes = 0;
 };
 
+// Scratch of the device octree build: per-cell level arrays (grown as levels
+// are added), per-level split counts, child ranges and octants, the scans of
+// the leaf / parent flags, and a pinned pair of level totals.
+struct TreeBufs {
+    int64_t cap_cell = 0, cap_lev = 0, cap_flags = 0;
+    int32_t *tb = nullptr, *te = nullptr, *lev = nullptr, *child0 = nullptr, *nchild = nullptr;
+    uint64_t *pre = nullptr;
+    int32_t *nk = nullptr, *flag = nullptr, *off = nullptr, *foff = nullptr;
+    int32_t *kb = nullptr, *ke = nullptr;
+    int8_t *ko = nullptr;
+    int32_t *lflag = nullptr, *pflag = nullptr, *lscan = nullptr, *pscan = nullptr;
+    uint8_t *scan_tmp = nullptr;
+    size_t scan_bytes = 0;
+    int32_t *h_tot = nullptr;
+};
+
 // Result sizes of a device traversal.
 struct DttResult {
     int64_t nm2l = 0, np2p = 0, p2p_pairs = 0;
@@ -127,6 +143,7 @@ struct FmmPlan {
     std::vector<int32_t> leaves, parents, kids, parent;
     std::vector<std::pair<int, int>> level_parents, level_kids;   // (offset, count) per level
     int64_t p2p_pairs = 0;
+    int64_t ncells = 0, nleaves = 0;   // of the device-built tree
     double ms_tree = 0, ms_traverse = 0, ms_upload = 0;
 };
 
@@ -194,13 +211,10 @@ struct FmmState {
     double *ssig = nullptr;
     uint8_t *sort_tmp = nullptr;
     size_t sort_bytes = 0;
-    // pinned host copies of the sorted keys and sigmas
-    uint64_t *h_keys = nullptr;
-    double *h_ssig = nullptr;
-    int64_t h_cap = 0;
     // statistics of the last evaluation
     double stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     DttBufs dtt;
+    TreeBufs tree;
 };
 
 // Estr over the near-field lists of the last FMM evaluation (J must be current);
@@ -225,7 +239,7 @@ hipError_t launch_fmm_dsig(hipStream_t st, const FmmState &S, int kernel, int pr
                            const DeviceField &f, const Scratch &s, double *DU, double *DJ);
 
 // Near-field work items of the current lists (S.chunks, S.nchunk, S.nslot;
-// S.part sized for 14 values per slot).  nleaf = S.plan.leaves.size().
+// S.part sized for 14 values per slot).  nleaf = S.plan.nleaves.
 hipError_t fmm_p2p_chunks(hipStream_t st, FmmState &S, int64_t nleaf);
 
 // Dual tree traversal on the device: fills m2l_off/m2l_src and p2p_off/p2p_src
@@ -233,6 +247,14 @@ hipError_t fmm_p2p_chunks(hipStream_t st, FmmState &S, int64_t nleaf);
 // buffers in S.
 hipError_t fmm_traverse_device(hipStream_t st, FmmState &S, int64_t ncell, double theta,
                                double sigcut, DttResult &res);
+
+// Octree of the sorted keys S.kout / sigmas S.ssig (n particles, bounding cube
+// pl.lo, pl.ext) built on the device: S.cells, S.leaves, S.parents, S.kids and
+// S.parent (grown as needed) and, on the host, pl.level_parents / level_kids,
+// pl.ncells and pl.nleaves.  The same tree as fmm_build_tree.
+hipError_t fmm_build_tree_device(hipStream_t st, FmmState &S, int64_t n, int ncrit, FmmPlan &pl,
+                                 int64_t *nc_out);
+void free_tree_bufs(TreeBufs &B);
 
 void fmm_build_tree(FmmPlan &pl, const uint64_t *key, const double *ssig, int64_t n, int ncrit);
 void fmm_traverse(FmmPlan &pl, double theta, double sigcut);

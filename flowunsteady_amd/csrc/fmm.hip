@@ -1767,7 +1767,7 @@ hipError_t launch_fmm_estr(hipStream_t st, const FmmState &S, int kernel, int tr
             f.X, f.G, f.sigma, f.J, DJ, s.perm, f.np, f.cap, s.srcE);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int64_t nleaf = (int64_t)S.plan.leaves.size();
+    const int64_t nleaf = (int64_t)S.plan.nleaves;
     if (nleaf == 0 || S.nchunk == 0) return hipSuccess;
     const unsigned g = (unsigned)S.nchunk;
 #define VPMX_FESTR3(KK, TR, DY)                                                                  \
@@ -1821,7 +1821,7 @@ hipError_t launch_fmm_dsig(hipStream_t st, const FmmState &S, int kernel, int pr
     fmm_zero_kernel<<<gridn(9 * f.cap, FMM_BLOCK), FMM_BLOCK, 0, st>>>(DJ, 9 * f.cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int64_t nleaf = (int64_t)S.plan.leaves.size();
+    const int64_t nleaf = (int64_t)S.plan.nleaves;
     if (nleaf == 0 || S.nchunk == 0) return hipSuccess;
     const unsigned g = (unsigned)S.nchunk;
 #define VPMX_DS(KK)                                                                             \
@@ -1964,7 +1964,7 @@ static hipError_t near_visc_go(hipStream_t st, const FmmState &S, int kernel,
         f.X, G, gld, f.sigma, s0, f.vol, mask, s.perm, n, f.cap, rec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int64_t nleaf = (int64_t)S.plan.leaves.size();
+    const int64_t nleaf = (int64_t)S.plan.nleaves;
     if (nleaf == 0 || S.nchunk == 0) return hipSuccess;
     const unsigned g = (unsigned)S.nchunk;
     switch (kernel) {
@@ -2329,6 +2329,335 @@ __global__ __launch_bounds__(FMM_BLOCK) void p2p_chunk_fill_kernel(
         q.part = n > 1 ? pbase[li] + j * nt : -1;
         chunks[coff[li] + j] = q;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Device octree
+// ---------------------------------------------------------------------------
+// The tree of fmm_build_tree, built on the device from the sorted 63-bit keys
+// level by level: every cell of a level holding more than ncrit particles
+// (and above the deepest level) is cut into its non-empty octants by binary
+// searches of the key range; the children of a level are appended in parent
+// order, each parent's in octant order (exclusive scans of the child counts),
+// so the breadth-first cell order and every cell are those of the host build.
+// One read-back per level (the next level's size and its number of parents).
+__device__ __forceinline__ int32_t key_lower_bound(const uint64_t *__restrict__ key, int32_t b,
+                                                   int32_t e, uint64_t v)
+{
+    while (b < e) {
+        const int32_t m = b + (e - b) / 2;
+        if (key[m] < v) b = m + 1; else e = m;
+    }
+    return b;
+}
+
+// Level cells [lb, lb + m): child count nk[i], split flag, and the ranges and
+// octants of the non-empty children; entries nk[m] = flag[m] = 0 close the scans.
+__global__ __launch_bounds__(FMM_BLOCK) void tree_split_kernel(
+    const uint64_t *__restrict__ key, const int32_t *__restrict__ tb,
+    const int32_t *__restrict__ te, const uint64_t *__restrict__ pre,
+    const int32_t *__restrict__ lev, int64_t lb, int64_t m, int ncrit, int32_t *__restrict__ nk,
+    int32_t *__restrict__ flag, int32_t *__restrict__ kb, int32_t *__restrict__ ke,
+    int8_t *__restrict__ ko)
+{
+    const int64_t i = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (i > m) return;
+    if (i == m) {
+        nk[m] = 0;
+        flag[m] = 0;
+        return;
+    }
+    const int64_t c = lb + i;
+    const int32_t b = tb[c], e = te[c], l = lev[c];
+    int cnt = 0;
+    if (e - b > ncrit && l < KEYBITS) {
+        const int shift = 3 * (KEYBITS - l - 1);
+        for (uint64_t o = 0; o < 8; o++) {
+            const uint64_t cp = (pre[c] << 3) | o;
+            const int32_t bb = key_lower_bound(key, b, e, cp << shift);
+            const int32_t ee = key_lower_bound(key, bb, e, (cp + 1) << shift);
+            if (ee == bb) continue;
+            kb[8 * i + cnt] = bb;
+            ke[8 * i + cnt] = ee;
+            ko[8 * i + cnt] = (int8_t)o;
+            cnt++;
+        }
+    }
+    nk[i] = cnt;
+    flag[i] = cnt > 0;
+}
+
+// Children of the level's cells from slot le + off[i]; parents' child links.
+__global__ __launch_bounds__(FMM_BLOCK) void tree_emit_kernel(
+    int64_t lb, int64_t m, int64_t le, const int32_t *__restrict__ nk,
+    const int32_t *__restrict__ off, const int32_t *__restrict__ kb,
+    const int32_t *__restrict__ ke, const int8_t *__restrict__ ko, int32_t *__restrict__ tb,
+    int32_t *__restrict__ te, uint64_t *__restrict__ pre, int32_t *__restrict__ lev,
+    int32_t *__restrict__ child0, int32_t *__restrict__ nchild, int32_t *__restrict__ parent)
+{
+    const int64_t i = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (i >= m) return;
+    const int64_t c = lb + i;
+    const int cnt = nk[i];
+    if (cnt == 0) {
+        child0[c] = -1;
+        nchild[c] = 0;
+        return;
+    }
+    const int64_t first = le + off[i];
+    child0[c] = (int32_t)first;
+    nchild[c] = cnt;
+    for (int r = 0; r < cnt; r++) {
+        const int64_t d = first + r;
+        tb[d] = kb[8 * i + r];
+        te[d] = ke[8 * i + r];
+        pre[d] = (pre[c] << 3) | (uint64_t)ko[8 * i + r];
+        lev[d] = lev[c] + 1;
+        parent[d] = (int32_t)c;
+    }
+}
+
+// Cell geometry (geometric box centre, half diagonal) and leaves' sigma max;
+// leaf / parent flags for the list compaction (entries nc close the scans).
+__global__ __launch_bounds__(FMM_BLOCK) void tree_cells_kernel(
+    int64_t nc, const int32_t *__restrict__ tb, const int32_t *__restrict__ te,
+    const uint64_t *__restrict__ pre, const int32_t *__restrict__ lev,
+    const int32_t *__restrict__ child0, const int32_t *__restrict__ nchild,
+    const double *__restrict__ ssig, double lo0, double lo1, double lo2, double ext,
+    FmmCellDev *__restrict__ cells, int32_t *__restrict__ lflag, int32_t *__restrict__ pflag)
+{
+    // product and sum rounded separately, as on the host: the acceptance
+    // test compares centre distances exactly
+#pragma clang fp contract(off)
+    const int64_t c = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (c > nc) return;
+    if (c == nc) {
+        lflag[nc] = 0;
+        pflag[nc] = 0;
+        return;
+    }
+    const int l = lev[c];
+    const uint64_t p = pre[c];
+    uint64_t ix = 0, iy = 0, iz = 0;
+    for (int k = 0; k < l; k++) {
+        const uint64_t o = (p >> (3 * (l - 1 - k))) & 7u;
+        ix = (ix << 1) | (o & 1u);
+        iy = (iy << 1) | ((o >> 1) & 1u);
+        iz = (iz << 1) | ((o >> 2) & 1u);
+    }
+    const double h = ext / (double)((uint64_t)1 << l);
+    FmmCellDev C;
+    C.begin = tb[c];
+    C.end = te[c];
+    C.child0 = child0[c];
+    C.nchild = nchild[c];
+    C.cx = lo0 + ((double)ix + 0.5) * h;
+    C.cy = lo1 + ((double)iy + 0.5) * h;
+    C.cz = lo2 + ((double)iz + 0.5) * h;
+    C.R = 0.8660254037844386 * h;
+    double s = 0.0;
+    if (C.nchild == 0)
+        for (int32_t j = C.begin; j < C.end; j++) s = fmax(s, ssig[j]);
+    C.smax = s;
+    cells[c] = C;
+    lflag[c] = C.nchild == 0;
+    pflag[c] = C.nchild != 0;
+}
+
+// Parents' sigma max from their children, one level at a time from the deepest.
+__global__ __launch_bounds__(FMM_BLOCK) void tree_smax_kernel(int64_t lb, int64_t m,
+                                                              FmmCellDev *__restrict__ cells)
+{
+    const int64_t c = lb + (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (c >= lb + m) return;
+    const FmmCellDev C = cells[c];
+    if (C.nchild == 0) return;
+    double s = 0.0;
+    for (int k = 0; k < C.nchild; k++) s = fmax(s, cells[C.child0 + k].smax);
+    cells[c].smax = s;
+}
+
+// leaves and parents in cell order (from the scans of their flags); kids =
+// every cell but the root, in cell order.
+__global__ __launch_bounds__(FMM_BLOCK) void tree_lists_kernel(
+    int64_t nc, const int32_t *__restrict__ lflag, const int32_t *__restrict__ lscan,
+    const int32_t *__restrict__ pscan, int32_t *__restrict__ leaves,
+    int32_t *__restrict__ parents, int32_t *__restrict__ kids)
+{
+    const int64_t c = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (c >= nc) return;
+    if (lflag[c])
+        leaves[lscan[c]] = (int32_t)c;
+    else
+        parents[pscan[c]] = (int32_t)c;
+    if (c > 0) kids[c - 1] = (int32_t)c;
+}
+
+template <class T>
+static hipError_t grow_keep(T **p, int64_t need, int64_t cap, int64_t keep, hipStream_t st)
+{
+    T *q = nullptr;
+    hipError_t e = hipMalloc((void **)&q, std::max<int64_t>(need, 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (*p && keep > 0) {
+        e = hipMemcpyAsync(q, *p, keep * sizeof(T), hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+    }
+    if (*p) (void)hipFree(*p);
+    *p = q;
+    (void)cap;
+    return hipSuccess;
+}
+
+void free_tree_bufs(TreeBufs &B)
+{
+    for (int32_t **q : {&B.tb, &B.te, &B.lev, &B.child0, &B.nchild, &B.nk, &B.flag, &B.off,
+                        &B.foff, &B.kb, &B.ke, &B.lflag, &B.pflag, &B.lscan, &B.pscan})
+        if (*q) { (void)hipFree(*q); *q = nullptr; }
+    if (B.pre) (void)hipFree(B.pre);
+    if (B.ko) (void)hipFree(B.ko);
+    if (B.scan_tmp) (void)hipFree(B.scan_tmp);
+    if (B.h_tot) (void)hipHostFree(B.h_tot);
+    B = TreeBufs{};
+}
+
+hipError_t fmm_build_tree_device(hipStream_t st, FmmState &S, int64_t n, int ncrit,
+                                 FmmPlan &pl, int64_t *nc_out)
+{
+    TreeBufs &B = S.tree;
+    if (!B.h_tot) VPMX_CHK(hipHostMalloc((void **)&B.h_tot, 4 * sizeof(int32_t)));
+    // cell arrays: grown (keeping the built levels) as levels are added
+    auto grow_cells = [&](int64_t need, int64_t keep) -> hipError_t {
+        if (need <= B.cap_cell) return hipSuccess;
+        const int64_t nc = std::max<int64_t>({need, B.cap_cell + B.cap_cell / 2, 4096});
+        VPMX_CHK(grow_keep(&B.tb, nc, B.cap_cell, keep, st));
+        VPMX_CHK(grow_keep(&B.te, nc, B.cap_cell, keep, st));
+        VPMX_CHK(grow_keep(&B.pre, nc, B.cap_cell, keep, st));
+        VPMX_CHK(grow_keep(&B.lev, nc, B.cap_cell, keep, st));
+        VPMX_CHK(grow_keep(&B.child0, nc, B.cap_cell, keep, st));
+        VPMX_CHK(grow_keep(&B.nchild, nc, B.cap_cell, keep, st));
+        VPMX_CHK(grow_keep(&S.parent, nc, B.cap_cell, keep, st));
+        B.cap_cell = nc;
+        return hipSuccess;
+    };
+    auto grow_level = [&](int64_t m) -> hipError_t {
+        if (m + 1 <= B.cap_lev) return hipSuccess;
+        const int64_t c = std::max<int64_t>({m + 1, B.cap_lev + B.cap_lev / 2, 4096});
+        VPMX_CHK(grow_keep(&B.nk, c, 0, 0, st));
+        VPMX_CHK(grow_keep(&B.flag, c, 0, 0, st));
+        VPMX_CHK(grow_keep(&B.off, c, 0, 0, st));
+        VPMX_CHK(grow_keep(&B.foff, c, 0, 0, st));
+        VPMX_CHK(grow_keep(&B.kb, 8 * c, 0, 0, st));
+        VPMX_CHK(grow_keep(&B.ke, 8 * c, 0, 0, st));
+        VPMX_CHK(grow_keep(&B.ko, 8 * c, 0, 0, st));
+        B.cap_lev = c;
+        return hipSuccess;
+    };
+    auto scan = [&](const int32_t *in, int32_t *out, int64_t len) -> hipError_t {
+        size_t need = 0;
+        VPMX_CHK(rocprim::exclusive_scan(nullptr, need, in, out, (int32_t)0, (size_t)len,
+                                         rocprim::plus<int32_t>(), st));
+        if (need > B.scan_bytes) {
+            if (B.scan_tmp) (void)hipFree(B.scan_tmp);
+            B.scan_tmp = nullptr;
+            B.scan_bytes = 0;
+            VPMX_CHK(hipMalloc((void **)&B.scan_tmp, need));
+            B.scan_bytes = need;
+        }
+        size_t bytes = B.scan_bytes;
+        return rocprim::exclusive_scan(B.scan_tmp, bytes, in, out, (int32_t)0, (size_t)len,
+                                       rocprim::plus<int32_t>(), st);
+    };
+
+    VPMX_CHK(grow_cells(1, 0));
+    const int32_t root[2] = {0, (int32_t)n};
+    const int32_t zero = 0, none = -1;
+    const uint64_t zpre = 0;
+    VPMX_CHK(hipMemcpyAsync(B.tb, &root[0], sizeof(int32_t), hipMemcpyHostToDevice, st));
+    VPMX_CHK(hipMemcpyAsync(B.te, &root[1], sizeof(int32_t), hipMemcpyHostToDevice, st));
+    VPMX_CHK(hipMemcpyAsync(B.lev, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    VPMX_CHK(hipMemcpyAsync(B.pre, &zpre, sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    VPMX_CHK(hipMemcpyAsync(S.parent, &none, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    std::vector<std::pair<int64_t, int64_t>> levels;   // [lb, le) per level
+    std::vector<int64_t> nsplit;                       // parents per level
+    int64_t lb = 0, le = 1;
+    for (;;) {
+        const int64_t m = le - lb;
+        VPMX_CHK(grow_level(m));
+        tree_split_kernel<<<gridn(m + 1, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
+            S.kout, B.tb, B.te, B.pre, B.lev, lb, m, ncrit, B.nk, B.flag, B.kb, B.ke, B.ko);
+        VPMX_CHK(hipGetLastError());
+        VPMX_CHK(scan(B.nk, B.off, m + 1));
+        VPMX_CHK(scan(B.flag, B.foff, m + 1));
+        VPMX_CHK(hipMemcpyAsync(&B.h_tot[0], B.off + m, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        VPMX_CHK(hipMemcpyAsync(&B.h_tot[1], B.foff + m, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        VPMX_CHK(hipStreamSynchronize(st));
+        const int64_t total = B.h_tot[0];
+        nsplit.push_back(B.h_tot[1]);
+        levels.push_back({lb, le});
+        VPMX_CHK(grow_cells(le + total, le));
+        tree_emit_kernel<<<gridn(m, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
+            lb, m, le, B.nk, B.off, B.kb, B.ke, B.ko, B.tb, B.te, B.pre, B.lev, B.child0, B.nchild,
+            S.parent);
+        VPMX_CHK(hipGetLastError());
+        if (total == 0) break;
+        lb = le;
+        le += total;
+    }
+    const int64_t nc = le;
+    // device cell arrays and lists
+    if (nc > S.cap_cells) {
+        const int64_t cap = std::max<int64_t>(nc, S.cap_cells + S.cap_cells / 2);
+        VPMX_CHK(grow_keep(&S.cells, cap, 0, 0, st));
+        VPMX_CHK(grow_keep(&S.leaves, cap, 0, 0, st));
+        VPMX_CHK(grow_keep(&S.parents, cap, 0, 0, st));
+        VPMX_CHK(grow_keep(&S.kids, cap, 0, 0, st));
+        VPMX_CHK(grow_keep(&S.m2l_off, cap + 1, 0, 0, st));
+        VPMX_CHK(grow_keep(&S.p2p_off, cap + 1, 0, 0, st));
+        S.cap_cells = cap;
+    }
+    if (nc + 1 > B.cap_flags) {
+        const int64_t cap = std::max<int64_t>(nc + 1, B.cap_flags + B.cap_flags / 2);
+        VPMX_CHK(grow_keep(&B.lflag, cap, 0, 0, st));
+        VPMX_CHK(grow_keep(&B.pflag, cap, 0, 0, st));
+        VPMX_CHK(grow_keep(&B.lscan, cap, 0, 0, st));
+        VPMX_CHK(grow_keep(&B.pscan, cap, 0, 0, st));
+        B.cap_flags = cap;
+    }
+    int32_t *pflag = B.pflag;
+    tree_cells_kernel<<<gridn(nc + 1, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
+        nc, B.tb, B.te, B.pre, B.lev, B.child0, B.nchild, S.ssig, pl.lo[0], pl.lo[1], pl.lo[2],
+        pl.ext, S.cells, B.lflag, pflag);
+    VPMX_CHK(hipGetLastError());
+    for (int l = (int)levels.size() - 2; l >= 0; l--) {
+        const int64_t m = levels[l].second - levels[l].first;
+        tree_smax_kernel<<<gridn(m, FMM_BLOCK), FMM_BLOCK, 0, st>>>(levels[l].first, m, S.cells);
+        VPMX_CHK(hipGetLastError());
+    }
+    VPMX_CHK(scan(B.lflag, B.lscan, nc + 1));
+    VPMX_CHK(scan(pflag, B.pscan, nc + 1));
+    tree_lists_kernel<<<gridn(nc, FMM_BLOCK), FMM_BLOCK, 0, st>>>(nc, B.lflag, B.lscan, B.pscan,
+                                                                  S.leaves, S.parents, S.kids);
+    VPMX_CHK(hipGetLastError());
+    // host metadata: per-level ranges of parents and kids
+    const int nlev = (int)levels.size();
+    pl.cells.clear();
+    pl.level_parents.assign(nlev, {0, 0});
+    pl.level_kids.assign(nlev, {0, 0});
+    int64_t np_ = 0;
+    for (int l = 0; l < nlev; l++) {
+        pl.level_parents[l] = {(int)np_, (int)nsplit[l]};
+        np_ += nsplit[l];
+        if (l > 0)
+            pl.level_kids[l] = {(int)(levels[l].first - 1),
+                                (int)(levels[l].second - levels[l].first)};
+    }
+    pl.nleaves = nc - np_;
+    pl.ncells = nc;
+    *nc_out = nc;
+    return hipSuccess;
 }
 
 hipError_t fmm_p2p_chunks(hipStream_t st, FmmState &S, int64_t nleaf)

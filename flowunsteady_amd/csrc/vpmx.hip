@@ -708,9 +708,8 @@ void free_fmm(FmmState *S)
     for (void *p : dptrs)
         if (p) (void)hipFree(p);
     if (B.h_totals) (void)hipHostFree(B.h_totals);
-    if (S->h_keys) (void)hipHostFree(S->h_keys);
     if (S->h_chunk_tot) (void)hipHostFree(S->h_chunk_tot);
-    if (S->h_ssig) (void)hipHostFree(S->h_ssig);
+    free_tree_bufs(S->tree);
     delete S;
 }
 
@@ -723,12 +722,6 @@ int fmm_reserve_n(vpmx_ctx *c, FmmState &S, int64_t n)
     VPMX_TRY(dalloc(c, &S.ssig, cap));
     S.sort_bytes = fmm_sort_temp_bytes(cap);
     VPMX_TRY(dalloc(c, &S.sort_tmp, S.sort_bytes));
-    if (S.h_keys) (void)hipHostFree(S.h_keys);
-    if (S.h_ssig) (void)hipHostFree(S.h_ssig);
-    S.h_keys = nullptr;
-    S.h_ssig = nullptr;
-    VPMX_HIP(c, hipHostMalloc((void **)&S.h_keys, cap * sizeof(uint64_t), hipHostMallocDefault));
-    VPMX_HIP(c, hipHostMalloc((void **)&S.h_ssig, cap * sizeof(double), hipHostMallocDefault));
     S.cap_n = cap;
     return VPMX_OK;
 }
@@ -773,84 +766,33 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = f
     VPMX_HIP(c, launch_fmm_keys(st, f, lo, scale, S.kin, c->s.vin));
     VPMX_HIP(c, launch_fmm_sort(st, S.sort_tmp, S.sort_bytes, S.kin, S.kout, c->s.vin, c->s.perm, n));
     VPMX_HIP(c, launch_fmm_gather(st, f, c->s.perm, c->s.src64, S.ssig));
-    VPMX_HIP(c, hipMemcpyAsync(S.h_keys, S.kout, n * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    VPMX_HIP(c, hipMemcpyAsync(S.h_ssig, S.ssig, n * sizeof(double), hipMemcpyDeviceToHost, st));
-    VPMX_TRY(sync(c));
 
-    // host: tree and traversal
+    // device: octree of the sorted keys (one count read-back per level)
     FmmPlan &pl = S.plan;
     const double t0 = now_ms();
     for (int k = 0; k < 3; k++) pl.lo[k] = lo[k];
     pl.ext = ext;
-    fmm_build_tree(pl, S.h_keys, S.h_ssig, n, P->ncrit);
-    const double t1 = now_ms();
-    const int64_t nc = (int64_t)pl.cells.size();
-    int maxlev = 0;
-    for (const auto &C : pl.cells) maxlev = std::max(maxlev, C.level);
-    pl.leaves.clear();
-    pl.parents.clear();
-    pl.kids.clear();
-    pl.parent.assign(nc, -1);
-    pl.level_parents.assign(maxlev + 1, {0, 0});
-    pl.level_kids.assign(maxlev + 1, {0, 0});
-    std::vector<FmmCellDev> hc(nc);
-    for (int64_t i = 0; i < nc; i++) {
-        const FmmCellHost &C = pl.cells[i];
-        hc[i] = FmmCellDev{(int32_t)C.begin, (int32_t)C.end, (int32_t)C.child0, C.nchild,
-                           C.c[0], C.c[1], C.c[2], C.R, C.smax};
-        if (C.nchild == 0) pl.leaves.push_back((int32_t)i);
-        for (int k = 0; k < C.nchild; k++) pl.parent[C.child0 + k] = (int32_t)i;
-    }
-    // cells are in breadth-first order, so one pass groups them by level
-    for (int64_t i = 0, cur = -1; i < nc; i++) {
-        const int lev = pl.cells[i].level;
-        for (; cur < lev; cur++) {
-            pl.level_parents[cur + 1].first = (int)pl.parents.size();
-            pl.level_kids[cur + 1].first = (int)pl.kids.size();
-        }
-        if (pl.cells[i].nchild) {
-            pl.parents.push_back((int32_t)i);
-            pl.level_parents[lev].second++;
-        }
-        if (lev > 0) {
-            pl.kids.push_back((int32_t)i);
-            pl.level_kids[lev].second++;
-        }
-    }
-    // device: tables (once per p), tree, lists, expansions
+    int64_t nc = 0;
+    VPMX_HIP(c, fmm_build_tree_device(st, S, n, P->ncrit, pl, &nc));
+    // device: tables (once per p), expansions
     if (S.dev_tab_p != P->p) {
         fmm_build_tables(S.tab, P->p);
         VPMX_TRY(fmm_tables_to_device(c, S));
     }
     const int K = S.tab.K;
-    if (nc > S.cap_cells) {
-        const int64_t cap = std::max<int64_t>(nc, S.cap_cells + S.cap_cells / 2);
-        VPMX_TRY(dalloc(c, &S.cells, cap));
-        VPMX_TRY(dalloc(c, &S.leaves, cap));
-        VPMX_TRY(dalloc(c, &S.parents, cap));
-        VPMX_TRY(dalloc(c, &S.kids, cap));
-        VPMX_TRY(dalloc(c, &S.parent, cap));
-        VPMX_TRY(dalloc(c, &S.m2l_off, cap + 1));
-        VPMX_TRY(dalloc(c, &S.p2p_off, cap + 1));
-        S.cap_cells = cap;
-    }
     if (nc * 3 * K > S.cap_coef) {
         const int64_t cap = std::max<int64_t>(nc * 3 * K, S.cap_coef + S.cap_coef / 2);
         VPMX_TRY(dalloc(c, &S.M, cap));
         VPMX_TRY(dalloc(c, &S.L, cap));
         S.cap_coef = cap;
     }
-    VPMX_TRY(upload_vec(c, S.cells, hc));
-    VPMX_TRY(upload_vec(c, S.leaves, pl.leaves));
-    VPMX_TRY(upload_vec(c, S.parents, pl.parents));
-    VPMX_TRY(upload_vec(c, S.kids, pl.kids));
-    VPMX_TRY(upload_vec(c, S.parent, pl.parent));
     VPMX_TRY(sync(c));
-    const double t2 = now_ms();
+    const double t1 = now_ms();
+    const double t2 = t1;
     // device: dual tree traversal -> M2L and P2P lists (CSR by target)
     DttResult dr;
     VPMX_HIP(c, fmm_traverse_device(st, S, nc, P->theta, sigcut, dr));
-    VPMX_HIP(c, fmm_p2p_chunks(st, S, (int64_t)pl.leaves.size()));
+    VPMX_HIP(c, fmm_p2p_chunks(st, S, pl.nleaves));
     const int64_t nm2l = dr.nm2l, np2p = dr.np2p, pairs = dr.p2p_pairs;
     const double t3 = now_ms();
     if (lists_only) {
@@ -858,7 +800,7 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = f
         S.stats[1] = t3 - t2;
         S.stats[2] = t2 - t1;
         S.stats[3] = (double)nc;
-        S.stats[4] = (double)pl.leaves.size();
+        S.stats[4] = (double)pl.nleaves;
         S.stats[5] = (double)nm2l;
         S.stats[6] = (double)np2p;
         S.stats[7] = (double)pairs;
@@ -869,7 +811,7 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = f
     d.P = P->p;
     d.K = K;
     d.ncell = (int)nc;
-    d.nleaf = (int)pl.leaves.size();
+    d.nleaf = (int)pl.nleaves;
     d.cells = S.cells;
     d.leaves = S.leaves;
     d.parents = S.parents;
@@ -918,11 +860,11 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = f
         VPMX_HIP(c, launch_fmm_eval(st, d, pl, P->kernel, P->precision, f, reset));
     }
     VPMX_TRY(sync(c));
-    S.stats[0] = t1 - t0;   // host tree build
+    S.stats[0] = t1 - t0;   // device tree build (wall, incl. its read-backs)
     S.stats[1] = t3 - t2;   // device traversal (incl. list sort)
-    S.stats[2] = t2 - t1;   // tree arrays to the device
+    S.stats[2] = 0.0;       // no tree data crosses from the host
     S.stats[3] = (double)nc;
-    S.stats[4] = (double)pl.leaves.size();
+    S.stats[4] = (double)pl.nleaves;
     S.stats[5] = (double)nm2l;
     S.stats[6] = (double)np2p;
     S.stats[7] = (double)pairs;

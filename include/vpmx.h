@@ -324,9 +324,11 @@ int vpmx_set_graph(vpmx_ctx *ctx, int32_t on);
  * Returns the summed milliseconds and the launch count since the last reset. */
 int vpmx_set_timing(vpmx_ctx *ctx, int32_t on);
 int vpmx_get_timing(vpmx_ctx *ctx, int32_t which, double *total_ms, int64_t *count);
-/* Statistics of the last FMM evaluation on ctx, in this order: host tree
- * build ms, host traversal ms, list upload ms, cells, leaves, M2L pairs, P2P
- * leaf pairs, P2P particle pairs, device evaluation ms (when timing is on).
+/* Statistics of the last FMM evaluation on ctx, in this order: tree build ms
+ * (on the device, wall time including its one count read-back per level),
+ * traversal ms (on the device, including the list sort), host-to-device tree
+ * copy ms (0: the tree is built where it is used), cells, leaves, M2L pairs,
+ * P2P leaf pairs, P2P particle pairs, device evaluation ms (when timing is on).
  * Writes min(n, 9) values. */
 int vpmx_fmm_stats(vpmx_ctx *ctx, double *out, int32_t n);
 /* Host-only: build the FMM tree and traversal lists for positions X (n,3)
