@@ -367,9 +367,23 @@ def fmm_section(ctx, n, orders, log, cpu=True, nref=65536):
     return res
 
 
+def _affinity():
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return None
+
+
+# Read before any OpenMP region runs: with OMP_PROC_BIND set, the OpenMP
+# runtime pins the calling thread to one place, after which this process
+# reports a single allowed CPU.
+_START_AFFINITY = _affinity()
+
+
 def cpu_info():
     """The host side of the CPU baseline: CPU model, logical CPUs, the CPUs
-    this process may run on, and the OpenMP settings the oracle runs with."""
+    this process was allowed at start, and the OpenMP settings the oracle runs
+    with."""
     model = None
     try:
         with open("/proc/cpuinfo") as fh:
@@ -379,11 +393,7 @@ def cpu_info():
                     break
     except OSError:
         pass
-    try:
-        affinity = len(os.sched_getaffinity(0))
-    except (AttributeError, OSError):
-        affinity = None
-    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": _START_AFFINITY,
             "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
             "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
             "OMP_PLACES": os.environ.get("OMP_PLACES")}

@@ -387,10 +387,11 @@ __global__ __launch_bounds__(FMM_BLOCK) void m2l_kernel(const FmmCellDev *__rest
 // (k, n) of B and (row, n) of C in block j): A at row + 4 j + 16 k, B at
 // n + 4 j + 16 k, C at n + 4 j + 16 row.
 //
-// E of a source by the recurrence of the derivatives of 1/r,
-//     n R^2 E_k = -(2n - 1) sum_i k_i r_i E_{k - e_i}
-//                 - (n - 1) sum_i k_i (k_i - 1) E_{k - 2 e_i},  n = |k|,
-// degree by degree, 16 lanes per source.
+// E of a source from the Taylor coefficients t_k = E_k / k! of 1/r, whose
+// recurrence (that of E_k divided by k!; the k_i factors cancel)
+//     n R^2 t_k = -(2n - 1) sum_i r_i t_{k - e_i} - (n - 1) sum_i t_{k - 2 e_i},
+// n = |k|, runs degree by degree, 16 lanes per source, in place; a last pass
+// multiplies by k!.
 // Tile t -> (ta, tb), in the order the tile index table holds them.
 template <int P>
 struct M2LTiles {
@@ -423,25 +424,27 @@ __device__ __forceinline__ void m2l_mfma_one(double (&C)[m2l_ntile1(P)],
     }
 }
 
-// Eight tiles: one 16-byte LDS read brings this lane's eight E indices, then
-// eight independent gathers of E feed eight MFMAs.
+// Eight tiles: one 16-byte LDS read brings this lane's eight E byte offsets
+// (one 16-bit half each, so extracting and adding the row base is a single
+// instruction), then eight independent gathers of E feed eight MFMAs.
 template <int P, int T0, int... J>
 __device__ __forceinline__ void m2l_mfma_batch(double (&C)[m2l_ntile1(P)],
                                                const double (&B)[m2l_ntile1(P)],
-                                               const uint16_t *tl, const double *Eb,
+                                               const uint16_t *tl, const char *Eb,
                                                std::integer_sequence<int, J...>)
 {
     const uint4 q = *reinterpret_cast<const uint4 *>(tl + T0);
     const unsigned w[4] = {q.x, q.y, q.z, q.w};
     double a[8];
-    ((a[J] = Eb[(w[J >> 1] >> (16 * (J & 1))) & 0xffffu]), ...);
+    ((a[J] = *reinterpret_cast<const double *>(
+          Eb + ((J & 1) ? (w[J >> 1] >> 16) : (w[J >> 1] & 0xffffu)))), ...);
     (m2l_mfma_one<P, T0 + J>(C, B, a[J]), ...);
 }
 
 template <int P, int T0>
 __device__ __forceinline__ void m2l_mfma_group(double (&C)[m2l_ntile1(P)],
                                                const double (&B)[m2l_ntile1(P)],
-                                               const uint16_t *tl, const double *Eb)
+                                               const uint16_t *tl, const char *Eb)
 {
     if constexpr (T0 < m2l_ntiles(P)) {
         m2l_mfma_batch<P, T0>(C, B, tl, Eb, std::make_integer_sequence<int, 8>{});
@@ -474,19 +477,20 @@ __global__ __launch_bounds__(FMM_BLOCK) void m2l_prescale_kernel(double *__restr
 // Persistent: block b serves cells k G + (b mod 8) W + b / 8 (k = 0, 1, ...;
 // G = gridDim.x = 8 W), so each XCD (workgroups go round-robin over the 8
 // XCDs) sweeps contiguous runs of cells whose sources overlap in its L2, and
-// the tables are staged in LDS once per block.  `erec` packs, per k, the E
-// indices of k - e_i and k - 2 e_i (K where absent: the zero slot) and k:
-// (j1x | j1y << 16, j1z | j2x << 16, j2y | j2z << 16, kx | ky << 8 | kz << 16).
+// the tables are staged in LDS once per block.  `erec` packs, per k, the
+// byte offsets in a row of k - e_i and k - 2 e_i (8 K where absent: the zero
+// slot) and k!: (j1x | j1y << 16, j1z | j2x << 16, j2y | j2z << 16, k!).
 // The four sources' E are rows of m2l_erow(P) doubles: K + 1 (the zero slot at
 // K), rounded so that the rows start 16 banks apart (the layout measured
-// fastest among the ones tried).
+// fastest among the ones tried).  Absent sources and the zero fourth B column
+// read the multipole row `zrow` (all zeros).
 template <int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P <= 8 ? 2 : 1)))
 void m2l_mfma_kernel(
     const FmmCellDev *__restrict__ cells, int ncell, const int64_t *__restrict__ off,
     const int32_t *__restrict__ src, const int4 *__restrict__ erec,
     const uint16_t *__restrict__ tile_idx, const double *__restrict__ lscale,
-    const double *__restrict__ M, double *__restrict__ L)
+    const double *__restrict__ M, int64_t zrow, double *__restrict__ L)
 {
     constexpr int K = m2l_ncoef(P), NT = m2l_ntile1(P), N8 = m2l_ntiles8(P), ER = m2l_erow(P);
     __shared__ double E[4 * ER];
@@ -499,8 +503,9 @@ void m2l_mfma_kernel(
     __syncthreads();
     const int nq = lane & 3, blk = (lane >> 2) & 3, hi = lane >> 4, sub = lane & 15;
     const uint16_t *tl = tab + (nq + 4 * hi) * N8;   // A element (row = lane & 3, k = lane >> 4)
-    const double *Eb = E + blk * ER;
+    const char *Eb = reinterpret_cast<const char *>(E + blk * ER);
     double *Ej = E + hi * ER;                   // E of source hi, built by lanes 16 hi ..
+    const char *Ejb = reinterpret_cast<const char *>(Ej);
     const int G = gridDim.x, W = G >> 3;
     const int first = (int)(blockIdx.x & 7) * W + (int)(blockIdx.x >> 3);
     for (int c = first; c < ncell; c += G) {
@@ -531,11 +536,11 @@ void m2l_mfma_kernel(
             // a 0/1 mask, so the loads issue back to back instead of one guarded
             // load at a time (the multipoles are finite).
             double Bv[NT];
-            const double bmask = (blk < nb && nq < 3) ? 1.0 : 0.0;
-            const double *Mb = M + ((size_t)ib * 3 + (nq < 3 ? nq : 0)) * K;
+            const double *Mb =
+                M + ((blk < nb && nq < 3) ? (size_t)ib * 3 + nq : (size_t)zrow * 3) * K;
 #pragma unroll
-            for (int ta = 0; ta < NT; ++ta)
-                Bv[ta] = Mb[min(4 * ta + hi, K - 1)] * (4 * ta + hi < K ? bmask : 0.0);
+            for (int ta = 0; ta < NT - 1; ++ta) Bv[ta] = Mb[4 * ta + hi];
+            Bv[NT - 1] = Mb[min(4 * (NT - 1) + hi, K - 1)] * (4 * (NT - 1) + hi < K ? 1.0 : 0.0);
             // E of source hi (absent sources: r = (1, 0, 0), finite E, zero B)
             double r0 = 1.0, r1 = 0.0, r2 = 0.0;
             if (hi < nb) {
@@ -550,21 +555,22 @@ void m2l_mfma_kernel(
             const double R2 = r0 * r0 + r1 * r1 + r2 * r2, iR2 = 1.0 / R2;
             wave_lds_sync();   // the previous group's reads of E are done
             if (sub == 0) Ej[0] = 1.0 / sqrt(R2);
+            auto ld = [&](unsigned b) { return *reinterpret_cast<const double *>(Ejb + b); };
 #pragma unroll
             for (int n = 1; n <= P; ++n) {
                 wave_lds_sync();
                 const double f1 = -(2.0 * n - 1.0) / n * iR2, f2 = -(n - 1.0) / n * iR2;
                 for (int k = m2l_ncoef(n - 1) + sub; k < m2l_ncoef(n); k += 16) {
                     const int4 q = rec[k];
-                    const double kx = q.w & 255, ky = (q.w >> 8) & 255, kz = (q.w >> 16) & 255;
-                    const double s1 = kx * r0 * Ej[q.x & 0xffff] + ky * r1 * Ej[q.x >> 16] +
-                                      kz * r2 * Ej[q.y & 0xffff];
-                    const double s2 = kx * (kx - 1.0) * Ej[q.y >> 16] +
-                                      ky * (ky - 1.0) * Ej[q.z & 0xffff] +
-                                      kz * (kz - 1.0) * Ej[q.z >> 16];
+                    const unsigned qx = (unsigned)q.x, qy = (unsigned)q.y, qz = (unsigned)q.z;
+                    const double s1 =
+                        r0 * ld(qx & 0xffffu) + r1 * ld(qx >> 16) + r2 * ld(qy & 0xffffu);
+                    const double s2 = ld(qy >> 16) + ld(qz & 0xffffu) + ld(qz >> 16);
                     Ej[k] = f1 * s1 + f2 * s2;
                 }
             }
+            wave_lds_sync();
+            for (int k = 1 + sub; k < K; k += 16) Ej[k] *= (double)(unsigned)rec[k].w;   // E = k! t
             wave_lds_sync();
             m2l_mfma_group<P, 0>(Cacc, Bv, tl, Eb);
         }
@@ -1301,16 +1307,18 @@ void fmm_build_tables(FmmTables &T, int P)
                 j[i] = T.rec1[4 * k + i] >= 0 ? T.rec1[4 * k + i] : K;
                 j[3 + i] = T.rec2[4 * k + i] >= 0 ? T.rec2[4 * k + i] : K;
             }
+            // byte offsets (8 j) of the entries the recurrence reads, and k!
+            // (<= 10! for the orders of the matrix-core M2L)
             const int *m = &M.mi[3 * k];
-            T.m2l_erec[4 * k] = j[0] | (j[1] << 16);
-            T.m2l_erec[4 * k + 1] = j[2] | (j[3] << 16);
-            T.m2l_erec[4 * k + 2] = j[4] | (j[5] << 16);
-            T.m2l_erec[4 * k + 3] = m[0] | (m[1] << 8) | (m[2] << 16);
+            T.m2l_erec[4 * k] = 8 * j[0] | (8 * j[1] << 16);
+            T.m2l_erec[4 * k + 1] = 8 * j[2] | (8 * j[3] << 16);
+            T.m2l_erec[4 * k + 2] = 8 * j[4] | (8 * j[5] << 16);
+            T.m2l_erec[4 * k + 3] = (int)(fact(m[0]) * fact(m[1]) * fact(m[2]));
         }
         T.m2l_tile_idx.clear();
         if (P >= 1 && P <= M2L_MFMA_MAX_P) {
             const int N8 = m2l_ntiles8(P);
-            T.m2l_tile_idx.assign((size_t)16 * N8, (uint16_t)K);
+            T.m2l_tile_idx.assign((size_t)16 * N8, (uint16_t)(8 * K));   // byte offsets
             int t = 0;
             for (int ta = 0; ta < m2l_ntile1(P); ta++)
                 for (int tb = 0; tb < m2l_nrows(P, ta); tb++, t++)
@@ -1319,7 +1327,7 @@ void fmm_build_tables(FmmTables &T, int P)
                         if (a >= K || b >= K) continue;
                         const int *ma = &M.mi[3 * a], *mb = &M.mi[3 * b];
                         const int s = M.idx(ma[0] + mb[0], ma[1] + mb[1], ma[2] + mb[2]);
-                        if (s >= 0) T.m2l_tile_idx[(size_t)e * N8 + t] = (uint16_t)s;
+                        if (s >= 0) T.m2l_tile_idx[(size_t)e * N8 + t] = (uint16_t)(8 * s);
                     }
         }
     }
@@ -2837,12 +2845,14 @@ hipError_t launch_fmm_eval(hipStream_t st, const FmmDev &d, const FmmPlan &pl, i
     if (d.ncell > 0 && P >= 1 && P <= M2L_MFMA_MAX_P) {
         const int64_t nm = (int64_t)d.ncell * 3 * K;
         m2l_prescale_kernel<<<gridn(nm, FMM_BLOCK), FMM_BLOCK, 0, st>>>(d.M, nm, K, d.m2l_bscale);
+        // the zero multipole row read for absent sources (M holds ncell + 1 rows)
+        fmm_zero_kernel<<<gridn(3 * K, FMM_BLOCK), FMM_BLOCK, 0, st>>>(d.M + nm, 3 * K);
 #define VPMX_M2LM(PP)                                                                           \
     case PP: {                                                                                  \
         const int grid = m2l_mfma_grid((const void *)m2l_mfma_kernel<PP>, PP, d.ncell);          \
         m2l_mfma_kernel<PP><<<grid, 64, 0, st>>>(d.cells, d.ncell, d.m2l_off, d.m2l_src,         \
                                                  d.m2l_erec, d.m2l_tile_idx, d.m2l_lscale, d.M, \
-                                                 d.L);                                          \
+                                                 (int64_t)d.ncell, d.L);                        \
         break;                                                                                  \
     }
         switch (P) {

@@ -780,8 +780,8 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = f
         VPMX_TRY(fmm_tables_to_device(c, S));
     }
     const int K = S.tab.K;
-    if (nc * 3 * K > S.cap_coef) {
-        const int64_t cap = std::max<int64_t>(nc * 3 * K, S.cap_coef + S.cap_coef / 2);
+    if ((nc + 1) * 3 * K > S.cap_coef) {   // + the zero row of the matrix-core M2L
+        const int64_t cap = std::max<int64_t>((nc + 1) * 3 * K, S.cap_coef + S.cap_coef / 2);
         VPMX_TRY(dalloc(c, &S.M, cap));
         VPMX_TRY(dalloc(c, &S.L, cap));
         S.cap_coef = cap;
