@@ -1115,8 +1115,10 @@ std::vector<unsigned char> step_key(vpmx_ctx *c, const vpmx_uj_params *U,
 
 // Runs one step on the resident state: per-step inputs (freestream table,
 // Morton sort), then the step's device work, replayed from a graph when graphs
-// are on and the step has no host-side phase (the FMM builds its tree on the
-// host).  `uinf_fields` holds 3 values per field, or NULL for S->Uinf everywhere.
+// are on and the step has no host-side phase (the FMM sizes each level of its
+// tree and lists from a count read back to the host, so FMM steps run
+// eagerly).  `uinf_fields` holds 3 values per field, or NULL for S->Uinf
+// everywhere.
 int run_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S,
              const double *uinf_fields, double dt, int relax, bool rk3)
 {

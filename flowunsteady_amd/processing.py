@@ -81,21 +81,24 @@ def fluid_domain(pfield: "vpm.ParticleField", Pmin, Pmax, NDIVS, Oaxis=None, O=N
 def preprocessing_fluiddomain_pfield(maxsigma: float, maxmagGamma: float):
     """A function of a particle field that prepares it for probing, in place,
     as generate_preprocessing_fluiddomain_pfield does
-    (src/FLOWUnsteady_postprocessing.jl:187-240): core sizes above maxsigma are
-    set to maxsigma, NaN strengths to 1e-12 per component, strengths above
-    maxmagGamma scaled down to it (the reference writes `Gamma *= ...`, which
-    rebinds a local copy and leaves the particle as it was; here the clip is
-    applied), and strengths below 1e-14 in magnitude get 1e-14 added."""
+    (src/FLOWUnsteady_postprocessing.jl:187-240), static particles included:
+    core sizes above maxsigma are set to maxsigma, NaN strengths to 1e-12 per
+    component, and strengths below 1e-14 in magnitude get 1e-14 added.
+    Strengths above maxmagGamma are counted but left as they are: the
+    reference's `Gamma *= maxmagGamma / magGamma` rebinds its local name to a
+    scaled copy, so the particle keeps its strength, and that behaviour is
+    kept.  Returns (shrunk sigmas, NaN strengths, strengths above maxmagGamma),
+    the counts the reference prints."""
     def fn(pfield: "vpm.ParticleField", *args, **kwargs):
         n = pfield.np
         s = pfield.particles[vpm.SIGMA_INDEX, :n]
+        nsig = int(np.count_nonzero(s > maxsigma))
         np.minimum(s, maxsigma, out=s)
         G = pfield.particles[vpm.GAMMA_INDEX, :n]
         nan = np.any(np.isnan(G), axis=0)
         G[:, nan] = 1e-12
         mag = np.linalg.norm(G, axis=0)
-        big = mag > maxmagGamma
-        if np.any(big):
-            G[:, big] *= maxmagGamma / mag[big]
+        nbig = int(np.count_nonzero(mag > maxmagGamma))
         G[:, mag < 1e-14] += 1e-14
+        return nsig, int(np.count_nonzero(nan)), nbig
     return fn

@@ -173,10 +173,13 @@ def test_fluid_domain_grid_and_preprocessing():
     pf = vpm.ParticleField(4)
     vpm.add_particles(pf, np.zeros((4, 3)), np.array([[np.nan, 0, 0], [3, 4, 0], [0, 0, 0],
                                                       [1e-3, 0, 0]]), [0.5, 0.05, 0.2, 0.01])
-    processing.preprocessing_fluiddomain_pfield(0.1, 1.0)(pf)
+    counts = processing.preprocessing_fluiddomain_pfield(0.1, 1.0)(pf)
+    assert counts == (2, 1, 1)
     np.testing.assert_array_equal(vpm.get_sigma(pf), [0.1, 0.05, 0.1, 0.01])
     G = vpm.get_Gamma(pf)
     np.testing.assert_array_equal(G[0], [1e-12, 1e-12, 1e-12])
-    np.testing.assert_allclose(G[1], [0.6, 0.8, 0.0])
+    # |Gamma| = 5 > maxmagGamma is left as it is, as the reference's rebinding
+    # `Gamma *= ...` leaves the particle
+    np.testing.assert_array_equal(G[1], [3, 4, 0])
     assert np.all(G[2] > 0)            # zero strength lifted off zero
     np.testing.assert_array_equal(G[3], [1e-3, 0, 0])
