@@ -26,6 +26,10 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 
+# the tracer-range API (roctxRangePush/Pop; no-ops unless a tool is attached)
+LIBS = ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"]
+
+
 def build(verbose: bool = False) -> str:
     """Compile every source (in parallel, one hipcc per file) and link."""
     objs, procs = [], []
@@ -48,7 +52,7 @@ def build(verbose: bool = False) -> str:
     if failed:
         raise RuntimeError("\n".join(failed))
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, *LIBS, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -556,18 +556,38 @@ void m2l_mfma_kernel(
             wave_lds_sync();   // the previous group's reads of E are done
             if (sub == 0) Ej[0] = 1.0 / sqrt(R2);
             auto ld = [&](unsigned b) { return *reinterpret_cast<const double *>(Ejb + b); };
+            // each degree's records are read during the previous degree, so
+            // only the E reads sit on the chain from degree to degree
+            constexpr int MAXIT = ((P + 1) * (P + 2) / 2 + 15) / 16;
+            auto recs = [&](int n, uint3 (&q)[MAXIT]) {
+#pragma unroll
+                for (int it = 0; it < MAXIT; ++it) {
+                    const int4 r = rec[min(m2l_ncoef(n - 1) + sub + 16 * it, K - 1)];
+                    q[it] = make_uint3((unsigned)r.x, (unsigned)r.y, (unsigned)r.z);
+                }
+            };
+            uint3 qc[MAXIT];
+            recs(1, qc);
 #pragma unroll
             for (int n = 1; n <= P; ++n) {
                 wave_lds_sync();
+                uint3 qn[MAXIT];
+                if (n < P) recs(n + 1, qn);
                 const double f1 = -(2.0 * n - 1.0) / n * iR2, f2 = -(n - 1.0) / n * iR2;
-                for (int k = m2l_ncoef(n - 1) + sub; k < m2l_ncoef(n); k += 16) {
-                    const int4 q = rec[k];
-                    const unsigned qx = (unsigned)q.x, qy = (unsigned)q.y, qz = (unsigned)q.z;
-                    const double s1 =
-                        r0 * ld(qx & 0xffffu) + r1 * ld(qx >> 16) + r2 * ld(qy & 0xffffu);
-                    const double s2 = ld(qy >> 16) + ld(qz & 0xffffu) + ld(qz >> 16);
-                    Ej[k] = f1 * s1 + f2 * s2;
+#pragma unroll
+                for (int it = 0; it < MAXIT; ++it) {
+                    const int k = m2l_ncoef(n - 1) + sub + 16 * it;
+                    if (k < m2l_ncoef(n)) {
+                        const unsigned qx = qc[it].x, qy = qc[it].y, qz = qc[it].z;
+                        const double s1 =
+                            r0 * ld(qx & 0xffffu) + r1 * ld(qx >> 16) + r2 * ld(qy & 0xffffu);
+                        const double s2 = ld(qy >> 16) + ld(qz & 0xffffu) + ld(qz >> 16);
+                        Ej[k] = f1 * s1 + f2 * s2;
+                    }
                 }
+                if (n < P)
+#pragma unroll
+                    for (int it = 0; it < MAXIT; ++it) qc[it] = qn[it];
             }
             wave_lds_sync();
             for (int k = 1 + sub; k < K; k += 16) Ej[k] *= (double)(unsigned)rec[k].w;   // E = k! t

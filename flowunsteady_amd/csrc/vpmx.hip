@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "device_math.h"
 #include "fmm.h"
 #include "internal.h"
@@ -81,6 +83,21 @@ hipEvent_t take_event(Timer &t)
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
 }
+
+// A named host-side range for tracers (rocprofv3 --marker-trace shows the
+// library's phases on its timeline; a no-op when no tool is attached).  Not
+// opened while a graph is being captured.
+struct MarkScope {
+    bool open = false;
+    MarkScope(const vpmx_ctx *c, const char *name)
+    {
+        if (!c->capturing) open = roctxRangePush(name) >= 0;
+    }
+    ~MarkScope()
+    {
+        if (open) (void)roctxRangePop();
+    }
+};
 
 struct TimedScope {
     vpmx_ctx *c;
@@ -730,6 +747,7 @@ int fmm_reserve_n(vpmx_ctx *c, FmmState &S, int64_t n)
 // near-field work items only (for the viscous sums), no evaluation.
 int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = false)
 {
+    MarkScope ms(c, lists_only ? "vpmx fmm lists" : "vpmx fmm uj");
     if (!c->fmm) c->fmm = new (std::nothrow) FmmState();
     if (!c->fmm) return fail(c, VPMX_ENOMEM, "out of host memory");
     FmmState &S = *c->fmm;
@@ -773,7 +791,10 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = f
     for (int k = 0; k < 3; k++) pl.lo[k] = lo[k];
     pl.ext = ext;
     int64_t nc = 0;
-    VPMX_HIP(c, fmm_build_tree_device(st, S, n, P->ncrit, pl, &nc));
+    {
+        MarkScope ms(c, "vpmx fmm tree");
+        VPMX_HIP(c, fmm_build_tree_device(st, S, n, P->ncrit, pl, &nc));
+    }
     // device: tables (once per p), expansions
     if (S.dev_tab_p != P->p) {
         fmm_build_tables(S.tab, P->p);
@@ -791,7 +812,10 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = f
     const double t2 = t1;
     // device: dual tree traversal -> M2L and P2P lists (CSR by target)
     DttResult dr;
-    VPMX_HIP(c, fmm_traverse_device(st, S, nc, P->theta, sigcut, dr));
+    {
+        MarkScope ms(c, "vpmx fmm traversal");
+        VPMX_HIP(c, fmm_traverse_device(st, S, nc, P->theta, sigcut, dr));
+    }
     VPMX_HIP(c, fmm_p2p_chunks(st, S, pl.nleaves));
     const int64_t nm2l = dr.nm2l, np2p = dr.np2p, pairs = dr.p2p_pairs;
     const double t3 = now_ms();
@@ -857,6 +881,7 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = f
     const double ms_before = c->timer.total_ms[T_FMM];
     {
         TimedScope ts(c, T_FMM);
+        MarkScope ms(c, "vpmx fmm evaluation");
         VPMX_HIP(c, launch_fmm_eval(st, d, pl, P->kernel, P->precision, f, reset));
     }
     VPMX_TRY(sync(c));
@@ -893,6 +918,7 @@ int enqueue_uj(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool sort = true
         if (B.nfields) return fail(c, VPMX_EUNSUPPORTED, "the FMM does not run on batched fields");
         return run_fmm(c, P, reset);
     }
+    MarkScope ms(c, "vpmx direct uj");
     if (B.nfields && (B.items_block != B.block || B.items_off != B.off))
         return fail(c, VPMX_EINVAL, "batch work list not prepared");
     if (P->precision == VPMX_FP64 && B.nfields) {
@@ -1012,6 +1038,7 @@ int enqueue_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S
 int core_spreading_reset(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S,
                          bool force, int *did, int *iters, double res[3])
 {
+    MarkScope ms(c, "vpmx core spreading");
     DeviceField &f = c->f;
     Scratch &s = c->s;
     hipStream_t st = c->stream;
@@ -1122,6 +1149,7 @@ std::vector<unsigned char> step_key(vpmx_ctx *c, const vpmx_uj_params *U,
 int run_step(vpmx_ctx *c, const vpmx_uj_params *U, const vpmx_step_params *S,
              const double *uinf_fields, double dt, int relax, bool rk3)
 {
+    MarkScope ms(c, rk3 ? "vpmx rk3 step" : "vpmx euler step");
     BatchState &B = c->b;
     if (S->sfs != VPMX_SFS_NONE && U->kernel == VPMX_KERNEL_SINGULAR)
         return fail(c, VPMX_EINVAL, "the SFS model needs a smooth kernel");
