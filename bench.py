@@ -290,9 +290,10 @@ def ring_section(ctx, log, cpu=True, steps=20, dt=0.01):
 def fmm_section(ctx, n, orders, log, cpu=True, nref=65536):
     """UJ by the FMM on the rotor-wake workload (3 helical tip filaments +
     root line, Gamma = 0.05 t + N(0, 5e-4), sigma = 0.03), ncrit = 50,
-    theta = 0.4, mixed near field, FP64 expansions; errors against the FP64
-    direct sum at `nref` sampled targets (the FP64 direct path equals the
-    oracle to 1e-12)."""
+    theta = 0.4, mixed near field, FP64 expansions; each order with the
+    core-aware acceptance (sigma_cut 7), and p = 8 also with the geometric one
+    FLOWUnsteady uses by default; errors against the FP64 direct sum at `nref`
+    sampled targets (the FP64 direct path equals the oracle to 1e-12)."""
     from flowunsteady_amd import _native as N
     from flowunsteady_amd import workloads
     X, G, s, v = workloads.rotor_wake(n=n, seed=3)
@@ -308,9 +309,14 @@ def fmm_section(ctx, n, orders, log, cpu=True, nref=65536):
            "orders": {}}
     P = N.UJParams()
     P.kernel, P.precision, P.transposed, P.reset = N.GAUSSIANERF, N.MIXED, 1, 1
-    P.fmm, P.ncrit, P.theta, P.sigma_cut = 1, 50, 0.4, -1.0   # core-aware criterion
-    for p in orders:
-        P.p = p
+    P.fmm, P.ncrit, P.theta = 1, 50, 0.4
+    # the core-aware criterion at every order, then FLOWUnsteady's default
+    # geometric one (nonzero_sigma = false) at p = 8
+    runs = [(p, -1.0, str(p)) for p in orders]
+    if 8 in orders:
+        runs.append((8, 0.0, "8_geometric"))
+    for p, sigcut, key in runs:
+        P.p, P.sigma_cut = p, sigcut
         ctx.check(N.lib().vpmx_uj_resident(ctx.ptr, ctypes.byref(P)))   # warm (tables)
         ctx.set_timing(True)
         reps = 2
@@ -323,7 +329,8 @@ def fmm_section(ctx, n, orders, log, cpu=True, nref=65536):
         U, J = fld.download()
         eu, ej = normwise(U[idx], Uref), normwise(J[idx], Jref)
         dev_s = st["ms_eval"] / 1e3
-        res["orders"][str(p)] = {
+        res["orders"][key] = {
+            "acceptance": "geometric (sigma_cut 0)" if sigcut == 0.0 else "core-aware (sigma_cut 7)",
             "ms_per_uj": 1e3 * wall,
             "direct_equivalent_interactions_per_s": float(n) * n / wall,
             "device_ms": st["ms_eval"], "tree_build_ms": st["ms_tree"],
@@ -332,7 +339,7 @@ def fmm_section(ctx, n, orders, log, cpu=True, nref=65536):
             "m2l_pairs": int(st["m2l_pairs"]), "p2p_pairs": int(st["p2p_pairs"]),
             "p2p_pairs_per_device_s": st["p2p_pairs"] / dev_s if dev_s > 0 else None,
             "rel_err_U": {"max": eu[0], "rms": eu[1]}, "rel_err_J": {"max": ej[0], "rms": ej[1]}}
-        log(f"fmm p={p}: {1e3 * wall:.1f} ms/UJ (device {st['ms_eval']:.1f} ms, tree "
+        log(f"fmm {key}: {1e3 * wall:.1f} ms/UJ (device {st['ms_eval']:.1f} ms, tree "
             f"{st['ms_tree']:.1f}, traverse {st['ms_traverse']:.1f}), err U {eu}, J {ej}")
     if cpu:
         from oracle import oracle as O
@@ -405,7 +412,10 @@ def cpu_baseline(X, G, s, budget_s=15.0, runs=5):
     from oracle import oracle as O
     n = len(X)
     threads = O.num_threads()
-    m = 64
+    # size the sample from a timed run of 16 targets per thread (fewer leave
+    # threads idle and overstate the time per target)
+    m = 16 * max(1, threads)
+    O.uj_direct("gaussianerf", X[:m], X, G, s)   # warm: thread pool, pages
     t0 = time.perf_counter()
     O.uj_direct("gaussianerf", X[:m], X, G, s)
     t = time.perf_counter() - t0
@@ -465,6 +475,15 @@ def full_step_run(n, steps, log, precisions=("mixed", "fp64"), every=10):
             dt = 0.05 * float(np.median(s)) / umax
             res["dt"] = dt
             res["dt_rule"] = "0.05 median(sigma) / max|U| of the initial field"
+            # accuracy of this FMM (geometric acceptance, p = 8) on the blob
+            idx = np.sort(np.random.default_rng(7).choice(n, min(65536, n), replace=False))
+            Ur, Jr = uj_at(ctx, X[idx], N.GAUSSIANERF, N.FP64)
+            eu = normwise(vpm.get_U(pf)[idx], Ur)
+            ej = normwise(vpm.get_J(pf)[idx], Jr)
+            res["fmm_error_vs_fp64_direct"] = {
+                "precision": prec, "targets": len(idx),
+                "rel_err_U": {"max": eu[0], "rms": eu[1]}, "rel_err_J": {"max": ej[0], "rms": ej[1]}}
+            log(f"full step: FMM ({prec}) vs FP64 direct at {len(idx)} targets: U {eu}, J {ej}")
         series = [(0,) + diag()]
         ctx.set_timing(True)
         kinds = (("fmm_device_ms", N.T_FMM), ("dsig_ms", N.T_DSIG), ("estr_ms", N.T_ESTR),

@@ -2379,6 +2379,17 @@ __device__ __forceinline__ int32_t key_lower_bound(const uint64_t *__restrict__ 
     return b;
 }
 
+// The root: every particle, level 0, empty prefix, no parent.
+__global__ void tree_root_kernel(int32_t n, int32_t *tb, int32_t *te, int32_t *lev, uint64_t *pre,
+                                 int32_t *parent)
+{
+    tb[0] = 0;
+    te[0] = n;
+    lev[0] = 0;
+    pre[0] = 0;
+    parent[0] = -1;
+}
+
 // Level cells [lb, lb + m): child count nk[i], split flag, and the ranges and
 // octants of the non-empty children; entries nk[m] = flag[m] = 0 close the scans.
 __global__ __launch_bounds__(FMM_BLOCK) void tree_split_kernel(
@@ -2521,8 +2532,9 @@ __global__ __launch_bounds__(FMM_BLOCK) void tree_lists_kernel(
     if (c > 0) kids[c - 1] = (int32_t)c;
 }
 
+// *p re-allocated for `need` elements, its first `keep` elements copied over.
 template <class T>
-static hipError_t grow_keep(T **p, int64_t need, int64_t cap, int64_t keep, hipStream_t st)
+static hipError_t grow_keep(T **p, int64_t need, int64_t keep, hipStream_t st)
 {
     T *q = nullptr;
     hipError_t e = hipMalloc((void **)&q, std::max<int64_t>(need, 1) * sizeof(T));
@@ -2535,7 +2547,6 @@ static hipError_t grow_keep(T **p, int64_t need, int64_t cap, int64_t keep, hipS
     }
     if (*p) (void)hipFree(*p);
     *p = q;
-    (void)cap;
     return hipSuccess;
 }
 
@@ -2560,26 +2571,26 @@ hipError_t fmm_build_tree_device(hipStream_t st, FmmState &S, int64_t n, int ncr
     auto grow_cells = [&](int64_t need, int64_t keep) -> hipError_t {
         if (need <= B.cap_cell) return hipSuccess;
         const int64_t nc = std::max<int64_t>({need, B.cap_cell + B.cap_cell / 2, 4096});
-        VPMX_CHK(grow_keep(&B.tb, nc, B.cap_cell, keep, st));
-        VPMX_CHK(grow_keep(&B.te, nc, B.cap_cell, keep, st));
-        VPMX_CHK(grow_keep(&B.pre, nc, B.cap_cell, keep, st));
-        VPMX_CHK(grow_keep(&B.lev, nc, B.cap_cell, keep, st));
-        VPMX_CHK(grow_keep(&B.child0, nc, B.cap_cell, keep, st));
-        VPMX_CHK(grow_keep(&B.nchild, nc, B.cap_cell, keep, st));
-        VPMX_CHK(grow_keep(&S.parent, nc, B.cap_cell, keep, st));
+        VPMX_CHK(grow_keep(&B.tb, nc, keep, st));
+        VPMX_CHK(grow_keep(&B.te, nc, keep, st));
+        VPMX_CHK(grow_keep(&B.pre, nc, keep, st));
+        VPMX_CHK(grow_keep(&B.lev, nc, keep, st));
+        VPMX_CHK(grow_keep(&B.child0, nc, keep, st));
+        VPMX_CHK(grow_keep(&B.nchild, nc, keep, st));
+        VPMX_CHK(grow_keep(&S.parent, nc, keep, st));
         B.cap_cell = nc;
         return hipSuccess;
     };
     auto grow_level = [&](int64_t m) -> hipError_t {
         if (m + 1 <= B.cap_lev) return hipSuccess;
         const int64_t c = std::max<int64_t>({m + 1, B.cap_lev + B.cap_lev / 2, 4096});
-        VPMX_CHK(grow_keep(&B.nk, c, 0, 0, st));
-        VPMX_CHK(grow_keep(&B.flag, c, 0, 0, st));
-        VPMX_CHK(grow_keep(&B.off, c, 0, 0, st));
-        VPMX_CHK(grow_keep(&B.foff, c, 0, 0, st));
-        VPMX_CHK(grow_keep(&B.kb, 8 * c, 0, 0, st));
-        VPMX_CHK(grow_keep(&B.ke, 8 * c, 0, 0, st));
-        VPMX_CHK(grow_keep(&B.ko, 8 * c, 0, 0, st));
+        VPMX_CHK(grow_keep(&B.nk, c, 0, st));
+        VPMX_CHK(grow_keep(&B.flag, c, 0, st));
+        VPMX_CHK(grow_keep(&B.off, c, 0, st));
+        VPMX_CHK(grow_keep(&B.foff, c, 0, st));
+        VPMX_CHK(grow_keep(&B.kb, 8 * c, 0, st));
+        VPMX_CHK(grow_keep(&B.ke, 8 * c, 0, st));
+        VPMX_CHK(grow_keep(&B.ko, 8 * c, 0, st));
         B.cap_lev = c;
         return hipSuccess;
     };
@@ -2600,14 +2611,8 @@ hipError_t fmm_build_tree_device(hipStream_t st, FmmState &S, int64_t n, int ncr
     };
 
     VPMX_CHK(grow_cells(1, 0));
-    const int32_t root[2] = {0, (int32_t)n};
-    const int32_t zero = 0, none = -1;
-    const uint64_t zpre = 0;
-    VPMX_CHK(hipMemcpyAsync(B.tb, &root[0], sizeof(int32_t), hipMemcpyHostToDevice, st));
-    VPMX_CHK(hipMemcpyAsync(B.te, &root[1], sizeof(int32_t), hipMemcpyHostToDevice, st));
-    VPMX_CHK(hipMemcpyAsync(B.lev, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st));
-    VPMX_CHK(hipMemcpyAsync(B.pre, &zpre, sizeof(uint64_t), hipMemcpyHostToDevice, st));
-    VPMX_CHK(hipMemcpyAsync(S.parent, &none, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    tree_root_kernel<<<1, 1, 0, st>>>((int32_t)n, B.tb, B.te, B.lev, B.pre, S.parent);
+    VPMX_CHK(hipGetLastError());
     std::vector<std::pair<int64_t, int64_t>> levels;   // [lb, le) per level
     std::vector<int64_t> nsplit;                       // parents per level
     int64_t lb = 0, le = 1;
@@ -2638,20 +2643,20 @@ hipError_t fmm_build_tree_device(hipStream_t st, FmmState &S, int64_t n, int ncr
     // device cell arrays and lists
     if (nc > S.cap_cells) {
         const int64_t cap = std::max<int64_t>(nc, S.cap_cells + S.cap_cells / 2);
-        VPMX_CHK(grow_keep(&S.cells, cap, 0, 0, st));
-        VPMX_CHK(grow_keep(&S.leaves, cap, 0, 0, st));
-        VPMX_CHK(grow_keep(&S.parents, cap, 0, 0, st));
-        VPMX_CHK(grow_keep(&S.kids, cap, 0, 0, st));
-        VPMX_CHK(grow_keep(&S.m2l_off, cap + 1, 0, 0, st));
-        VPMX_CHK(grow_keep(&S.p2p_off, cap + 1, 0, 0, st));
+        VPMX_CHK(grow_keep(&S.cells, cap, 0, st));
+        VPMX_CHK(grow_keep(&S.leaves, cap, 0, st));
+        VPMX_CHK(grow_keep(&S.parents, cap, 0, st));
+        VPMX_CHK(grow_keep(&S.kids, cap, 0, st));
+        VPMX_CHK(grow_keep(&S.m2l_off, cap + 1, 0, st));
+        VPMX_CHK(grow_keep(&S.p2p_off, cap + 1, 0, st));
         S.cap_cells = cap;
     }
     if (nc + 1 > B.cap_flags) {
         const int64_t cap = std::max<int64_t>(nc + 1, B.cap_flags + B.cap_flags / 2);
-        VPMX_CHK(grow_keep(&B.lflag, cap, 0, 0, st));
-        VPMX_CHK(grow_keep(&B.pflag, cap, 0, 0, st));
-        VPMX_CHK(grow_keep(&B.lscan, cap, 0, 0, st));
-        VPMX_CHK(grow_keep(&B.pscan, cap, 0, 0, st));
+        VPMX_CHK(grow_keep(&B.lflag, cap, 0, st));
+        VPMX_CHK(grow_keep(&B.pflag, cap, 0, st));
+        VPMX_CHK(grow_keep(&B.lscan, cap, 0, st));
+        VPMX_CHK(grow_keep(&B.pscan, cap, 0, st));
         B.cap_flags = cap;
     }
     int32_t *pflag = B.pflag;
