@@ -572,11 +572,9 @@ __device__ __forceinline__ void uj_mixed2_sums(const double4 *__restrict__ tgt, 
             const float4 h = shi[jj], l = slo[jj];
             mixed_pair_d<K>((xh - h.x) + (xl - l.x), (yh - h.y) + (yl - l.y),
                             (zh - h.z) + (zl - l.z), h.w, sgm[jj], acc);
+            if ((jj & (MIXED_FOLD - 1)) == MIXED_FOLD - 1) mixed_fold(d, acc);
         }
-        d[0] += acc.U0; d[1] += acc.U1; d[2] += acc.U2;
-        d[3] += acc.W0; d[4] += acc.W1; d[5] += acc.W2;
-        d[6] += acc.J0; d[7] += acc.J1; d[8] += acc.J2; d[9] += acc.J3;
-        d[10] += acc.J4; d[11] += acc.J5; d[12] += acc.J6; d[13] += acc.J7;
+        mixed_fold(d, acc);
     }
 }
 

@@ -254,8 +254,8 @@ __device__ __forceinline__ void uj_f64_block(const double4 *__restrict__ src, in
 // Mixed precision: the 14 FP64 sums of one target block against sources
 // [s0, s1).  Targets are the sorted slots [t0, t1) of `tgt`; sources the
 // sorted slots of (spos, sgam).  Coordinates are taken relative to the block's
-// first target in FP64 and rounded to FP32; each tile's FP32 partials are
-// folded into FP64.  d = (U0..2, W0..2, J0..7) before the -1/(4 pi) factor.
+// first target in FP64 and rounded to FP32; FP32 partials are folded into
+// FP64 every MIXED_FOLD pairs (once per tile for far tiles).  d = (U0..2, W0..2, J0..7) before the -1/(4 pi) factor.
 //
 // With tile bounds `tb` (per BLK-slot tile counted from slot `torg`: (centre,
 // radius), (largest sigma, -, -, -); the block's own targets are tile
@@ -276,6 +276,7 @@ __device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, i
                                               int64_t torg, float4 *tpos, float4 *tgam,
                                               float4 *tlo, double d[14])
 {
+    static_assert(BLK % MIXED_FOLD == 0, "tiles hold whole fold groups");
     const int t = threadIdx.x;
     const int64_t s = t0 + t;
     const bool active = s < t1;
@@ -326,32 +327,41 @@ __device__ __forceinline__ void uj_mixed_sums(const double4 *__restrict__ tgt, i
         if constexpr (DS) {
             if (!far) {
                 if (cnt == BLK) {
+                    for (int jb = 0; jb < BLK; jb += MIXED_FOLD) {
 #pragma unroll 4
-                    for (int jj = 0; jj < BLK; ++jj)
-                        mixed_pair_dsig_hl<K>(tpos[jj], tlo[jj], tgam[jj], xi, yi, zi, xl, yl,
-                                              zl, acc);
+                        for (int jj = jb; jj < jb + MIXED_FOLD; ++jj)
+                            mixed_pair_dsig_hl<K>(tpos[jj], tlo[jj], tgam[jj], xi, yi, zi, xl,
+                                                  yl, zl, acc);
+                        mixed_fold(d, acc);
+                    }
                 } else {
-                    for (int jj = 0; jj < cnt; ++jj)
+                    for (int jj = 0; jj < cnt; ++jj) {
                         mixed_pair_dsig_hl<K>(tpos[jj], tlo[jj], tgam[jj], xi, yi, zi, xl, yl,
                                               zl, acc);
+                        if ((jj & (MIXED_FOLD - 1)) == MIXED_FOLD - 1) mixed_fold(d, acc);
+                    }
                 }
             }
         } else if (far) {
+            // far pairs are at least 7 core sizes (or a gap) away: their terms
+            // stay small and one partial per tile suffices
 #pragma unroll 4
             for (int jj = 0; jj < BLK; ++jj)
                 mixed_pair_far<K>(tpos[jj], tgam[jj], xi, yi, zi, acc);
         } else if (cnt == BLK) {
+            for (int jb = 0; jb < BLK; jb += MIXED_FOLD) {
 #pragma unroll 4
-            for (int jj = 0; jj < BLK; ++jj)
-                mixed_pair_hl<K>(tpos[jj], &tlo[jj], tgam[jj], xi, yi, zi, xl, yl, zl, acc);
+                for (int jj = jb; jj < jb + MIXED_FOLD; ++jj)
+                    mixed_pair_hl<K>(tpos[jj], &tlo[jj], tgam[jj], xi, yi, zi, xl, yl, zl, acc);
+                mixed_fold(d, acc);
+            }
         } else {
-            for (int jj = 0; jj < cnt; ++jj)
+            for (int jj = 0; jj < cnt; ++jj) {
                 mixed_pair_hl<K>(tpos[jj], &tlo[jj], tgam[jj], xi, yi, zi, xl, yl, zl, acc);
+                if ((jj & (MIXED_FOLD - 1)) == MIXED_FOLD - 1) mixed_fold(d, acc);
+            }
         }
-        d[0] += acc.U0; d[1] += acc.U1; d[2] += acc.U2;
-        d[3] += acc.W0; d[4] += acc.W1; d[5] += acc.W2;
-        d[6] += acc.J0; d[7] += acc.J1; d[8] += acc.J2; d[9] += acc.J3;
-        d[10] += acc.J4; d[11] += acc.J5; d[12] += acc.J6; d[13] += acc.J7;
+        mixed_fold(d, acc);
     }
 }
 
