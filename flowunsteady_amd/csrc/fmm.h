@@ -61,6 +61,16 @@ struct FmmCellDev {
     double cx, cy, cz, R, smax;
 };
 
+// Operator table of the matrix-core M2L.  A cell of level l has centre
+// lo + (i + 1/2) ext 2^-l (integer coordinates i, kept per cell as (ix, iy,
+// iz, l)), so a pair's separation is (ext 2^-(lf+1)) v with v an integer
+// vector in half-widths of the finer cell (level lf), and its derivative
+// operator is a scaled copy of E(v).  Rows are kept for |v_i| <= M2L_TAB_V
+// and target-side level gaps dA = lf - l_target <= M2L_TAB_D; other pairs
+// derive E by the recurrence.
+constexpr int M2L_TAB_V = 23, M2L_TAB_D = 2, M2L_TAB_W = 2 * M2L_TAB_V + 1;
+constexpr int64_t M2L_TAB_ROWS = (int64_t)(M2L_TAB_D + 1) * M2L_TAB_W * M2L_TAB_W * M2L_TAB_W;
+
 // Near-field work item: entries [e0, e1) of target leaf c's P2P list.  A leaf
 // whose list is longer than P2P_CHUNK entries is cut into several items (one
 // workgroup each), which write FP64 partial sums from slot `part` (the leaf's
@@ -168,6 +178,9 @@ struct FmmDev {
     const uint16_t *m2l_tile_idx = nullptr;
     const double *m2l_bscale = nullptr, *m2l_lscale = nullptr;
     const int4 *m2l_erec = nullptr;
+    const int4 *ijk = nullptr;        // per cell (ix, iy, iz, level)
+    const double *etab = nullptr;     // M2L_TAB_ROWS x K operator rows
+    double ext = 1.0;                 // edge of the root cell
     // near-field work items (fmm_p2p_chunks)
     const P2PChunk *chunks = nullptr;
     const int64_t *chunk_off = nullptr, *part_base = nullptr;
@@ -202,6 +215,7 @@ struct FmmState {
     // device tree and lists
     int64_t cap_cells = 0, cap_m2l = 0, cap_p2p = 0, cap_coef = 0, cap_n = 0;
     FmmCellDev *cells = nullptr;
+    int4 *ijk = nullptr;
     int32_t *leaves = nullptr, *parents = nullptr, *kids = nullptr, *parent = nullptr;
     int64_t *m2l_off = nullptr, *p2p_off = nullptr;
     int32_t *m2l_src = nullptr, *p2p_src = nullptr;
@@ -211,6 +225,9 @@ struct FmmState {
     double *ssig = nullptr;
     uint8_t *sort_tmp = nullptr;
     size_t sort_bytes = 0;
+    // matrix-core M2L operator rows of order etab_p
+    double *etab = nullptr;
+    int etab_p = -1;
     // statistics of the last evaluation
     double stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     DttBufs dtt;
@@ -255,6 +272,9 @@ hipError_t fmm_traverse_device(hipStream_t st, FmmState &S, int64_t ncell, doubl
 hipError_t fmm_build_tree_device(hipStream_t st, FmmState &S, int64_t n, int ncrit, FmmPlan &pl,
                                  int64_t *nc_out);
 void free_tree_bufs(TreeBufs &B);
+// The operator table of the matrix-core M2L for order P (1..M2L_MFMA_MAX_P),
+// built once per order from the uploaded recurrence records S.m2l_erec.
+hipError_t fmm_m2l_table(hipStream_t st, FmmState &S, int P);
 
 void fmm_build_tree(FmmPlan &pl, const uint64_t *key, const double *ssig, int64_t n, int ncrit);
 void fmm_traverse(FmmPlan &pl, double theta, double sigcut);

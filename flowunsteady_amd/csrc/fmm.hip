@@ -464,14 +464,148 @@ __device__ __forceinline__ void wave_lds_sync()
 
 constexpr int m2l_erow(int P) { return (m2l_ncoef(P) + 1 + 31) / 32 * 32 + 8; }
 
-// M <- m' = (-1)^|a| / a! m_a in place, once the upward pass is done (the
-// multipoles have no reader after the M2L).
+// 1 / ext^n for n = 0 .. M2L_MFMA_MAX_P + 1 (the scales of the M2L's
+// dimensionless expansions), by repeated products.
+__device__ __forceinline__ void m2l_einv(double ext, double *einv)
+{
+    double p = 1.0;
+    einv[0] = 1.0;
+    for (int n = 1; n <= M2L_MFMA_MAX_P + 1; ++n) {
+        p *= ext;
+        einv[n] = 1.0 / p;
+    }
+}
+
+// M <- m'' = (-1)^|a| / a! m_a tau^-|a| in place, tau = ext 2^-(l+1) the half
+// width of the cell (level l), once the upward pass is done (the multipoles
+// have no reader after the M2L).  erec[k].w >> 24 = |k|.
 __global__ __launch_bounds__(FMM_BLOCK) void m2l_prescale_kernel(double *__restrict__ M, int64_t n,
                                                                  int K,
-                                                                 const double *__restrict__ bscale)
+                                                                 const double *__restrict__ bscale,
+                                                                 const int4 *__restrict__ ijk,
+                                                                 const int4 *__restrict__ erec,
+                                                                 double ext)
 {
+    __shared__ double einv[M2L_MFMA_MAX_P + 2];
+    if (threadIdx.x == 0) m2l_einv(ext, einv);
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
-    if (i < n) M[i] *= bscale[i % K];
+    if (i >= n) return;
+    const int k = (int)(i % K), dg = erec[k].w >> 24;
+    const int l = ijk[i / (3 * K)].w;
+    M[i] *= ldexp(bscale[k] * einv[dg], dg * (l + 1));
+}
+
+// One entry of the Taylor-coefficient recurrence from its six predecessors
+// (written with explicit fused operations, so the table and the kernel's own
+// recurrence round alike).
+__device__ __forceinline__ double m2l_taylor(double f1, double f2, double r0, double r1, double r2,
+                                             double a0, double a1, double a2, double b0, double b1,
+                                             double b2)
+{
+    return fma(f1, fma(r2, a2, fma(r1, a1, r0 * a0)), f2 * ((b0 + b1) + b2));
+}
+
+// E_k = k! 2^(dA (|k| + 1)) t_k: the factor from the record word k! | |k| << 24.
+__device__ __forceinline__ double m2l_kscale(int w, int dA)
+{
+    return ldexp((double)(w & 0xffffff), dA * ((w >> 24) + 1));
+}
+
+__host__ __device__ constexpr int64_t m2l_tab_row(int dA, int vx, int vy, int vz)
+{
+    return (((int64_t)dA * M2L_TAB_W + vx + M2L_TAB_V) * M2L_TAB_W + vy + M2L_TAB_V) * M2L_TAB_W +
+           vz + M2L_TAB_V;
+}
+
+// Geometry of an M2L pair from the cells' (ix, iy, iz, level): separation
+// (ext 2^-(lf+1)) r, r integer (exact in FP64), and the pair's table row.
+struct M2LPair {
+    double r0, r1, r2;
+    int dA;
+    bool hit;
+    int64_t row;
+};
+
+__device__ __forceinline__ M2LPair m2l_pair(const int4 a, const int4 s)
+{
+    const int lf = max(a.w, s.w), da = lf - a.w, ds = lf - s.w;
+    const long long vx = ((2ll * a.x + 1) << da) - ((2ll * s.x + 1) << ds);
+    const long long vy = ((2ll * a.y + 1) << da) - ((2ll * s.y + 1) << ds);
+    const long long vz = ((2ll * a.z + 1) << da) - ((2ll * s.z + 1) << ds);
+    M2LPair p;
+    p.r0 = (double)vx;
+    p.r1 = (double)vy;
+    p.r2 = (double)vz;
+    p.dA = da;
+    p.hit = da <= M2L_TAB_D && llabs(vx) <= M2L_TAB_V && llabs(vy) <= M2L_TAB_V &&
+            llabs(vz) <= M2L_TAB_V;
+    p.row = p.hit ? m2l_tab_row(da, (int)vx, (int)vy, (int)vz) : 0;
+    return p;
+}
+
+// Table rows E_k(v) 2^(dA (|k| + 1)), k < K, one thread per row (the zero
+// offset row holds zeros), by the recurrence of m2l_mfma_kernel.
+template <int P>
+__global__ __launch_bounds__(FMM_BLOCK) void m2l_etab_kernel(const int4 *__restrict__ erec,
+                                                             double *__restrict__ tab)
+{
+    constexpr int K = m2l_ncoef(P);
+    const int64_t row = (int64_t)blockIdx.x * FMM_BLOCK + threadIdx.x;
+    if (row >= M2L_TAB_ROWS) return;
+    int64_t q = row;
+    const int vz = (int)(q % M2L_TAB_W) - M2L_TAB_V;
+    q /= M2L_TAB_W;
+    const int vy = (int)(q % M2L_TAB_W) - M2L_TAB_V;
+    q /= M2L_TAB_W;
+    const int vx = (int)(q % M2L_TAB_W) - M2L_TAB_V;
+    const int dA = (int)(q / M2L_TAB_W);
+    double *t = tab + row * K;
+    if (vx == 0 && vy == 0 && vz == 0) {
+        for (int k = 0; k < K; ++k) t[k] = 0.0;
+        return;
+    }
+    const double r0 = vx, r1 = vy, r2 = vz;
+    const double R2 = r0 * r0 + r1 * r1 + r2 * r2, iR2 = 1.0 / R2;
+    t[0] = 1.0 / sqrt(R2);
+    auto ld = [&](unsigned b) { return b == 8u * K ? 0.0 : t[b >> 3]; };
+    for (int n = 1; n <= P; ++n) {
+        const double f1 = -(2.0 * n - 1.0) / n * iR2, f2 = -(n - 1.0) / n * iR2;
+        for (int k = m2l_ncoef(n - 1); k < m2l_ncoef(n); ++k) {
+            const int4 e = erec[k];
+            const unsigned qx = e.x, qy = e.y, qz = e.z;
+            t[k] = m2l_taylor(f1, f2, r0, r1, r2, ld(qx & 0xffffu), ld(qx >> 16), ld(qy & 0xffffu),
+                              ld(qy >> 16), ld(qz & 0xffffu), ld(qz >> 16));
+        }
+    }
+    for (int k = 0; k < K; ++k) t[k] *= m2l_kscale(erec[k].w, dA);
+}
+
+// Degree of entry 4 t + h (h = 0..3) of a column or row tile: base[t] +
+// (h >= step[t]) (a tile spans at most one degree boundary).
+template <int P>
+struct M2LDeg {
+    struct Map {
+        int base[m2l_ntile1(P)], step[m2l_ntile1(P)];
+    };
+    static constexpr Map make()
+    {
+        Map m{};
+        for (int t = 0; t < m2l_ntile1(P); ++t) {
+            m.base[t] = m2l_degree(4 * t);
+            m.step[t] = 4;
+            for (int h = 3; h >= 1; --h)
+                if (4 * t + h < m2l_ncoef(P) && m2l_degree(4 * t + h) > m.base[t]) m.step[t] = h;
+        }
+        return m;
+    }
+    static constexpr Map map = make();
+};
+
+template <int P>
+__device__ __forceinline__ int m2l_tile_degree(int t, int h)
+{
+    return M2LDeg<P>::map.base[t] + (h >= M2LDeg<P>::map.step[t] ? 1 : 0);
 }
 
 // Persistent: block b serves cells k G + (b mod 8) W + b / 8 (k = 0, 1, ...;
@@ -479,7 +613,18 @@ __global__ __launch_bounds__(FMM_BLOCK) void m2l_prescale_kernel(double *__restr
 // XCDs) sweeps contiguous runs of cells whose sources overlap in its L2, and
 // the tables are staged in LDS once per block.  `erec` packs, per k, the
 // byte offsets in a row of k - e_i and k - 2 e_i (8 K where absent: the zero
-// slot) and k!: (j1x | j1y << 16, j1z | j2x << 16, j2y | j2z << 16, k!).
+// slot) and k! | |k| << 24: (j1x | j1y << 16, j1z | j2x << 16, j2y | j2z << 16,
+// k! | |k| << 24).
+//
+// Units: with u = ext 2^-(lf+1) (half the finer cell's width) a pair's
+// separation is u v, v integer, and E_k(u v) = u^-(|k|+1) E_k(v).  The
+// multipoles arrive as m''_a = m'_a tau_S^-|a| (m2l_prescale_kernel) and
+// the kernel accumulates lambda_b = sum_a Ehat_{a+b} m''_a 2^((l_A - l_S)|a|)
+// with Ehat_k = 2^(dA (|k|+1)) E_k(v) (dA = lf - l_A): l'_b =
+// tau_A^-(|b|+1) lambda_b, tau = the half width of each cell.  Ehat is copied
+// from the operator table when all four sources of a group have rows there,
+// and otherwise built by the recurrence from v; the powers of two are exact.
+//
 // The four sources' E are rows of m2l_erow(P) doubles: K + 1 (the zero slot at
 // K), rounded so that the rows start 16 banks apart (the layout measured
 // fastest among the ones tried).  Absent sources and the zero fourth B column
@@ -487,19 +632,24 @@ __global__ __launch_bounds__(FMM_BLOCK) void m2l_prescale_kernel(double *__restr
 template <int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P <= 8 ? 2 : 1)))
 void m2l_mfma_kernel(
-    const FmmCellDev *__restrict__ cells, int ncell, const int64_t *__restrict__ off,
+    const int4 *__restrict__ ijk, int ncell, const int64_t *__restrict__ off,
     const int32_t *__restrict__ src, const int4 *__restrict__ erec,
     const uint16_t *__restrict__ tile_idx, const double *__restrict__ lscale,
-    const double *__restrict__ M, int64_t zrow, double *__restrict__ L)
+    const double *__restrict__ etab, double ext, const double *__restrict__ M, int64_t zrow,
+    double *__restrict__ L)
 {
     constexpr int K = m2l_ncoef(P), NT = m2l_ntile1(P), N8 = m2l_ntiles8(P), ER = m2l_erow(P);
+    constexpr int MAXE = (K + 15) / 16;   // table entries per lane of a source's 16
     __shared__ double E[4 * ER];
     __shared__ __align__(16) uint16_t tab[16 * N8];
     __shared__ int4 rec[K];
+    __shared__ double einv[M2L_MFMA_MAX_P + 2];
+    __shared__ double tsc[M2L_MFMA_MAX_P + 2];   // tau_A^-n of the current cell
     const int lane = threadIdx.x;
     for (int i = lane; i < 16 * N8; i += 64) tab[i] = tile_idx[i];
     for (int i = lane; i < K; i += 64) rec[i] = erec[i];
     if (lane < 4) E[lane * ER + K] = 0.0;
+    if (lane == 0) m2l_einv(ext, einv);
     __syncthreads();
     const int nq = lane & 3, blk = (lane >> 2) & 3, hi = lane >> 4, sub = lane & 15;
     const uint16_t *tl = tab + (nq + 4 * hi) * N8;   // A element (row = lane & 3, k = lane >> 4)
@@ -509,31 +659,51 @@ void m2l_mfma_kernel(
     const int G = gridDim.x, W = G >> 3;
     const int first = (int)(blockIdx.x & 7) * W + (int)(blockIdx.x >> 3);
     for (int c = first; c < ncell; c += G) {
-        const FmmCellDev A = cells[c];
+        const int4 A = ijk[c];
+        wave_lds_sync();   // the previous cell's reads of tsc are done
+        if (lane <= P + 1) tsc[lane] = ldexp(einv[lane], lane * (A.w + 1));
         double Cacc[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) Cacc[t] = 0.0;
         const int64_t e0 = off[c], e1 = off[c + 1];
-        // Source indices run two groups ahead and centres one group ahead, so
-        // no wait for them lands behind the burst of B loads (loads complete
-        // in issue order).  Indices past the list are clamped to its last entry.
+        // Source indices run two groups ahead and cell coordinates one group
+        // ahead, so no wait for them lands behind the burst of B loads (loads
+        // complete in issue order).  Indices past the list are clamped to its
+        // last entry.
         const int64_t elast = e1 - 1;
         int ib = 0, ib1 = 0, ie1 = 0;
-        double sx = 0.0, sy = 0.0, sz = 0.0;
+        int4 S = make_int4(0, 0, 0, 0);
         if (e0 < e1) {
             ib = src[min(e0 + blk, elast)];
             ib1 = src[min(e0 + 4 + blk, elast)];
             ie1 = src[min(e0 + 4 + hi, elast)];
-            const FmmCellDev S = cells[src[min(e0 + hi, elast)]];
-            sx = S.cx; sy = S.cy; sz = S.cz;
+            S = ijk[src[min(e0 + hi, elast)]];
         }
         for (int64_t eb = e0; eb < e1; eb += 4) {
             const int nb = (int)min<int64_t>(4, e1 - eb);
-            const FmmCellDev Sn = cells[ie1];   // next group's centre of source hi
+            const int4 Sn = ijk[ie1];   // next group's coordinates of source hi
             const int ie2 = src[min(eb + 8 + hi, elast)], ib2 = src[min(eb + 8 + blk, elast)];
-            // B operand: m'_blk[4 ta + k][n], k = lane >> 4, n = lane & 3 (M holds
-            // m' here).  Every lane loads from a valid address and multiplies by
-            // a 0/1 mask, so the loads issue back to back instead of one guarded
+            // geometry of source hi (absent sources: v = (2, 0, 0), finite E, zero B)
+            M2LPair g = m2l_pair(A, S);
+            if (hi >= nb) {
+                g.r0 = 2.0;
+                g.r1 = 0.0;
+                g.r2 = 0.0;
+                g.dA = 0;
+                g.hit = true;
+                g.row = m2l_tab_row(0, 2, 0, 0);
+            }
+            const bool tabled = __all(g.hit);
+            double ev[MAXE];
+            if (tabled) {
+                const double *er = etab + g.row * K;
+#pragma unroll
+                for (int it = 0; it < MAXE; ++it) ev[it] = er[min(sub + 16 * it, K - 1)];
+            }
+            const int lsb = ijk[ib].w;   // level of source blk
+            // B operand: m''_blk[4 ta + k][n], k = lane >> 4, n = lane & 3.
+            // Every lane loads from a valid address and multiplies by a 0/1
+            // mask, so the loads issue back to back instead of one guarded
             // load at a time (the multipoles are finite).
             double Bv[NT];
             const double *Mb =
@@ -541,67 +711,60 @@ void m2l_mfma_kernel(
 #pragma unroll
             for (int ta = 0; ta < NT - 1; ++ta) Bv[ta] = Mb[4 * ta + hi];
             Bv[NT - 1] = Mb[min(4 * (NT - 1) + hi, K - 1)] * (4 * (NT - 1) + hi < K ? 1.0 : 0.0);
-            // E of source hi (absent sources: r = (1, 0, 0), finite E, zero B)
-            double r0 = 1.0, r1 = 0.0, r2 = 0.0;
-            if (hi < nb) {
-                r0 = A.cx - sx;
-                r1 = A.cy - sy;
-                r2 = A.cz - sz;
-            }
-            sx = Sn.cx; sy = Sn.cy; sz = Sn.cz;
+            S = Sn;
             ie1 = ie2;
             ib = ib1;
             ib1 = ib2;
-            const double R2 = r0 * r0 + r1 * r1 + r2 * r2, iR2 = 1.0 / R2;
             wave_lds_sync();   // the previous group's reads of E are done
-            if (sub == 0) Ej[0] = 1.0 / sqrt(R2);
-            auto ld = [&](unsigned b) { return *reinterpret_cast<const double *>(Ejb + b); };
-            // each degree's records are read during the previous degree, so
-            // only the E reads sit on the chain from degree to degree
-            constexpr int MAXIT = ((P + 1) * (P + 2) / 2 + 15) / 16;
-            auto recs = [&](int n, uint3 (&q)[MAXIT]) {
+            if (tabled) {
 #pragma unroll
-                for (int it = 0; it < MAXIT; ++it) {
-                    const int4 r = rec[min(m2l_ncoef(n - 1) + sub + 16 * it, K - 1)];
-                    q[it] = make_uint3((unsigned)r.x, (unsigned)r.y, (unsigned)r.z);
+                for (int it = 0; it < MAXE; ++it) {
+                    const int k = sub + 16 * it;
+                    if (k < K) Ej[k] = ev[it];
                 }
-            };
-            uint3 qc[MAXIT];
-            recs(1, qc);
+            } else {
+                const double r0 = g.r0, r1 = g.r1, r2 = g.r2;
+                const double R2 = r0 * r0 + r1 * r1 + r2 * r2, iR2 = 1.0 / R2;
+                if (sub == 0) Ej[0] = 1.0 / sqrt(R2);
+                auto ld = [&](unsigned b) { return *reinterpret_cast<const double *>(Ejb + b); };
+                constexpr int MAXIT = ((P + 1) * (P + 2) / 2 + 15) / 16;
 #pragma unroll
-            for (int n = 1; n <= P; ++n) {
-                wave_lds_sync();
-                uint3 qn[MAXIT];
-                if (n < P) recs(n + 1, qn);
-                const double f1 = -(2.0 * n - 1.0) / n * iR2, f2 = -(n - 1.0) / n * iR2;
+                for (int n = 1; n <= P; ++n) {
+                    wave_lds_sync();
+                    const double f1 = -(2.0 * n - 1.0) / n * iR2, f2 = -(n - 1.0) / n * iR2;
 #pragma unroll
-                for (int it = 0; it < MAXIT; ++it) {
-                    const int k = m2l_ncoef(n - 1) + sub + 16 * it;
-                    if (k < m2l_ncoef(n)) {
-                        const unsigned qx = qc[it].x, qy = qc[it].y, qz = qc[it].z;
-                        const double s1 =
-                            r0 * ld(qx & 0xffffu) + r1 * ld(qx >> 16) + r2 * ld(qy & 0xffffu);
-                        const double s2 = ld(qy >> 16) + ld(qz & 0xffffu) + ld(qz >> 16);
-                        Ej[k] = f1 * s1 + f2 * s2;
+                    for (int it = 0; it < MAXIT; ++it) {
+                        const int k = m2l_ncoef(n - 1) + sub + 16 * it;
+                        if (k < m2l_ncoef(n)) {
+                            const int4 q = rec[k];
+                            const unsigned qx = q.x, qy = q.y, qz = q.z;
+                            Ej[k] = m2l_taylor(f1, f2, r0, r1, r2, ld(qx & 0xffffu), ld(qx >> 16),
+                                               ld(qy & 0xffffu), ld(qy >> 16), ld(qz & 0xffffu),
+                                               ld(qz >> 16));
+                        }
                     }
                 }
-                if (n < P)
-#pragma unroll
-                    for (int it = 0; it < MAXIT; ++it) qc[it] = qn[it];
+                wave_lds_sync();
+                for (int k = sub; k < K; k += 16) Ej[k] *= m2l_kscale(rec[k].w, g.dA);
             }
-            wave_lds_sync();
-            for (int k = 1 + sub; k < K; k += 16) Ej[k] *= (double)(unsigned)rec[k].w;   // E = k! t
+            // level gap of source blk: m''_a 2^((l_A - l_S) |a|), a = 4 ta + hi
+            const int eB = A.w - lsb;
+            if (__any(eB != 0)) {
+#pragma unroll
+                for (int ta = 0; ta < NT; ++ta) Bv[ta] = ldexp(Bv[ta], eB * m2l_tile_degree<P>(ta, hi));
+            }
             wave_lds_sync();
             m2l_mfma_group<P, 0>(Cacc, Bv, tl, Eb);
         }
-        // sum the four blocks (lanes 4 and 8 apart), scale by 1/b!, write l
+        // sum the four blocks (lanes 4 and 8 apart), scale by tau_A^-(|b|+1) / b!,
+        // write l
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             double v = Cacc[t];
             v += __shfl_xor(v, 4);
             v += __shfl_xor(v, 8);
-            const int b = 4 * t + hi;
-            const double ls = lscale[min(b, K - 1)];
+            const int b = 4 * t + hi, dg = m2l_tile_degree<P>(t, hi);
+            const double ls = lscale[min(b, K - 1)] * tsc[dg + 1];
             if (blk == 0 && nq < 3 && b < K) L[((size_t)c * 3 + nq) * K + b] = v * ls;
         }
     }
@@ -1327,13 +1490,15 @@ void fmm_build_tables(FmmTables &T, int P)
                 j[i] = T.rec1[4 * k + i] >= 0 ? T.rec1[4 * k + i] : K;
                 j[3 + i] = T.rec2[4 * k + i] >= 0 ? T.rec2[4 * k + i] : K;
             }
-            // byte offsets (8 j) of the entries the recurrence reads, and k!
-            // (<= 10! for the orders of the matrix-core M2L)
+            // byte offsets (8 j) of the entries the recurrence reads, and
+            // k! | |k| << 24 (k! <= 10! < 2^22 for the orders of the
+            // matrix-core M2L)
             const int *m = &M.mi[3 * k];
             T.m2l_erec[4 * k] = 8 * j[0] | (8 * j[1] << 16);
             T.m2l_erec[4 * k + 1] = 8 * j[2] | (8 * j[3] << 16);
             T.m2l_erec[4 * k + 2] = 8 * j[4] | (8 * j[5] << 16);
-            T.m2l_erec[4 * k + 3] = (int)(fact(m[0]) * fact(m[1]) * fact(m[2]));
+            T.m2l_erec[4 * k + 3] =
+                (int)(fact(m[0]) * fact(m[1]) * fact(m[2])) | ((m[0] + m[1] + m[2]) << 24);
         }
         T.m2l_tile_idx.clear();
         if (P >= 1 && P <= M2L_MFMA_MAX_P) {
@@ -2463,7 +2628,8 @@ __global__ __launch_bounds__(FMM_BLOCK) void tree_cells_kernel(
     const uint64_t *__restrict__ pre, const int32_t *__restrict__ lev,
     const int32_t *__restrict__ child0, const int32_t *__restrict__ nchild,
     const double *__restrict__ ssig, double lo0, double lo1, double lo2, double ext,
-    FmmCellDev *__restrict__ cells, int32_t *__restrict__ lflag, int32_t *__restrict__ pflag)
+    FmmCellDev *__restrict__ cells, int4 *__restrict__ ijk, int32_t *__restrict__ lflag,
+    int32_t *__restrict__ pflag)
 {
     // product and sum rounded separately, as on the host: the acceptance
     // test compares centre distances exactly
@@ -2499,6 +2665,7 @@ __global__ __launch_bounds__(FMM_BLOCK) void tree_cells_kernel(
         for (int32_t j = C.begin; j < C.end; j++) s = fmax(s, ssig[j]);
     C.smax = s;
     cells[c] = C;
+    ijk[c] = make_int4((int)ix, (int)iy, (int)iz, l);
     lflag[c] = C.nchild == 0;
     pflag[c] = C.nchild != 0;
 }
@@ -2560,6 +2727,32 @@ void free_tree_bufs(TreeBufs &B)
     if (B.scan_tmp) (void)hipFree(B.scan_tmp);
     if (B.h_tot) (void)hipHostFree(B.h_tot);
     B = TreeBufs{};
+}
+
+hipError_t fmm_m2l_table(hipStream_t st, FmmState &S, int P)
+{
+    if (P < 1 || P > M2L_MFMA_MAX_P || S.etab_p == P) return hipSuccess;
+    if (!S.m2l_erec) return hipErrorInvalidValue;
+    if (S.etab) {
+        VPMX_CHK(hipStreamSynchronize(st));
+        VPMX_CHK(hipFree(S.etab));
+        S.etab = nullptr;
+        S.etab_p = -1;
+    }
+    const int K = m2l_ncoef(P);
+    VPMX_CHK(hipMalloc((void **)&S.etab, (size_t)M2L_TAB_ROWS * K * sizeof(double)));
+    const unsigned g = gridn(M2L_TAB_ROWS, FMM_BLOCK);
+#define VPMX_ETAB(PP)                                                                  \
+    case PP: m2l_etab_kernel<PP><<<g, FMM_BLOCK, 0, st>>>(S.m2l_erec, S.etab); break;
+    switch (P) {
+        VPMX_ETAB(1) VPMX_ETAB(2) VPMX_ETAB(3) VPMX_ETAB(4) VPMX_ETAB(5)
+        VPMX_ETAB(6) VPMX_ETAB(7) VPMX_ETAB(8) VPMX_ETAB(9) VPMX_ETAB(10)
+    default: return hipErrorInvalidValue;
+    }
+#undef VPMX_ETAB
+    VPMX_CHK(hipGetLastError());
+    S.etab_p = P;
+    return hipSuccess;
 }
 
 hipError_t fmm_build_tree_device(hipStream_t st, FmmState &S, int64_t n, int ncrit,
@@ -2644,6 +2837,7 @@ hipError_t fmm_build_tree_device(hipStream_t st, FmmState &S, int64_t n, int ncr
     if (nc > S.cap_cells) {
         const int64_t cap = std::max<int64_t>(nc, S.cap_cells + S.cap_cells / 2);
         VPMX_CHK(grow_keep(&S.cells, cap, 0, st));
+        VPMX_CHK(grow_keep(&S.ijk, cap, 0, st));
         VPMX_CHK(grow_keep(&S.leaves, cap, 0, st));
         VPMX_CHK(grow_keep(&S.parents, cap, 0, st));
         VPMX_CHK(grow_keep(&S.kids, cap, 0, st));
@@ -2662,7 +2856,7 @@ hipError_t fmm_build_tree_device(hipStream_t st, FmmState &S, int64_t n, int ncr
     int32_t *pflag = B.pflag;
     tree_cells_kernel<<<gridn(nc + 1, FMM_BLOCK), FMM_BLOCK, 0, st>>>(
         nc, B.tb, B.te, B.pre, B.lev, B.child0, B.nchild, S.ssig, pl.lo[0], pl.lo[1], pl.lo[2],
-        pl.ext, S.cells, B.lflag, pflag);
+        pl.ext, S.cells, S.ijk, B.lflag, pflag);
     VPMX_CHK(hipGetLastError());
     for (int l = (int)levels.size() - 2; l >= 0; l--) {
         const int64_t m = levels[l].second - levels[l].first;
@@ -2869,15 +3063,17 @@ hipError_t launch_fmm_eval(hipStream_t st, const FmmDev &d, const FmmPlan &pl, i
     }
     if (d.ncell > 0 && P >= 1 && P <= M2L_MFMA_MAX_P) {
         const int64_t nm = (int64_t)d.ncell * 3 * K;
-        m2l_prescale_kernel<<<gridn(nm, FMM_BLOCK), FMM_BLOCK, 0, st>>>(d.M, nm, K, d.m2l_bscale);
+        if (!d.ijk || !d.etab) return hipErrorInvalidValue;
+        m2l_prescale_kernel<<<gridn(nm, FMM_BLOCK), FMM_BLOCK, 0, st>>>(d.M, nm, K, d.m2l_bscale,
+                                                                       d.ijk, d.m2l_erec, d.ext);
         // the zero multipole row read for absent sources (M holds ncell + 1 rows)
         fmm_zero_kernel<<<gridn(3 * K, FMM_BLOCK), FMM_BLOCK, 0, st>>>(d.M + nm, 3 * K);
 #define VPMX_M2LM(PP)                                                                           \
     case PP: {                                                                                  \
         const int grid = m2l_mfma_grid((const void *)m2l_mfma_kernel<PP>, PP, d.ncell);          \
-        m2l_mfma_kernel<PP><<<grid, 64, 0, st>>>(d.cells, d.ncell, d.m2l_off, d.m2l_src,         \
-                                                 d.m2l_erec, d.m2l_tile_idx, d.m2l_lscale, d.M, \
-                                                 (int64_t)d.ncell, d.L);                        \
+        m2l_mfma_kernel<PP><<<grid, 64, 0, st>>>(d.ijk, d.ncell, d.m2l_off, d.m2l_src,           \
+                                                 d.m2l_erec, d.m2l_tile_idx, d.m2l_lscale,      \
+                                                 d.etab, d.ext, d.M, (int64_t)d.ncell, d.L);    \
         break;                                                                                  \
     }
         switch (P) {

@@ -714,7 +714,7 @@ void free_fmm(FmmState *S)
                     S->d2b, S->m2m_idx, S->l2l_idx, S->m2l_tidx, S->m2m_coef, S->l2l_coef,
                     S->m2l_tcoef, S->m2l_chunks, S->m2l_cfirst, S->m2l_tile_idx, S->m2l_bscale,
                     S->m2l_lscale, S->m2l_erec, S->chunk_buf, S->chunks, S->part, S->chunk_tmp,
-                    S->cells, S->leaves, S->parents, S->kids, S->parent,
+                    S->cells, S->ijk, S->etab, S->leaves, S->parents, S->kids, S->parent,
                     S->m2l_off, S->p2p_off, S->m2l_src, S->p2p_src, S->M, S->L, S->kin, S->kout,
                     S->ssig, S->sort_tmp};
     for (void *p : ptrs)
@@ -800,6 +800,7 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = f
         fmm_build_tables(S.tab, P->p);
         VPMX_TRY(fmm_tables_to_device(c, S));
     }
+    VPMX_HIP(c, fmm_m2l_table(st, S, P->p));
     const int K = S.tab.K;
     if ((nc + 1) * 3 * K > S.cap_coef) {   // + the zero row of the matrix-core M2L
         const int64_t cap = std::max<int64_t>((nc + 1) * 3 * K, S.cap_coef + S.cap_coef / 2);
@@ -872,6 +873,9 @@ int run_fmm(vpmx_ctx *c, const vpmx_uj_params *P, int reset, bool lists_only = f
     d.m2l_bscale = S.m2l_bscale;
     d.m2l_lscale = S.m2l_lscale;
     d.m2l_erec = S.m2l_erec;
+    d.ijk = S.ijk;
+    d.etab = S.etab;
+    d.ext = pl.ext;
     d.chunks = S.chunks;
     d.chunk_off = S.chunk_off;
     d.part_base = S.part_base;

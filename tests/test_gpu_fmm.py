@@ -184,3 +184,26 @@ def test_fmm_long_near_lists():
     mu, mj = normwise(out["mixed"][0], out["fp64"][0]), normwise(out["mixed"][1], out["fp64"][1])
     print("mixed vs fp64", mu, mj)
     assert mu[0] < 5e-6 and mj[0] < 1e-5
+
+
+@pytest.mark.parametrize("p", [4, 8, 10])
+def test_fmm_m2l_operator_table_and_recurrence(p):
+    """A dense blob with a sparse, distant shell under the geometric criterion:
+    the blob's pairs take the matrix-core M2L's operator table, the pairs
+    between blob cells and the coarse shell cells (level gaps and offsets past
+    the table) the in-kernel recurrence; both against the oracle's FMM."""
+    X, G, s, _ = workloads.gaussian_blob(n=20000, seed=6)
+    rng = np.random.default_rng(7)
+    d = rng.standard_normal((600, 3))
+    Xs = 40.0 * d / np.linalg.norm(d, axis=1, keepdims=True)
+    X = np.vstack([X, Xs])
+    G = np.vstack([G, 1e-3 * rng.standard_normal((600, 3))])
+    s = np.concatenate([s, np.full(600, 0.05)])
+    pf = fmm_field(X, G, s, p=p, ncrit=50, theta=0.4, sigma_cut=0.0, nonzero_sigma=False)
+    U, J = run(pf)
+    st = vpm._context(0).fmm_stats()
+    Uo, Jo, so = O.uj_fmm("gaussianerf", X, G, s, p=p, ncrit=50, theta=0.4, sigcut=0.0)
+    assert int(st["m2l_pairs"]) == so["nm2l"]
+    eu, ej = normwise(U, Uo), normwise(J, Jo)
+    print("blob + shell", p, eu, ej, st)
+    assert eu[0] < 1e-12 and ej[0] < 1e-12
