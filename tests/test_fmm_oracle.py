@@ -1,5 +1,9 @@
 """Pins of the oracle's CPU FMM (oracle/fmm_oracle.c) against the direct sum
-and closed forms -- nothing here restates the expansion algebra."""
+and closed forms -- nothing here restates the expansion algebra, except the
+M2L sum written out by its definition to check the matrix-core M2L's unit
+convention (lattice offsets) against direct separations."""
+import math
+
 import numpy as np
 import pytest
 
@@ -84,3 +88,45 @@ def test_sample_mode_matches_full():
     Us, Js, st = O.uj_fmm("gaussianerf", X, G, s, p=6, sample=idx)
     np.testing.assert_array_equal(Us[idx], Uf[idx])
     np.testing.assert_array_equal(Js[idx], Jf[idx])
+
+
+def test_m2l_lattice_offset_scaling():
+    """The unit convention of the matrix-core M2L's operator table: with
+    u = ext 2^-(lf+1) and a pair's separation u v (v integer, lf the finer
+    level), l'_b = tau_A^-(|b|+1) sum_a [2^(dA(|a+b|+1)) E_{a+b}(v)]
+    [m'_a tau_S^-|a| 2^((l_A - l_S)|a|)], tau = a cell's half width and
+    dA = lf - l_A, equals the direct l'_b = sum_a E_{a+b}(c_A - c_S) m'_a."""
+    P = 6
+    rng = np.random.default_rng(3)
+    ext, lo = 1.7, np.array([-0.3, 0.2, 0.1])
+    _, mi = O.fmm_deriv(P, np.array([1.0, 0.0, 0.0]))
+    mi = [tuple(m) for m in mi]
+    idx = {m: i for i, m in enumerate(mi)}
+    deg = np.array([sum(m) for m in mi])
+    fact = np.array([float(math.prod(math.factorial(x) for x in m)) for m in mi])
+
+    def E(r):
+        b, _ = O.fmm_deriv(P, np.asarray(r, float))
+        return b * fact
+
+    def m2l(Ek, mp):
+        out = np.zeros(len(mi))
+        for j, b in enumerate(mi):
+            for i, a in enumerate(mi):
+                if sum(a) + sum(b) <= P:
+                    out[j] += Ek[idx[(a[0] + b[0], a[1] + b[1], a[2] + b[2])]] * mp[i]
+        return out
+
+    for lA, lS in ((5, 5), (6, 5), (5, 7), (4, 6)):
+        iA, iS = rng.integers(0, 2 ** lA, 3), rng.integers(0, 2 ** lS, 3)
+        hA, hS = ext / 2 ** lA, ext / 2 ** lS
+        cA, cS = lo + (iA + 0.5) * hA, lo + (iS + 0.5) * hS
+        mp = rng.standard_normal(len(mi))
+        ref = m2l(E(cA - cS), mp)
+        lf = max(lA, lS)
+        dA, dS = lf - lA, lf - lS
+        v = (2 * iA + 1) * 2 ** dA - (2 * iS + 1) * 2 ** dS
+        Ehat = E(v) * 2.0 ** (dA * (deg + 1))
+        mpp = mp * (hS / 2) ** (-deg.astype(float)) * 2.0 ** ((lA - lS) * deg)
+        out = m2l(Ehat, mpp) * (hA / 2) ** (-(deg + 1).astype(float))
+        assert np.abs(out - ref).max() < 1e-13 * np.abs(ref).max(), (lA, lS)
